@@ -1,0 +1,166 @@
+"""OpenSSL libcrypto (system library, called through ctypes) as an independent cross-check of the
+oracle: ECDSA signing and EC point multiplication on secp256k1 / P-256, and Ed25519 keys and
+signatures. Used only by the CPU tests."""
+import ctypes
+import ctypes.util
+
+NID = {0: 714, 1: 415}  # NID_secp256k1, NID_X9_62_prime256v1
+EVP_PKEY_ED25519 = 1087
+
+_c = None
+
+
+def available():
+    try:
+        crypto()
+        return True
+    except OSError:
+        return False
+
+
+def crypto():
+    global _c
+    if _c is None:
+        name = ctypes.util.find_library("crypto") or "libcrypto.so.3"
+        C = ctypes.CDLL(name)
+        vp = ctypes.c_void_p
+        for fn, res, args in [
+            ("EC_KEY_new_by_curve_name", vp, [ctypes.c_int]),
+            ("EC_KEY_free", None, [vp]),
+            ("EC_KEY_get0_group", vp, [vp]),
+            ("EC_KEY_set_private_key", ctypes.c_int, [vp, vp]),
+            ("EC_KEY_set_public_key", ctypes.c_int, [vp, vp]),
+            ("EC_POINT_new", vp, [vp]),
+            ("EC_POINT_free", None, [vp]),
+            ("EC_POINT_mul", ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+            ("EC_POINT_oct2point", ctypes.c_int, [vp, vp, ctypes.c_char_p, ctypes.c_size_t, vp]),
+            ("EC_POINT_point2oct", ctypes.c_size_t, [vp, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, vp]),
+            ("EC_POINT_is_at_infinity", ctypes.c_int, [vp, vp]),
+            ("BN_bin2bn", vp, [ctypes.c_char_p, ctypes.c_int, vp]),
+            ("BN_bn2binpad", ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
+            ("BN_free", None, [vp]),
+            ("ECDSA_do_sign", vp, [ctypes.c_char_p, ctypes.c_int, vp]),
+            ("ECDSA_SIG_get0", None, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+            ("ECDSA_SIG_free", None, [vp]),
+            ("EVP_PKEY_new_raw_private_key", vp, [ctypes.c_int, vp, ctypes.c_char_p, ctypes.c_size_t]),
+            ("EVP_PKEY_new_raw_public_key", vp, [ctypes.c_int, vp, ctypes.c_char_p, ctypes.c_size_t]),
+            ("EVP_PKEY_get_raw_public_key", ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]),
+            ("EVP_PKEY_free", None, [vp]),
+            ("EVP_MD_CTX_new", vp, []),
+            ("EVP_MD_CTX_free", None, [vp]),
+            ("EVP_DigestSignInit", ctypes.c_int, [vp, vp, vp, vp, vp]),
+            ("EVP_DigestSign", ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p,
+                                              ctypes.c_size_t]),
+            ("EVP_DigestVerifyInit", ctypes.c_int, [vp, vp, vp, vp, vp]),
+            ("EVP_DigestVerify", ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p,
+                                                ctypes.c_size_t]),
+        ]:
+            f = getattr(C, fn)
+            f.restype = res
+            f.argtypes = args
+        _c = C
+    return _c
+
+
+def _bn(x: int):
+    b = x.to_bytes(32, "big")
+    return crypto().BN_bin2bn(b, 32, None)
+
+
+def pubkey(curve: int, sk: int) -> bytes:
+    C = crypto()
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    pt = C.EC_POINT_new(grp)
+    k = _bn(sk)
+    assert C.EC_POINT_mul(grp, pt, k, None, None, None) == 1
+    buf = ctypes.create_string_buffer(65)
+    assert C.EC_POINT_point2oct(grp, pt, 4, buf, 65, None) == 65
+    C.BN_free(k)
+    C.EC_POINT_free(pt)
+    C.EC_KEY_free(key)
+    return buf.raw[1:]
+
+
+def point_mul(curve: int, P: bytes, k: int):
+    """k*P for P = X||Y (64 bytes); returns None for infinity."""
+    C = crypto()
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    pt = C.EC_POINT_new(grp)
+    res = C.EC_POINT_new(grp)
+    assert C.EC_POINT_oct2point(grp, pt, b"\x04" + P, 65, None) == 1
+    kb = _bn(k)
+    assert C.EC_POINT_mul(grp, res, None, pt, kb, None) == 1
+    out = None
+    if not C.EC_POINT_is_at_infinity(grp, res):
+        buf = ctypes.create_string_buffer(65)
+        assert C.EC_POINT_point2oct(grp, res, 4, buf, 65, None) == 65
+        out = buf.raw[1:]
+    C.BN_free(kb)
+    C.EC_POINT_free(pt)
+    C.EC_POINT_free(res)
+    C.EC_KEY_free(key)
+    return out
+
+
+def ecdsa_sign(curve: int, sk: int, digest: bytes):
+    """OpenSSL ECDSA signature (random nonce) -> (r, s) integers."""
+    C = crypto()
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    k = _bn(sk)
+    assert C.EC_KEY_set_private_key(key, k) == 1
+    pt = C.EC_POINT_new(grp)
+    assert C.EC_POINT_mul(grp, pt, k, None, None, None) == 1
+    assert C.EC_KEY_set_public_key(key, pt) == 1
+    sig = C.ECDSA_do_sign(digest, len(digest), key)
+    assert sig
+    r = ctypes.c_void_p()
+    s = ctypes.c_void_p()
+    C.ECDSA_SIG_get0(sig, ctypes.byref(r), ctypes.byref(s))
+    rb = ctypes.create_string_buffer(32)
+    sb = ctypes.create_string_buffer(32)
+    C.BN_bn2binpad(r, rb, 32)
+    C.BN_bn2binpad(s, sb, 32)
+    C.ECDSA_SIG_free(sig)
+    C.EC_POINT_free(pt)
+    C.BN_free(k)
+    C.EC_KEY_free(key)
+    return int.from_bytes(rb.raw, "big"), int.from_bytes(sb.raw, "big")
+
+
+def ed25519_pubkey(seed: bytes) -> bytes:
+    C = crypto()
+    pk = C.EVP_PKEY_new_raw_private_key(EVP_PKEY_ED25519, None, seed, 32)
+    out = ctypes.create_string_buffer(32)
+    ln = ctypes.c_size_t(32)
+    assert C.EVP_PKEY_get_raw_public_key(pk, out, ctypes.byref(ln)) == 1
+    C.EVP_PKEY_free(pk)
+    return out.raw
+
+
+def ed25519_sign(seed: bytes, msg: bytes) -> bytes:
+    C = crypto()
+    pk = C.EVP_PKEY_new_raw_private_key(EVP_PKEY_ED25519, None, seed, 32)
+    ctx = C.EVP_MD_CTX_new()
+    assert C.EVP_DigestSignInit(ctx, None, None, None, pk) == 1
+    out = ctypes.create_string_buffer(64)
+    ln = ctypes.c_size_t(64)
+    assert C.EVP_DigestSign(ctx, out, ctypes.byref(ln), msg, len(msg)) == 1
+    C.EVP_MD_CTX_free(ctx)
+    C.EVP_PKEY_free(pk)
+    return out.raw
+
+
+def ed25519_verify(sig: bytes, pub: bytes, msg: bytes) -> bool:
+    C = crypto()
+    pk = C.EVP_PKEY_new_raw_public_key(EVP_PKEY_ED25519, None, pub, 32)
+    if not pk:
+        return False
+    ctx = C.EVP_MD_CTX_new()
+    assert C.EVP_DigestVerifyInit(ctx, None, None, None, pk) == 1
+    rc = C.EVP_DigestVerify(ctx, sig, 64, msg, len(msg))
+    C.EVP_MD_CTX_free(ctx)
+    C.EVP_PKEY_free(pk)
+    return rc == 1

@@ -1,0 +1,227 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+Each part of the oracle is checked against something independent of its own code:
+  - SHA-512 / SHA-256: hashlib on every padding boundary;
+  - field arithmetic for all six moduli: Python integers (pow, modular inverse);
+  - mod-l wide reduction (reference src/tests/ed25519_reduce_fr.rs:30 checks x % l): Python ints;
+  - group law / scalar multiplication: pure-Python affine arithmetic (tests/pyref.py) and OpenSSL;
+  - secp256k1 recovery: the reference's known-answer vector (src/curve_algos/secp256k1_ecdsa.rs:264-322),
+    OpenSSL-made signatures, the round trip recover(sign(sk)) == sk*G (src/tests/secp256k1_ecdsa.rs:94-100),
+    and 158 hand-built edge cases per curve against pyref;
+  - Ed25519: RFC 8032 section 7.1 vectors, OpenSSL signatures, 90 edge cases against pyref.
+"""
+import hashlib
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import edge_cases
+import oracle_lib as O
+import openssl_lib as S
+import pyref
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODULI = [
+    pyref.CURVES[0]["p"], pyref.CURVES[0]["n"], pyref.CURVES[1]["p"], pyref.CURVES[1]["n"], pyref.EP, pyref.EL,
+]
+
+
+def test_sha2_matches_hashlib():
+    rnd = random.Random(1)
+    for n in list(range(0, 300)) + [1000, 4095, 4096, 4097, 10000]:
+        d = bytes(rnd.randrange(256) for _ in range(n))
+        assert O.sha512(d) == hashlib.sha512(d).digest(), n
+        assert O.sha256(d) == hashlib.sha256(d).digest(), n
+
+
+@pytest.mark.parametrize("mod", range(6))
+def test_field_ops_closed_form(mod):
+    m = MODULI[mod]
+    rnd = random.Random(mod)
+    vals = [0, 1, 2, m - 1, m - 2, (m + 1) // 2] + [rnd.randrange(m) for _ in range(40)]
+    for a in vals:
+        b = rnd.randrange(m)
+        assert O.mod_op("mul", mod, a, b)[1] == a * b % m
+        e = rnd.randrange(2**256)
+        assert O.mod_op("pow", mod, a, e)[1] == pow(a, e, m)
+        if a:
+            inv = O.mod_op("inv", mod, a)[1]
+            assert inv == pow(a, -1, m) and inv * a % m == 1
+        else:
+            assert O.mod_op("inv", mod, a)[1] == 0
+        if mod in (0, 2):  # p = 3 mod 4
+            ok, r = O.mod_op("sqrt", mod, a)
+            is_sq = a == 0 or pow(a, (m - 1) // 2, m) == 1
+            assert bool(ok) == is_sq
+            if is_sq:
+                assert r * r % m == a
+
+
+def test_reduce512_mod_l():
+    rnd = random.Random(5)
+    for x in [0, 1, pyref.EL, pyref.EL - 1, 2**512 - 1] + [rnd.randrange(2**512) for _ in range(200)]:
+        got = int.from_bytes(O.reduce512_l(x.to_bytes(64, "little")), "little")
+        assert got == x % pyref.EL
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_curve_constants_and_group_order(curve):
+    c = pyref.CURVES[curve]
+    Gx, Gy = c["G"]
+    p, n = c["p"], c["n"]
+    assert (Gy * Gy - Gx**3 - c["a"] * Gx - c["b"]) % p == 0
+    G = Gx.to_bytes(32, "big") + Gy.to_bytes(32, "big")
+    assert O.secp_mul(curve, G, n)[0] == 1  # n*G = infinity
+    rc, negG = O.secp_mul(curve, G, n - 1)
+    assert rc == 0 and negG == Gx.to_bytes(32, "big") + (p - Gy).to_bytes(32, "big")
+    assert O.secp_pubkey(curve, (1).to_bytes(32, "big")) == G
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_scalar_mul_vs_affine_and_openssl(curve):
+    c = pyref.CURVES[curve]
+    rnd = random.Random(10 + curve)
+    n = c["n"]
+    G = c["G"][0].to_bytes(32, "big") + c["G"][1].to_bytes(32, "big")
+    for _ in range(6):
+        k = rnd.randrange(1, n)
+        P = O.secp_pubkey(curve, k.to_bytes(32, "big"))
+        assert P == S.pubkey(curve, k)
+        Pa = pyref.w_mul(c, c["G"], k)
+        assert P == Pa[0].to_bytes(32, "big") + Pa[1].to_bytes(32, "big")
+        u1, u2 = rnd.randrange(n), rnd.randrange(n)
+        rc, Q = O.secp_mul2(curve, P, u1, u2)
+        Qa = pyref.w_add(c, pyref.w_mul(c, c["G"], u1), pyref.w_mul(c, Pa, u2))
+        assert Q == Qa[0].to_bytes(32, "big") + Qa[1].to_bytes(32, "big")
+        j = rnd.randrange(n)
+        assert O.secp_mul(curve, P, j)[1] == S.point_mul(curve, P, j)
+    # mul2 through the exceptional cases: P = G, P = -G
+    negG = O.secp_mul(curve, G, n - 1)[1]
+    assert O.secp_mul2(curve, G, 3, n - 3)[0] == 1
+    assert O.secp_mul2(curve, G, 3, 3)[1] == O.secp_mul(curve, G, 6)[1]
+    assert O.secp_mul2(curve, negG, 5, 5)[0] == 1
+
+
+def test_secp256k1_reference_kat():
+    kat = json.load(open(os.path.join(HERE, "golden", "secp256k1_kat.json")))
+    digest = hashlib.sha256(kat["message_utf8"].encode()).digest()
+    assert digest.hex() == kat["digest"]
+    sig = bytes.fromhex(kat["signature"])
+    assert kat["recid"] == 0
+    st, pub = O.secp_recover(0, sig, digest)
+    assert st == 0 and pub.hex() == kat["pubkey"]
+    assert pyref.secp_recover(0, sig, digest) == (0, pub)
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_recover_openssl_signatures(curve):
+    """OpenSSL-made signatures: exactly one parity recovers OpenSSL's public key."""
+    n = pyref.CURVES[curve]["n"]
+    rnd = random.Random(20 + curve)
+    for _ in range(12):
+        sk = rnd.randrange(1, n)
+        d = bytes(rnd.randrange(256) for _ in range(32))
+        r, s = S.ecdsa_sign(curve, sk, d)
+        if s > n // 2:
+            s = n - s
+        pub = S.pubkey(curve, sk)
+        hits = 0
+        for odd in (0, 1):
+            sig = bytearray(r.to_bytes(32, "big") + s.to_bytes(32, "big"))
+            sig[32] |= odd << 7
+            st, q = O.secp_recover(curve, bytes(sig), d)
+            assert st == 0
+            assert (st, q) == pyref.secp_recover(curve, bytes(sig), d)
+            hits += q == pub
+        assert hits == 1
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_recover_edge_cases_vs_closed_form(curve):
+    cases = edge_cases.secp_cases(curve)
+    exp = edge_cases.secp_expected(curve, cases)
+    statuses = set()
+    for (name, sig, dig), (st, pub) in zip(cases, exp):
+        assert O.secp_recover(curve, sig, dig) == (st, pub), name
+        statuses.add(st)
+    assert statuses == {0, 1, 2, 3, 4}
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_generator_round_trip_and_determinism(curve):
+    sigs, dig, pubs, cor = O.gen_secp(curve, 1234, 64, corrupt_ppm=250000, nthreads=3)
+    sigs1, dig1, pubs1, cor1 = O.gen_secp(curve, 1234, 64, corrupt_ppm=250000, nthreads=1)
+    assert (sigs == sigs1).all() and (dig == dig1).all() and (pubs == pubs1).all() and (cor == cor1).all()
+    assert 5 < cor.sum() < 30
+    out, st = O.recover_batch(curve, sigs, dig, nthreads=4)
+    for i in range(64):
+        ref = pyref.secp_recover(curve, bytes(sigs[i]), bytes(dig[i]))
+        assert (int(st[i]), bytes(out[i])) == ref
+        if not cor[i]:
+            assert st[i] == 0 and bytes(out[i]) == bytes(pubs[i])
+            sk_pub = bytes(pubs[i])
+            assert sk_pub  # pubs come from sk*G in the generator
+        else:
+            assert bytes(out[i]) != bytes(pubs[i])
+    # low-s: s < n/2 and fuel's parity bit
+    n = pyref.CURVES[curve]["n"]
+    for i in np.nonzero(cor == 0)[0]:
+        s = int.from_bytes(bytes(sigs[i, 32:]), "big") & ((1 << 255) - 1)
+        assert s <= n // 2
+
+
+def test_ed25519_rfc8032_vectors():
+    for t in json.load(open(os.path.join(HERE, "golden", "rfc8032_ed25519.json")))["vectors"]:
+        sk, pk, m, sig = (bytes.fromhex(t[k]) for k in ("secret", "public", "message", "signature"))
+        assert O.ed25519_pubkey(sk) == pk
+        assert O.ed25519_sign(sk, m) == sig
+        assert O.ed25519_verify(sig, pk, m)
+        assert pyref.ed25519_verify(sig, pk, m)
+        bad = bytearray(sig)
+        bad[5] ^= 1
+        assert not O.ed25519_verify(bytes(bad), pk, m)
+
+
+def test_ed25519_vs_openssl():
+    rnd = random.Random(3)
+    for L in (0, 1, 31, 32, 100, 127, 128, 500):
+        seed = bytes(rnd.randrange(256) for _ in range(32))
+        m = bytes(rnd.randrange(256) for _ in range(L))
+        sig = O.ed25519_sign(seed, m)
+        assert sig == S.ed25519_sign(seed, m)
+        pk = O.ed25519_pubkey(seed)
+        assert pk == S.ed25519_pubkey(seed)
+        assert O.ed25519_verify(sig, pk, m) and S.ed25519_verify(sig, pk, m)
+        for field in range(3):
+            b = bytearray(sig)
+            mm = bytearray(m)
+            if field == 2 and L:
+                mm[rnd.randrange(L)] ^= 1 << rnd.randrange(8)
+            else:
+                b[rnd.randrange(32) + 32 * (field % 2)] ^= 1 << rnd.randrange(8)
+            assert O.ed25519_verify(bytes(b), pk, bytes(mm)) == S.ed25519_verify(bytes(b), pk, bytes(mm)) == False  # noqa
+
+
+def test_ed25519_edge_cases_vs_closed_form():
+    cases = edge_cases.ed_cases()
+    exp = edge_cases.ed_expected(cases)
+    for (name, sig, pk, m), e in zip(cases, exp):
+        assert O.ed25519_verify(sig, pk, m) == e, name
+    assert 10 < sum(exp) < len(exp)  # both outcomes are exercised
+
+
+def test_ed25519_generator():
+    rnd = np.random.default_rng(0)
+    lens = rnd.integers(0, 300, 40)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    sigs, pks, msgs, cor = O.gen_ed25519(99, 40, off, corrupt_ppm=300000, nthreads=2)
+    sigs1, pks1, msgs1, cor1 = O.gen_ed25519(99, 40, off, corrupt_ppm=300000, nthreads=1)
+    assert (sigs == sigs1).all() and (msgs == msgs1).all() and (cor == cor1).all()
+    valid = O.verify_batch(sigs, pks, msgs, off, nthreads=3)
+    for i in range(40):
+        m = bytes(msgs[off[i]:off[i + 1]])
+        assert bool(valid[i]) == pyref.ed25519_verify(bytes(sigs[i]), bytes(pks[i]), m)
+        assert bool(valid[i]) == (not cor[i])
