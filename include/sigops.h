@@ -1,0 +1,106 @@
+/*
+ * sigops.h -- C ABI of the MI355X (gfx950) batch signature engine (libsigops.so).
+ *
+ * This boundary replaces the reference's wgpu plumbing (src/gpu.rs:5-186) and WGSL assembly
+ * (src/shader.rs:20-647) behind its public entry points:
+ *   secp256k1_ecdsa::ecrecover / ecrecover_single_shader   src/secp256k1_ecdsa.rs:61-66, :215-219
+ *   secp256r1_ecdsa::ecrecover / ecrecover_single_shader   src/secp256r1_ecdsa.rs:62-67, :216-220
+ *   ed25519_eddsa::ecverify / ecverify_single               src/ed25519_eddsa.rs:67-73, :259-264
+ * Plain pointers and sizes only. The caller owns every host buffer; the library owns device memory.
+ * One context may be used by one host thread at a time. No function throws or unwinds.
+ *
+ * Return values: 0 on success, a negative SIGOPS_E_* code on failure (the reference maps its only
+ * failure, a missing `success` canary, to ShaderFailureError: src/lib.rs:12-14). Per-item results
+ * never fail the call; invalid signatures are reported per item.
+ */
+#ifndef SIGOPS_H
+#define SIGOPS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* call status */
+#define SIGOPS_OK 0
+#define SIGOPS_E_ARG (-1)     /* bad argument (NULL pointer with n > 0, n too large, bad scheme) */
+#define SIGOPS_E_DEVICE (-2)  /* HIP runtime error; sigops_last_error() has the text */
+#define SIGOPS_E_NOMEM (-3)   /* device or pinned-host allocation failed */
+#define SIGOPS_E_CANARY (-4)  /* a kernel did not write every result (ShaderFailureError) */
+
+/* per-item status for secp recovery (same codes as the oracle). On any non-zero status the 64
+   output bytes are zero, which is what a drop-in wrapper returns (the reference has no per-item
+   error channel). */
+#define SIGOPS_ST_OK 0
+#define SIGOPS_ST_BAD_R 1     /* r == 0 or r >= n */
+#define SIGOPS_ST_BAD_S 2     /* s == 0 or s >= n after clearing the recovery bit */
+#define SIGOPS_ST_NO_POINT 3  /* r is not the x-coordinate of a curve point */
+#define SIGOPS_ST_INFINITY 4  /* the recovered key is the point at infinity */
+
+/* per-item status for Ed25519 verification (valid[i] = status == 0) */
+#define SIGOPS_ED_OK 0
+#define SIGOPS_ED_BAD_S 1     /* s >= l (non-canonical scalar) */
+#define SIGOPS_ED_BAD_KEY 2   /* public key does not decompress */
+#define SIGOPS_ED_MISMATCH 3  /* [s]B - [k]A does not compress to R */
+
+/* schemes of the mixed batch */
+#define SIGOPS_SCHEME_SECP256K1 0
+#define SIGOPS_SCHEME_SECP256R1 1
+#define SIGOPS_SCHEME_ED25519 2
+
+typedef struct sigops_ctx sigops_ctx;
+
+typedef struct {
+  int device;               /* HIP device ordinal */
+  uint32_t blocks_per_cu;   /* resident 256-thread blocks per CU for the persistent kernels (0: default) */
+  uint32_t max_batch;       /* largest n accepted per call (0: 1 << 22, the reference's limit,
+                               src/benchmarks/mod.rs:45) */
+} sigops_config;
+
+/* Creates a context on one GPU: uploads the fixed-base generator tables to HBM once (the
+   reference re-uploads them on every call, src/secp256k1_ecdsa.rs:109) and allocates per-lane
+   scratch. cfg may be NULL. Returns NULL on failure. */
+sigops_ctx* sigops_create(const sigops_config* cfg);
+void sigops_destroy(sigops_ctx* ctx);
+/* text of the last error on this context ("" if none); valid until the next call */
+const char* sigops_last_error(const sigops_ctx* ctx);
+
+/* ---- plain per-scheme entry points: host buffers in and out, transfers included ---- */
+
+/* secp256k1 public-key recovery, fuel-crypto format.
+   sigs: n x 64 bytes r||s, big-endian, recovery parity in bit 7 of byte 32;
+   digests: n x 32 bytes (message digests, taken mod n);
+   out_pub: n x 64 bytes X||Y big-endian (zeros for a failed item); status: n bytes or NULL.
+   Replaces secp256k1_ecdsa::ecrecover / ecrecover_single_shader (src/secp256k1_ecdsa.rs:61-274). */
+int sigops_secp256k1_recover(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                             uint8_t* out_pub, uint8_t* status);
+
+/* secp256r1 (P-256) recovery with the same layout.
+   Replaces secp256r1_ecdsa::ecrecover / ecrecover_single_shader (src/secp256r1_ecdsa.rs:62-275). */
+int sigops_secp256r1_recover(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                             uint8_t* out_pub, uint8_t* status);
+
+/* Ed25519 verification with ed25519-dalek `verify` rules.
+   sigs: n x 64 bytes R||s; pubkeys: n x 32 bytes; message i is msgs[msg_off[i] .. msg_off[i+1]);
+   valid: n bytes (1 valid, 0 invalid); status: n bytes (SIGOPS_ED_*) or NULL.
+   Replaces ed25519_eddsa::ecverify / ecverify_single (src/ed25519_eddsa.rs:67-324), whose
+   messages are fixed 32-byte digests (a special case: msg_off[i] = 32 i). */
+int sigops_ed25519_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status);
+
+/* ---- device-resident variants: pointers are device memory, work is enqueued on `stream`
+   (a hipStream_t, NULL for the default stream) and the call returns without synchronising.
+   The context's scratch is shared, so calls on one context must not overlap. ---- */
+int sigops_secp256k1_recover_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_digests, size_t n,
+                                 void* d_out_pub, void* d_status, void* stream);
+int sigops_secp256r1_recover_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_digests, size_t n,
+                                 void* d_out_pub, void* d_status, void* stream);
+/* d_msgs must stay readable for 16 bytes past its last message byte */
+int sigops_ed25519_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
+                              const uint64_t* d_msg_off, size_t n, void* d_status, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+// devcheck.cpp -- host (g++) build of the device headers in wgpu-sigops_amd/csrc, exposing the
+// per-item algorithms through C functions so that CPU tests can compare them with the oracle
+// before anything runs on a GPU. Test infrastructure only; the product runs the HIP kernels.
+#include <cstring>
+
+#include "ed25519.hpp"
+#include "secp.hpp"
+
+using namespace sigops;
+
+namespace {
+struct GTabHost {
+  const u32* base;
+  void load(u32 idx, fe& x, fe& y) const {
+    const u32* e = base + idx * 20;
+    for (int k = 0; k < NL; k++) {
+      x.v[k] = e[k];
+      y.v[k] = e[9 + k];
+    }
+  }
+};
+struct BTabHost {
+  void load(u32 idx, fe& ypx, fe& ymx, fe& t2d) const {
+    const u32* e = ED_BTABLE + idx * 28;
+    for (int k = 0; k < NL; k++) {
+      ypx.v[k] = e[k];
+      ymx.v[k] = e[9 + k];
+      t2d.v[k] = e[18 + k];
+    }
+  }
+};
+struct JTabHost {
+  jac t[8];
+  void store(int e, const jac& p) { t[e] = p; }
+  jac load(u32 e) const { return t[e]; }
+};
+struct CTabHost {
+  cached t[8];
+  void store(int e, const cached& p) { t[e] = p; }
+  cached load(u32 e) const { return t[e]; }
+};
+struct MemHost {
+  const u32* w;
+  u32 word(u64 i) const { return w[i]; }
+};
+
+template <class C>
+int recover_one(const unsigned char* sig, const unsigned char* dig, unsigned char* out, const u32* gtab,
+                int force_careful) {
+  u32 sr[16], dr[8], o[16];
+  memcpy(sr, sig, 64);
+  memcpy(dr, dig, 32);
+  secp_pro pro;
+  secp_prologue<C>(sr, dr, pro);
+  GTabHost gt{gtab};
+  memset(out, 0, 64);
+  if (pro.status != ST_OK) return (int)pro.status;
+  bool exc = false;
+  JTabHost tab;
+  jac Q = secp_ladder<C>(pro, tab, gt, exc);
+  int careful = exc || force_careful;
+  if (!careful) {
+    secp_output<C>(Q, o);
+    memcpy(out, o, 64);
+    return 0;
+  }
+  jacf Qc = secp_ladder_careful<C>(pro, gt);
+  if (Qc.inf) return (int)ST_INFINITY;
+  secp_output<C>(Qc.P, o);
+  memcpy(out, o, 64);
+  return exc ? 0x100 : 0;  // flag that the careful path ran (status OK)
+}
+}  // namespace
+
+extern "C" {
+// returns status; bit 8 set when the fast ladder met an exceptional addition
+int dc_secp_recover(int curve, const unsigned char* sig, const unsigned char* dig, unsigned char* out,
+                    int force_careful) {
+  if (curve == 0) return recover_one<CurveK1>(sig, dig, out, K1_GTABLE, force_careful);
+  return recover_one<CurveR1>(sig, dig, out, R1_GTABLE, force_careful);
+}
+
+int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const unsigned char* msg, unsigned len) {
+  u32 sr[16], pr[8];
+  memcpy(sr, sig, 64);
+  memcpy(pr, pk, 32);
+  // message in a word-aligned buffer with 16 bytes of slack
+  u32* buf = new u32[(len + 16) / 4 + 4]();
+  memcpy(buf, msg, len);
+  MemHost mem{buf};
+  CTabHost tab;
+  BTabHost bt;
+  u32 st = ed25519_verify_item(sr, pr, mem, 0, len, tab, bt);
+  delete[] buf;
+  return (int)st;
+}
+
+// field op self-check: op 0 mul, 1 sqr, 2 inv, 3 add, 4 sub; mod 0..5; values as 32 BE bytes (plain)
+void dc_field(int op, int mod, const unsigned char* a, const unsigned char* b, unsigned char* out) {
+  u32 ra[8], rb[8], wa[8], wb[8], wo[8], ro[8];
+  memcpy(ra, a, 32);
+  memcpy(rb, b, 32);
+  words_from_be_bytes(ra, wa);
+  words_from_be_bytes(rb, wb);
+  fe x = fe_from_words(wa), y = fe_from_words(wb), r;
+#define DO(F)                                              \
+  {                                                        \
+    fe xm = to_mont<F>(x), ym = to_mont<F>(y), rm;         \
+    switch (op) {                                          \
+      case 0: rm = mont_mul<F>(xm, ym); break;             \
+      case 1: rm = mont_sqr<F>(xm); break;                 \
+      case 2: rm = finv<F>(xm); break;                     \
+      case 3: rm = fadd<F>(xm, ym); break;                 \
+      default: rm = fsub<F>(xm, ym); break;                \
+    }                                                      \
+    r = from_mont<F>(rm);                                  \
+  }
+  switch (mod) {
+    case 0: DO(ModPk1) break;
+    case 1: DO(ModNk1) break;
+    case 2: DO(ModPr1) break;
+    case 3: DO(ModNr1) break;
+    case 4: DO(ModPed) break;
+    default: DO(ModL) break;
+  }
+  fe_to_words(r, wo);
+  words_to_be_bytes(wo, ro);
+  memcpy(out, ro, 32);
+}
+
+void dc_sha512_ed(const unsigned char* hdr64, const unsigned char* msg, unsigned len, unsigned char* out) {
+  u32 hdr[16], h[16];
+  memcpy(hdr, hdr64, 64);
+  u32* buf = new u32[(len + 16) / 4 + 4]();
+  memcpy(buf, msg, len);
+  MemHost mem{buf};
+  sha512_ed(hdr, mem, 0, len, h);
+  memcpy(out, h, 64);
+  delete[] buf;
+}
+}

@@ -1,0 +1,56 @@
+"""ctypes binding of tests/devcheck/libdevcheck.so: the device headers built for the host."""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "devcheck", "devcheck.cpp")
+SO = os.path.join(HERE, "devcheck", "libdevcheck.so")
+CSRC = os.path.join(REPO, "wgpu-sigops_amd", "csrc")
+
+_lib = None
+
+
+def build():
+    deps = [SRC] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    if os.path.exists(SO) and all(os.path.getmtime(SO) >= os.path.getmtime(d) for d in deps):
+        return SO
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-I" + CSRC,
+                    "-o", SO, SRC], check=True)
+    return SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(build())
+        c_p = ctypes.c_char_p
+        L.dc_secp_recover.argtypes = [ctypes.c_int, c_p, c_p, c_p, ctypes.c_int]
+        L.dc_ed25519_verify.argtypes = [c_p, c_p, c_p, ctypes.c_uint]
+        L.dc_field.argtypes = [ctypes.c_int, ctypes.c_int, c_p, c_p, c_p]
+        L.dc_sha512_ed.argtypes = [c_p, c_p, ctypes.c_uint, c_p]
+        _lib = L
+    return _lib
+
+
+def secp_recover(curve, sig, dig, force_careful=False):
+    out = ctypes.create_string_buffer(64)
+    st = lib().dc_secp_recover(curve, sig, dig, out, int(force_careful))
+    return st, out.raw
+
+
+def ed25519_verify(sig, pk, msg):
+    return lib().dc_ed25519_verify(sig, pk, msg, len(msg))
+
+
+def field(op, mod, a, b=0):
+    out = ctypes.create_string_buffer(32)
+    lib().dc_field(op, mod, a.to_bytes(32, "big"), b.to_bytes(32, "big"), out)
+    return int.from_bytes(out.raw, "big")
+
+
+def sha512_ed(hdr, msg):
+    out = ctypes.create_string_buffer(64)
+    lib().dc_sha512_ed(hdr, msg, len(msg), out)
+    return out.raw

@@ -1,0 +1,67 @@
+"""The device algorithms (wgpu-sigops_amd/csrc/*.hpp) compiled for the host with g++ and compared
+with the oracle and the closed forms, so that a logic error is caught on the CPU before any GPU
+run. The GPU tests (test_gpu_*.py) then check the HIP build of the same code."""
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+import devcheck_lib as D
+import edge_cases
+import oracle_lib as O
+import pyref
+
+MODULI = [pyref.CURVES[0]["p"], pyref.CURVES[0]["n"], pyref.CURVES[1]["p"], pyref.CURVES[1]["n"], pyref.EP, pyref.EL]
+
+
+@pytest.mark.parametrize("mod", range(6))
+def test_field_ops(mod):
+    m = MODULI[mod]
+    rnd = random.Random(mod)
+    for a, b in [(0, 0), (m - 1, m - 1), (1, m - 1)] + [(rnd.randrange(m), rnd.randrange(m)) for _ in range(60)]:
+        assert D.field(0, mod, a, b) == a * b % m
+        assert D.field(1, mod, a) == a * a % m
+        assert D.field(3, mod, a, b) == (a + b) % m
+        assert D.field(4, mod, a, b) == (a - b) % m
+        if a:
+            assert D.field(2, mod, a) == pow(a, -1, m)
+
+
+def test_sha512_preimage_stream():
+    rnd = random.Random(2)
+    for L in list(range(0, 260)) + [1000, 4096]:
+        hdr = bytes(rnd.randrange(256) for _ in range(64))
+        msg = bytes(rnd.randrange(256) for _ in range(L))
+        assert D.sha512_ed(hdr, msg) == hashlib.sha512(hdr + msg).digest()
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_recover_random_and_edges(curve):
+    sigs, dig, pubs, cor = O.gen_secp(curve, 11, 64, corrupt_ppm=150000)
+    for i in range(64):
+        st, out = D.secp_recover(curve, bytes(sigs[i]), bytes(dig[i]))
+        assert (st & 0xFF, out) == O.secp_recover(curve, bytes(sigs[i]), bytes(dig[i]))
+    cases = edge_cases.secp_cases(curve)
+    hits = 0
+    for (name, sig, d), exp in zip(cases, edge_cases.secp_expected(curve, cases)):
+        st, out = D.secp_recover(curve, sig, d)
+        hits += st >> 8
+        assert (st & 0xFF, out) == exp, name
+    assert hits > 0  # the careful path was exercised
+    for i in range(8):  # careful path forced on ordinary inputs
+        st, out = D.secp_recover(curve, bytes(sigs[i]), bytes(dig[i]), force_careful=True)
+        assert (st & 0xFF, out) == O.secp_recover(curve, bytes(sigs[i]), bytes(dig[i]))
+
+
+def test_ed25519_random_and_edges():
+    cases = edge_cases.ed_cases()
+    for (name, sig, pk, m), exp in zip(cases, edge_cases.ed_expected(cases)):
+        assert (D.ed25519_verify(sig, pk, m) == 0) == exp, name
+    rng = np.random.default_rng(4)
+    lens = rng.integers(0, 300, 48)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    sigs, pks, msgs, cor = O.gen_ed25519(8, 48, off, corrupt_ppm=200000)
+    for i in range(48):
+        m = bytes(msgs[off[i]:off[i + 1]])
+        assert (D.ed25519_verify(bytes(sigs[i]), bytes(pks[i]), m) == 0) == (not cor[i])

@@ -1,0 +1,187 @@
+"""MI355X batch signature engine: Python binding of libsigops.so (C ABI in include/sigops.h).
+
+Two layers:
+
+* array API on a ``Context`` (numpy uint8 arrays in, numpy arrays out), used by the tests and the
+  benchmark: ``secp256k1_recover``, ``secp256r1_recover``, ``ed25519_verify``, plus ``*_dev``
+  variants over device pointers (inputs already resident in HBM);
+* a mirror of the reference's public functions with the same names, argument meaning and error
+  behaviour -- ``secp256k1_ecdsa.ecrecover`` / ``ecrecover_single_shader`` (src/secp256k1_ecdsa.rs:61,215),
+  ``secp256r1_ecdsa.*`` (src/secp256r1_ecdsa.rs:62,216), ``ed25519_eddsa.ecverify`` /
+  ``ecverify_single`` (src/ed25519_eddsa.rs:67,259), ``precompute.*_bases`` (src/precompute.rs:36-69)
+  and ``ShaderFailureError`` (src/lib.rs:12-14).
+
+There is no CPU fallback: if the library or a GPU is missing, every call raises.
+"""
+import ctypes
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libsigops.so")
+
+OK, E_ARG, E_DEVICE, E_NOMEM, E_CANARY = 0, -1, -2, -3, -4
+ST_OK, ST_BAD_R, ST_BAD_S, ST_NO_POINT, ST_INFINITY = 0, 1, 2, 3, 4
+ED_OK, ED_BAD_S, ED_BAD_KEY, ED_MISMATCH = 0, 1, 2, 3
+SCHEME_SECP256K1, SCHEME_SECP256R1, SCHEME_ED25519 = 0, 1, 2
+
+
+class ShaderFailureError(RuntimeError):
+    """Raised when a kernel did not produce every result (the reference's success-canary error)."""
+
+
+class SigopsError(RuntimeError):
+    """Any other failure of the native library (bad argument, HIP error, allocation failure)."""
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("blocks_per_cu", ctypes.c_uint32), ("max_batch", ctypes.c_uint32)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def build(force=False):
+    """Compile libsigops.so for gfx950 in-tree (make -C wgpu-sigops_amd)."""
+    cmd = ["make", "-C", PKG_DIR, "-j4"]
+    if force:
+        cmd.insert(1, "-B")
+    subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+def lib():
+    """Load libsigops.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise SigopsError(f"{LIB_PATH} is not built; run wgpu-sigops_amd build (make -C wgpu-sigops_amd)")
+            L = ctypes.CDLL(LIB_PATH)
+            vp, sz, c_p = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p
+            L.sigops_create.restype = vp
+            L.sigops_create.argtypes = [ctypes.POINTER(_Config)]
+            L.sigops_destroy.argtypes = [vp]
+            L.sigops_last_error.restype = ctypes.c_char_p
+            L.sigops_last_error.argtypes = [vp]
+            for name in ("sigops_secp256k1_recover", "sigops_secp256r1_recover"):
+                getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
+            L.sigops_ed25519_verify.argtypes = [vp, c_p, c_p, c_p, c_p, sz, c_p, c_p]
+            for name in ("sigops_secp256k1_recover_dev", "sigops_secp256r1_recover_dev"):
+                getattr(L, name).argtypes = [vp, vp, vp, sz, vp, vp, vp]
+            L.sigops_ed25519_verify_dev.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp]
+            _lib = L
+        return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
+
+
+def _u8(a, shape_tail, name):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    if a.ndim == 1 and shape_tail:
+        a = a.reshape(-1, shape_tail)
+    if shape_tail and (a.ndim != 2 or a.shape[1] != shape_tail):
+        raise ValueError(f"{name} must be (n, {shape_tail}) bytes")
+    return a
+
+
+class Context:
+    """One GPU context: resident generator tables, per-lane scratch and reusable I/O buffers."""
+
+    def __init__(self, device=0, blocks_per_cu=0, max_batch=0):
+        cfg = _Config(device, blocks_per_cu, max_batch)
+        self._h = lib().sigops_create(ctypes.byref(cfg))
+        if not self._h:
+            raise SigopsError("sigops_create failed (no usable GPU?)")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sigops_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc):
+        if rc == OK:
+            return
+        msg = lib().sigops_last_error(self._h).decode(errors="replace")
+        if rc == E_CANARY:
+            raise ShaderFailureError(msg)
+        raise SigopsError(f"libsigops error {rc}: {msg}")
+
+    def _secp(self, fn, sigs, digests):
+        sigs = _u8(sigs, 64, "sigs")
+        digests = _u8(digests, 32, "digests")
+        n = sigs.shape[0]
+        if digests.shape[0] != n:
+            raise ValueError("sigs and digests differ in length")
+        out = np.zeros((n, 64), np.uint8)
+        st = np.zeros(n, np.uint8)
+        self._check(fn(self._h, _ptr(sigs), _ptr(digests), n, _ptr(out), _ptr(st)))
+        return out, st
+
+    def secp256k1_recover(self, sigs, digests):
+        """(n,64) fuel signatures, (n,32) digests -> ((n,64) X||Y, (n,) status)"""
+        return self._secp(lib().sigops_secp256k1_recover, sigs, digests)
+
+    def secp256r1_recover(self, sigs, digests):
+        return self._secp(lib().sigops_secp256r1_recover, sigs, digests)
+
+    def ed25519_verify(self, sigs, pubkeys, msgs, msg_off):
+        """(n,64) R||s, (n,32) keys, concatenated messages + (n+1) offsets -> ((n,) bool, (n,) status)"""
+        sigs = _u8(sigs, 64, "sigs")
+        pubkeys = _u8(pubkeys, 32, "pubkeys")
+        n = sigs.shape[0]
+        msg_off = np.ascontiguousarray(msg_off, dtype=np.uint64)
+        if pubkeys.shape[0] != n or msg_off.shape[0] != n + 1:
+            raise ValueError("sigs, pubkeys and msg_off (n+1) differ in length")
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint8).reshape(-1)
+        if n and int(msg_off[-1]) > msgs.size:
+            raise ValueError("msg_off points past the message buffer")
+        valid = np.zeros(n, np.uint8)
+        st = np.zeros(n, np.uint8)
+        self._check(lib().sigops_ed25519_verify(self._h, _ptr(sigs), _ptr(pubkeys), _ptr(msgs) if msgs.size else None,
+                                                _ptr(msg_off), n, _ptr(valid), _ptr(st)))
+        return valid.astype(bool), st
+
+    # device-resident variants (torch tensors or raw ints as device pointers)
+    def secp_recover_dev(self, curve, d_sigs, d_digests, n, d_out, d_status, stream=0):
+        fn = lib().sigops_secp256k1_recover_dev if curve == 0 else lib().sigops_secp256r1_recover_dev
+        self._check(fn(self._h, d_sigs, d_digests, n, d_out, d_status, stream or None))
+
+    def ed25519_verify_dev(self, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream=0):
+        self._check(lib().sigops_ed25519_verify_dev(self._h, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status,
+                                                    stream or None))
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context()
+    return _default_ctx
+
+
+from . import ed25519_eddsa, precompute, secp256k1_ecdsa, secp256r1_ecdsa  # noqa: E402,F401

@@ -1,0 +1,32 @@
+"""Shared body of the secp256k1 / secp256r1 mirrors."""
+import numpy as np
+
+
+def _as_bytes_array(items, width, what):
+    if isinstance(items, np.ndarray):
+        arr = np.ascontiguousarray(items, dtype=np.uint8).reshape(-1, width)
+        return arr
+    buf = bytearray()
+    for x in items:
+        b = bytes(x)
+        if len(b) != width:
+            raise ValueError(f"each {what} must be {width} bytes")
+        buf += b
+    return np.frombuffer(bytes(buf), dtype=np.uint8).reshape(-1, width)
+
+
+def ecrecover_impl(curve, signatures, messages, ctx):
+    from . import default_context
+
+    if len(signatures) != len(messages):
+        raise ValueError("signatures and messages differ in length")
+    if len(signatures) > 256 * 256 * 256 * 64:
+        raise ValueError("too many signatures")
+    if len(signatures) == 0:
+        return [], np.zeros(0, np.uint8)
+    sigs = _as_bytes_array(signatures, 64, "signature")
+    digs = _as_bytes_array(messages, 32, "message")
+    ctx = ctx or default_context()
+    fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
+    out, st = fn(sigs, digs)
+    return [bytes(row) for row in out], st

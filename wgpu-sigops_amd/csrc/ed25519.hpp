@@ -1,0 +1,257 @@
+// ed25519.hpp -- Ed25519 verification with ed25519-dalek `VerifyingKey::verify` semantics, one
+// signature per lane.
+//
+// Reference flow: src/wgsl/ed25519_eddsa.wgsl:6-63 (-A, [s]B + [k](-A)),
+// src/wgsl/ed25519_utils.wgsl:27-118 (pow_p58, sqrt_ratio_i, decompression),
+// src/wgsl/main/ed25519_eddsa_main_0.wgsl:23-86 (decoding, A.y mod p, preimage) and _5.wgsl:20-46
+// (compress and byte-compare with R). Acceptance rules follow dalek (the reference test oracle,
+// src/tests/ed25519_eddsa.rs:26) rather than the WGSL, which skips the s < l check:
+//   s >= l -> reject; A must decompress (y taken mod p, sign bit applied also to x = 0);
+//   k = SHA-512(R || A || M) mod l over the given bytes; accept iff compress([s]B - [k]A) == R.
+//
+// Group law: extended twisted Edwards (a = -1) with the unified formulas of Hisil-Wong-Carter-
+// Dawson 2008 in the completed-point form used by curve25519-dalek; they are complete on the
+// whole curve (d is a non-square), so torsion components of A need no special handling and there
+// is no exceptional path. Scalar multiplication shares one doubling chain (Strauss-Shamir):
+//   [s]B: w = 8 signed odd digits over an LDS-staged table of (2j+1)B in affine Niels form;
+//   [k](-A): w = 4 signed odd digits over a per-lane table of odd multiples in cached form.
+#pragma once
+#include "secp.hpp"  // nibble / byte8 helpers and status codes
+#include "sha512.hpp"
+
+namespace sigops {
+
+typedef ModPed Fe;
+typedef ModL Fl;
+
+struct ext {
+  fe X, Y, Z, T;
+};
+struct cached {  // (Y+X, Y-X, Z, 2d*T)
+  fe ypx, ymx, z, t2d;
+};
+
+DEV ext ext_identity() {
+  ext r;
+  r.X = fe_zero();
+  r.Y = fe_from_const(Fe::ONE);
+  r.Z = r.Y;
+  r.T = fe_zero();
+  return r;
+}
+
+DEV cached to_cached(const ext& P) {
+  cached c;
+  c.ypx = fadd<Fe>(P.Y, P.X);
+  c.ymx = fsub<Fe>(P.Y, P.X);
+  c.z = P.Z;
+  c.t2d = mont_mul<Fe>(P.T, fe_from_const(ED_D2));
+  return c;
+}
+
+DEV cached neg_cached(const cached& c) {
+  cached r;
+  r.ypx = c.ymx;
+  r.ymx = c.ypx;
+  r.z = c.z;
+  r.t2d = fneg<Fe>(c.t2d);
+  return r;
+}
+
+// completed point (X:Y:Z:T) with x = X/Z, y = Y/T -> extended
+DEV ext completed_to_ext(const fe& X, const fe& Y, const fe& Z, const fe& T, bool want_t) {
+  ext r;
+  r.X = mont_mul<Fe>(X, T);
+  r.Y = mont_mul<Fe>(Y, Z);
+  r.Z = mont_mul<Fe>(Z, T);
+  r.T = want_t ? mont_mul<Fe>(X, Y) : fe_zero();
+  return r;
+}
+
+// doubling (a = -1); T of the input is not used
+DEV ext edbl(const ext& P, bool want_t) {
+  const fe XX = mont_sqr<Fe>(P.X);
+  const fe YY = mont_sqr<Fe>(P.Y);
+  const fe ZZ2 = fdbl<Fe>(mont_sqr<Fe>(P.Z));
+  const fe XpY2 = mont_sqr<Fe>(fadd<Fe>(P.X, P.Y));
+  const fe YYpXX = fadd<Fe>(YY, XX);
+  const fe YYmXX = fsub<Fe>(YY, XX);
+  return completed_to_ext(fsub<Fe>(XpY2, YYpXX), YYpXX, YYmXX, fsub<Fe>(ZZ2, YYmXX), want_t);
+}
+
+DEV ext eadd_cached(const ext& P, const cached& Q, bool want_t) {
+  const fe PP = mont_mul<Fe>(fadd<Fe>(P.Y, P.X), Q.ypx);
+  const fe MM = mont_mul<Fe>(fsub<Fe>(P.Y, P.X), Q.ymx);
+  const fe TT2d = mont_mul<Fe>(P.T, Q.t2d);
+  const fe ZZ2 = fdbl<Fe>(mont_mul<Fe>(P.Z, Q.z));
+  return completed_to_ext(fsub<Fe>(PP, MM), fadd<Fe>(PP, MM), fadd<Fe>(ZZ2, TT2d), fsub<Fe>(ZZ2, TT2d), want_t);
+}
+
+// affine Niels (y+x, y-x, 2d*x*y), optionally negated
+DEV ext eadd_niels(const ext& P, const fe& ypx, const fe& ymx, const fe& t2d, bool want_t) {
+  const fe PP = mont_mul<Fe>(fadd<Fe>(P.Y, P.X), ypx);
+  const fe MM = mont_mul<Fe>(fsub<Fe>(P.Y, P.X), ymx);
+  const fe TT2d = mont_mul<Fe>(P.T, t2d);
+  const fe ZZ2 = fdbl<Fe>(P.Z);
+  return completed_to_ext(fsub<Fe>(PP, MM), fadd<Fe>(PP, MM), fadd<Fe>(ZZ2, TT2d), fsub<Fe>(ZZ2, TT2d), want_t);
+}
+
+// dalek FieldElement::sqrt_ratio_i (field.rs): returns was_nonzero_square, r = nonnegative root
+DEV bool sqrt_ratio_i(const fe& u, const fe& v, fe& r_out) {
+  const fe v3 = mont_mul<Fe>(mont_sqr<Fe>(v), v);
+  const fe v7 = mont_mul<Fe>(mont_sqr<Fe>(v3), v);
+  fe r = mont_mul<Fe>(mont_mul<Fe>(u, v3), fpow_const<Fe>(mont_mul<Fe>(u, v7), ED_P58_EXP));
+  const fe check = mont_mul<Fe>(v, mont_sqr<Fe>(r));
+  const fe i = fe_from_const(ED_SQRTM1);
+  const fe neg_u = fneg<Fe>(u);
+  const bool correct = feq<Fe>(check, u);
+  const bool flipped = feq<Fe>(check, neg_u);
+  const bool flipped_i = feq<Fe>(check, mont_mul<Fe>(neg_u, i));
+  const fe r_prime = mont_mul<Fe>(i, r);
+  r = fsel(flipped || flipped_i, r_prime, r);
+  const bool r_neg = (from_mont<Fe>(r).v[0] & 1u) != 0;
+  r = fsel(r_neg, fneg<Fe>(r), r);
+  r_out = r;
+  return correct || flipped;
+}
+
+enum : u32 { ED_OK = 0, ED_BAD_S = 1, ED_BAD_KEY = 2, ED_MISMATCH = 3 };
+
+// sig_raw: 16 raw words (R || s); pk_raw: 8 raw words; message in mem at msg_off, len bytes.
+// BTab: load(idx, ypx, ymx, t2d) for (2 idx + 1) B. Tab: per-lane store(e, cached) / load(e).
+template <class Tab, class BTab, class Mem>
+DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Mem& mem, u64 msg_off,
+                            u32 len, Tab& tab, const BTab& bt) {
+  u32 status = ED_OK;
+  // s (little-endian bytes = little-endian words)
+  u32 s[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) s[i] = sig_raw[8 + i];
+  if (!words_lt(s, ED_L32)) status = ED_BAD_S;
+  // A: y with bit 255 cleared, reduced mod p
+  u32 yw[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) yw[i] = pk_raw[i];
+  const bool x_sign = (yw[7] >> 31) != 0;
+  yw[7] &= 0x7fffffffu;
+  if (!words_lt(yw, ED_P32)) {
+    u32 t[8];
+    words_sub(yw, ED_P32, t);
+#pragma unroll
+    for (int i = 0; i < 8; i++) yw[i] = t[i];
+  }
+  const fe y = to_mont<Fe>(fe_from_words(yw));
+  const fe one = fe_from_const(Fe::ONE);
+  const fe yy = mont_sqr<Fe>(y);
+  const fe u = fsub<Fe>(yy, one);
+  const fe v = fadd<Fe>(mont_mul<Fe>(yy, fe_from_const(ED_D)), one);
+  fe x;
+  const bool ok = sqrt_ratio_i(u, v, x);
+  if (!ok && status == ED_OK) status = ED_BAD_KEY;
+  if (x_sign) x = fneg<Fe>(x);
+  // k = SHA-512(R || A || M) mod l
+  u32 hdr[16], h[16];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    hdr[i] = sig_raw[i];
+    hdr[8 + i] = pk_raw[i];
+  }
+  sha512_ed(hdr, mem, msg_off, len, h);
+  u32 k[8];
+  {
+    const fe hi = mont_mul<Fl>(fe_from_words(h + 8), fe_from_const(L_C256));
+    const fe lo = mont_mul<Fl>(fe_from_words(h), fe_from_const(L_C0));
+    fe_to_words(fcanon<Fl>(fadd<Fl>(hi, lo)), k);
+  }
+  // odd representatives: s even -> l - s with B negated; k even -> k + 1 and a final +A
+  const bool b_neg = (s[0] & 1u) == 0;
+  const bool k_even = (k[0] & 1u) == 0;
+  u32 s1[8], k1[8];
+  {
+    u32 t[8];
+    words_sub(ED_L32, s, t);  // only used when s < l (else the result is discarded)
+    u32 c = k_even ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      s1[i] = b_neg ? t[i] : s[i];
+      const u64 kk = (u64)k[i] + c;
+      k1[i] = (u32)kk;
+      c = (u32)(kk >> 32);
+    }
+  }
+  u32 c1[8], c2[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    c1[i] = (s1[i] >> 1) | (i < 7 ? s1[i + 1] << 31 : 0x80000000u);
+    c2[i] = (k1[i] >> 1) | (i < 7 ? k1[i + 1] << 31 : 0x80000000u);
+  }
+  // table of odd multiples of -A
+  ext nA;
+  nA.X = fneg<Fe>(x);
+  nA.Y = y;
+  nA.Z = one;
+  nA.T = mont_mul<Fe>(nA.X, y);
+  const cached nA2 = to_cached(edbl(nA, true));
+  ext T = nA;
+  tab.store(0, to_cached(nA));
+#pragma unroll 1
+  for (int e = 1; e < 8; e++) {
+    T = eadd_cached(T, nA2, true);
+    tab.store(e, to_cached(T));
+  }
+  ext acc = eadd_cached(ext_identity(), tab.load(nibble(c2, 63) - 8), true);
+#pragma unroll 1
+  for (int i = 62; i >= 0; i--) {
+    acc = edbl(acc, false);
+    acc = edbl(acc, false);
+    acc = edbl(acc, false);
+    acc = edbl(acc, true);
+    {
+      const u32 c = nibble(c2, i);
+      const bool neg = c < 8;
+      const u32 idx = (neg ? ~c : c) & 7u;
+      cached E = tab.load(idx);
+      if (neg) E = neg_cached(E);
+      acc = eadd_cached(acc, E, true);
+    }
+    if ((i & 1) == 0) {
+      const u32 c = byte8(c1, i >> 1);
+      const bool neg = (c < 128) != b_neg;
+      const u32 idx = ((c < 128) ? ~c : c) & 127u;
+      fe ypx, ymx, t2d;
+      bt.load(idx, ypx, ymx, t2d);
+      if (neg) {
+        const fe tmp = ypx;
+        ypx = ymx;
+        ymx = tmp;
+        t2d = fneg<Fe>(t2d);
+      }
+      acc = eadd_niels(acc, ypx, ymx, t2d, true);
+    }
+  }
+  {
+    // + A when k was made odd by adding one, else + identity (complete formulas)
+    cached corr = neg_cached(tab.load(0));
+    if (!k_even) {
+      corr.ypx = one;
+      corr.ymx = one;
+      corr.z = one;
+      corr.t2d = fe_zero();
+    }
+    acc = eadd_cached(acc, corr, false);
+  }
+  // compress and compare with R
+  const fe zi = finv<Fe>(acc.Z);
+  const fe xa = from_mont<Fe>(mont_mul<Fe>(acc.X, zi));
+  const fe ya = from_mont<Fe>(mont_mul<Fe>(acc.Y, zi));
+  u32 enc[8];
+  fe_to_words(ya, enc);
+  enc[7] ^= (xa.v[0] & 1u) << 31;
+  u32 diff = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) diff |= enc[i] ^ sig_raw[i];
+  if (status == ED_OK && diff != 0) status = ED_MISMATCH;
+  return status;
+}
+
+}  // namespace sigops

@@ -1,0 +1,17 @@
+// kern_ed.hip -- instantiates the Ed25519 verification kernel and provides its launch helper.
+#include "kernels_impl.hpp"
+
+namespace sigops {
+
+hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* msgs, const u64* msg_off,
+                                 void* status, u32 n, const void* btab, void* ctab, u32 slots, u32 grid,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const u32 want = (n + 255) / 256;
+  const u32 g = want < grid ? want : grid;
+  k_ed25519_verify<<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, (const u32*)msgs, msg_off, (u8*)status,
+                                    n, (const uint4*)btab, (uint4*)ctab, slots);
+  return hipGetLastError();
+}
+
+}  // namespace sigops

@@ -1,0 +1,232 @@
+// kernels_impl.hpp -- gfx950 kernels of the signature engine (instantiated by kern_*.hip).
+//
+// One lane per signature, persistent grid-stride kernels (the grid is sized to the resident
+// capacity of the GPU, so a lane's slot index also names its private table in HBM). Per kernel:
+//   * the fixed-base table (odd multiples of G or B, uploaded to HBM once per context) is staged
+//     into LDS by each workgroup and read with ds_read_b128;
+//   * the per-signature variable-base table lives in HBM scratch, laid out [entry][chunk][slot]
+//     so that lanes that pick the same entry read contiguous 16-byte chunks;
+//   * secp lanes whose fast ladder meets an exceptional addition are appended to a pending list
+//     and finished by k_secp_careful, enqueued right after on the same stream (no host round trip).
+// Replaces the reference's 19 WGSL entry points (src/wgsl/main/*.wgsl), whose multi-stage variants
+// write every intermediate point to memory between dispatches (src/secp256k1_ecdsa.rs:78-193).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ed25519.hpp"
+#include "kernels.hpp"
+#include "secp.hpp"
+
+namespace sigops {
+
+// ---- table accessors ----
+struct GTabLds {  // secp fixed-base: entry = 5 x uint4 (x[9], y[9], pad[2])
+  const uint4* t;
+  DEV void load(u32 idx, fe& x, fe& y) const {
+    u32 w[20];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const uint4 q = t[idx * 5 + c];
+      w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) { x.v[k] = w[k]; y.v[k] = w[9 + k]; }
+  }
+};
+
+struct BTabLds {  // Ed25519 fixed-base: entry = 7 x uint4 (ypx[9], ymx[9], t2d[9], pad)
+  const uint4* t;
+  DEV void load(u32 idx, fe& ypx, fe& ymx, fe& t2d) const {
+    u32 w[28];
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+      const uint4 q = t[idx * 7 + c];
+      w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) { ypx.v[k] = w[k]; ymx.v[k] = w[9 + k]; t2d.v[k] = w[18 + k]; }
+  }
+};
+
+// per-lane table in HBM: chunk c of entry e of slot s at base[(e * CH + c) * slots + s]
+template <int CH>
+struct LaneTab {
+  uint4* base;
+  u32 slots, slot;
+  DEV void put(int e, const u32* w) {
+#pragma unroll
+    for (int c = 0; c < CH; c++)
+      base[((u64)e * CH + c) * slots + slot] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
+  }
+  DEV void get(u32 e, u32* w) const {
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const uint4 q = base[((u64)e * CH + c) * slots + slot];
+      w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
+    }
+  }
+};
+
+struct JTabDev : LaneTab<7> {  // Jacobian point: 27 words + pad
+  DEV void store(int e, const jac& p) {
+    u32 w[28];
+#pragma unroll
+    for (int k = 0; k < NL; k++) { w[k] = p.X.v[k]; w[9 + k] = p.Y.v[k]; w[18 + k] = p.Z.v[k]; }
+    w[27] = 0;
+    put(e, w);
+  }
+  DEV jac load(u32 e) const {
+    u32 w[28];
+    get(e, w);
+    jac p;
+#pragma unroll
+    for (int k = 0; k < NL; k++) { p.X.v[k] = w[k]; p.Y.v[k] = w[9 + k]; p.Z.v[k] = w[18 + k]; }
+    return p;
+  }
+};
+
+struct CTabDev : LaneTab<9> {  // cached Edwards point: 36 words
+  DEV void store(int e, const cached& p) {
+    u32 w[36];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      w[k] = p.ypx.v[k]; w[9 + k] = p.ymx.v[k]; w[18 + k] = p.z.v[k]; w[27 + k] = p.t2d.v[k];
+    }
+    put(e, w);
+  }
+  DEV cached load(u32 e) const {
+    u32 w[36];
+    get(e, w);
+    cached p;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      p.ypx.v[k] = w[k]; p.ymx.v[k] = w[9 + k]; p.z.v[k] = w[18 + k]; p.t2d.v[k] = w[27 + k];
+    }
+    return p;
+  }
+};
+
+struct GTabGlobal {
+  const uint4* t;
+  DEV void load(u32 idx, fe& x, fe& y) const { GTabLds{t}.load(idx, x, y); }
+};
+
+struct MemDev {
+  const u32* w;
+  DEV u32 word(u64 i) const { return w[i]; }
+};
+
+DEV void load_words(const uint4* p, u32* w, int nq) {
+  for (int c = 0; c < nq; c++) {
+    const uint4 q = p[c];
+    w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
+  }
+}
+
+// ---- kernels ----
+template <class C>
+__global__ __launch_bounds__(256) void k_secp_recover(const uint4* __restrict__ sigs,
+                                                      const uint4* __restrict__ digs, uint4* __restrict__ out,
+                                                      u8* __restrict__ status, u32 n,
+                                                      const uint4* __restrict__ gtab, uint4* __restrict__ rtab,
+                                                      u32 slots, u32* __restrict__ pend_list,
+                                                      u32* __restrict__ pend_count) {
+  __shared__ uint4 sg[GTAB_ENTRIES * 5];
+  for (int t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) sg[t] = gtab[t];
+  __syncthreads();
+  const GTabLds gt{sg};
+  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+  JTabDev tab;
+  tab.base = rtab;
+  tab.slots = slots;
+  tab.slot = slot;
+  const u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = slot; i < n; i += stride) {
+    u32 sr[16], dr[8], o[16];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(digs + 2 * (u64)i, dr, 2);
+    secp_pro pro;
+    secp_prologue<C>(sr, dr, pro);
+    u32 st = pro.status;
+    if (st == ST_OK) {
+      bool exc = false;
+      const jac Q = secp_ladder<C>(pro, tab, gt, exc);
+      if (exc) {
+        st = ST_PENDING;
+        pend_list[atomicAdd(pend_count, 1u)] = i;
+      } else {
+        secp_output<C>(Q, o);
+      }
+    }
+    if (st != ST_OK) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) o[k] = 0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[4 * (u64)i + c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
+    status[i] = (u8)st;
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(64) void k_secp_careful(const uint4* __restrict__ sigs, const uint4* __restrict__ digs,
+                                                     uint4* __restrict__ out, u8* __restrict__ status,
+                                                     const uint4* __restrict__ gtab,
+                                                     const u32* __restrict__ pend_list,
+                                                     const u32* __restrict__ pend_count) {
+  const u32 cnt = *pend_count;
+  const GTabGlobal gt{gtab};
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const u32 i = pend_list[j];
+    u32 sr[16], dr[8], o[16];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(digs + 2 * (u64)i, dr, 2);
+    secp_pro pro;
+    secp_prologue<C>(sr, dr, pro);
+    const jacf Q = secp_ladder_careful<C>(pro, gt);
+    u32 st = ST_OK;
+    if (Q.inf) {
+      st = ST_INFINITY;
+#pragma unroll
+      for (int k = 0; k < 16; k++) o[k] = 0;
+    } else {
+      secp_output<C>(Q.P, o);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[4 * (u64)i + c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
+    status[i] = (u8)st;
+  }
+}
+
+static __global__ __launch_bounds__(256) void k_ed25519_verify(const uint4* __restrict__ sigs,
+                                                        const uint4* __restrict__ pks,
+                                                        const u32* __restrict__ msgs,
+                                                        const u64* __restrict__ msg_off, u8* __restrict__ status,
+                                                        u32 n, const uint4* __restrict__ btab,
+                                                        uint4* __restrict__ ctab, u32 slots) {
+  __shared__ uint4 sb[BTAB_ENTRIES * 7];
+  for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) sb[t] = btab[t];
+  __syncthreads();
+  const BTabLds bt{sb};
+  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+  CTabDev tab;
+  tab.base = ctab;
+  tab.slots = slots;
+  tab.slot = slot;
+  const MemDev mem{msgs};
+  const u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = slot; i < n; i += stride) {
+    u32 sr[16], pr[8];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(pks + 2 * (u64)i, pr, 2);
+    const u64 off = msg_off[i];
+    const u32 len = (u32)(msg_off[i + 1] - off);
+    status[i] = (u8)ed25519_verify_item(sr, pr, mem, off, len, tab, bt);
+  }
+}
+
+static __global__ void k_fill_u32(u32* p, u32 v, u32 n) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace sigops

@@ -1,0 +1,101 @@
+// sha512.hpp -- SHA-512 (FIPS 180-4) of R || A || M for Ed25519, one message per lane.
+//
+// The reference hashes exactly 96 bytes in one block (src/wgsl/sha512.wgsl:114-194, padding
+// hard-coded at :122-123) because its messages are fixed 32-byte digests; here M has any length,
+// so the padded preimage is generated word by word: 64 header bytes (R || A) from registers, the
+// message from memory, then 0x80, zeros and the 128-bit bit length. Lanes of a wavefront with
+// fewer blocks go idle while the longest message finishes, which is why the batched kernel groups
+// Ed25519 items by block count.
+#pragma once
+#include "field.hpp"
+
+namespace sigops {
+
+DEV u64 rotr64(u64 x, int n) { return (x >> n) | (x << (64 - n)); }
+DEV u64 bswap64(u64 x) { return __builtin_bswap64(x); }
+
+DEV void sha512_compress(u64 h[8], u64 w[16]) {
+  u64 a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+#pragma unroll
+  for (int t = 0; t < 80; t++) {
+    if (t >= 16) {
+      const u64 w15 = w[(t + 1) & 15], w2 = w[(t + 14) & 15];
+      const u64 s0 = rotr64(w15, 1) ^ rotr64(w15, 8) ^ (w15 >> 7);
+      const u64 s1 = rotr64(w2, 19) ^ rotr64(w2, 61) ^ (w2 >> 6);
+      w[t & 15] += s0 + w[(t + 9) & 15] + s1;
+    }
+    const u64 S1 = rotr64(e, 14) ^ rotr64(e, 18) ^ rotr64(e, 41);
+    const u64 ch = (e & f) ^ (~e & g);
+    const u64 t1 = hh + S1 + ch + SHA512_K[t] + w[t & 15];
+    const u64 S0 = rotr64(a, 28) ^ rotr64(a, 34) ^ rotr64(a, 39);
+    const u64 maj = (a & b) | (c & (a | b));
+    const u64 t2 = S0 + maj;
+    hh = g;
+    g = f;
+    f = e;
+    e = d + t1;
+    d = c;
+    c = b;
+    b = a;
+    a = t1 + t2;
+  }
+  h[0] += a; h[1] += b; h[2] += c; h[3] += d;
+  h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+}
+
+// 8 little-endian bytes at byte offset off of a 4-byte-aligned buffer (reads 12 bytes starting at
+// the aligned word below off; the buffer carries >= 16 bytes of slack after the last message).
+template <class Mem>
+DEV u64 load8_le(const Mem& mem, u64 off) {
+  const u64 wa = off >> 2;
+  const u32 sh = (u32)(off & 3u) * 8u;
+  const u32 d0 = mem.word(wa), d1 = mem.word(wa + 1), d2 = mem.word(wa + 2);
+  const u64 lo = (u64)d0 | ((u64)d1 << 32);
+  const u64 hi = (u64)d2;
+  return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+}
+
+// SHA-512(hdr64 || msg[0..len)) where hdr is 16 raw u32 words (R then A, as loaded from memory).
+// Output: the 64 digest bytes as 16 little-endian u32 words.
+template <class Mem>
+DEV void sha512_ed(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 out[16]) {
+  u64 h[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) h[i] = SHA512_IV[i];
+  const u64 total = 64ull + len;
+  const u32 nblk = (u32)((total + 1 + 16 + 127) >> 7);
+  for (u32 b = 0; b < nblk; b++) {
+    u64 w[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const u64 o = (u64)b * 128 + 8 * q;  // byte offset in the preimage
+      u64 word;
+      if (o < 64) {
+        const int hq = (int)(o >> 3);
+        word = bswap64((u64)hdr[2 * hq] | ((u64)hdr[2 * hq + 1] << 32));
+      } else {
+        const u64 mo = o - 64;
+        u64 le = 0;
+        if (mo < len) le = load8_le(mem, msg_off + mo);
+        const i64 keep = (i64)len - (i64)mo;  // message bytes in this word
+        if (keep < 8) {
+          const u64 m = keep <= 0 ? 0ull : ((1ull << (8 * keep)) - 1ull);
+          le &= m;
+          if (keep >= 0) le |= 0x80ull << (8 * keep);
+        }
+        word = bswap64(le);
+        if (b == nblk - 1 && q == 15) word = total * 8ull;
+      }
+      w[q] = word;
+    }
+    sha512_compress(h, w);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u64 le = bswap64(h[i]);
+    out[2 * i] = (u32)le;
+    out[2 * i + 1] = (u32)(le >> 32);
+  }
+}
+
+}  // namespace sigops

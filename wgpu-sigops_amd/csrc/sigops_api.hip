@@ -1,0 +1,256 @@
+// sigops_api.hip -- host runtime behind include/sigops.h.
+//
+// Replaces the reference's per-call wgpu life cycle (src/gpu.rs:5-186: adapter + device creation,
+// WGSL compilation, buffer creation, dispatch, staging readback and device.destroy on every call,
+// SURVEY Q8) with a context that is created once: code objects are loaded with the library, the
+// generator tables are uploaded to HBM once, and per-lane scratch plus I/O buffers are reused.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sigops.h"
+#include "consts.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+constexpr uint8_t CANARY = 0xEE;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct sigops_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t grid = 0;
+  uint32_t slots = 0;
+  uint32_t max_batch = 1u << 22;
+  void* d_gtab_k1 = nullptr;
+  void* d_gtab_r1 = nullptr;
+  void* d_btab = nullptr;
+  void* d_lane_tab = nullptr;
+  uint32_t* d_pend = nullptr;  // [0] = count, [1 ..] = list
+  DevBuf in, aux, msg, off, out, st;
+  std::vector<uint8_t> host_status;
+  std::vector<uint64_t> host_off;
+  std::string err;
+};
+
+namespace {
+
+int fail(sigops_ctx* c, int code, const std::string& what) {
+  if (c) c->err = what;
+  return code;
+}
+
+int hip_fail(sigops_ctx* c, hipError_t e, const char* where) {
+  return fail(c, SIGOPS_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                     \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return hip_fail((c), e_, #expr); \
+  } while (0)
+
+int ensure(sigops_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.cap >= bytes) return SIGOPS_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  const size_t want = std::max<size_t>(bytes, 4096);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) return fail(c, SIGOPS_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b.cap = want;
+  return SIGOPS_OK;
+}
+
+__global__ void k_fill_bytes(uint8_t* p, uint8_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int secp_recover_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_digs, size_t n, void* d_out,
+                     void* d_status, hipStream_t s) {
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (n == 0) return SIGOPS_OK;
+  if (curve == 0)
+    HIPCHK(c, sigops::launch_secp_recover_CurveK1(d_sigs, d_digs, d_out, d_status, (uint32_t)n, c->d_gtab_k1,
+                                                 c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+  else
+    HIPCHK(c, sigops::launch_secp_recover_CurveR1(d_sigs, d_digs, d_out, d_status, (uint32_t)n, c->d_gtab_r1,
+                                                 c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+  return SIGOPS_OK;
+}
+
+int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8_t* digs, size_t n,
+                      uint8_t* out_pub, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  if (n == 0) return SIGOPS_OK;
+  if (!sigs || !digs || !out_pub) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  int rc;
+  if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) || (rc = ensure(c, c->out, n * 64)) ||
+      (rc = ensure(c, c->st, n)))
+    return rc;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->aux.p, digs, n * 32, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = secp_recover_dev(c, curve, c->in.p, c->aux.p, n, c->out.p, c->st.p, s))) return rc;
+  uint8_t* st = status;
+  if (!st) {
+    c->host_status.resize(n);
+    st = c->host_status.data();
+  }
+  HIPCHK(c, hipMemcpyAsync(out_pub, c->out.p, n * 64, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; i++)
+    if (st[i] > SIGOPS_ST_INFINITY) return fail(c, SIGOPS_E_CANARY, "a recovery result was not written");
+  return SIGOPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sigops_ctx* sigops_create(const sigops_config* cfg) {
+  sigops_ctx* c = new (std::nothrow) sigops_ctx();
+  if (!c) return nullptr;
+  sigops_config def{};
+  if (!cfg) cfg = &def;
+  c->device = cfg->device;
+  if (cfg->max_batch) c->max_batch = cfg->max_batch;
+  auto bad = [&](hipError_t e, const char* what) {
+    fprintf(stderr, "sigops_create: %s: %s\n", what, hipGetErrorString(e));
+    sigops_destroy(c);
+    return (sigops_ctx*)nullptr;
+  };
+  hipError_t e;
+  if ((e = hipSetDevice(c->device)) != hipSuccess) return bad(e, "hipSetDevice");
+  int cus = 0;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)) != hipSuccess)
+    return bad(e, "hipDeviceGetAttribute");
+  const uint32_t bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : 2;
+  c->grid = (uint32_t)cus * bpc;
+  c->slots = c->grid * 256;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
+  if ((e = hipMalloc(&c->d_gtab_k1, sizeof(sigops::K1_GTABLE))) != hipSuccess) return bad(e, "hipMalloc");
+  if ((e = hipMalloc(&c->d_gtab_r1, sizeof(sigops::R1_GTABLE))) != hipSuccess) return bad(e, "hipMalloc");
+  if ((e = hipMalloc(&c->d_btab, sizeof(sigops::ED_BTABLE))) != hipSuccess) return bad(e, "hipMalloc");
+  if ((e = hipMemcpy(c->d_gtab_k1, sigops::K1_GTABLE, sizeof(sigops::K1_GTABLE), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return bad(e, "upload");
+  if ((e = hipMemcpy(c->d_gtab_r1, sigops::R1_GTABLE, sizeof(sigops::R1_GTABLE), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return bad(e, "upload");
+  if ((e = hipMemcpy(c->d_btab, sigops::ED_BTABLE, sizeof(sigops::ED_BTABLE), hipMemcpyHostToDevice)) != hipSuccess)
+    return bad(e, "upload");
+  const size_t lane_bytes =
+      (size_t)c->slots * std::max(sigops::SECP_LANE_TAB_BYTES, sigops::ED_LANE_TAB_BYTES);
+  if ((e = hipMalloc(&c->d_lane_tab, lane_bytes)) != hipSuccess) return bad(e, "hipMalloc lane tables");
+  if ((e = hipMalloc((void**)&c->d_pend, ((size_t)c->max_batch + 1) * sizeof(uint32_t))) != hipSuccess)
+    return bad(e, "hipMalloc pending list");
+  return c;
+}
+
+void sigops_destroy(sigops_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
+                  c->msg.p, c->off.p, c->out.p, c->st.p})
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* sigops_last_error(const sigops_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int sigops_secp256k1_recover(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                             uint8_t* out_pub, uint8_t* status) {
+  return secp_recover_host(c, 0, sigs, digests, n, out_pub, status);
+}
+
+int sigops_secp256r1_recover(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                             uint8_t* out_pub, uint8_t* status) {
+  return secp_recover_host(c, 1, sigs, digests, n, out_pub, status);
+}
+
+int sigops_secp256k1_recover_dev(sigops_ctx* c, const void* d_sigs, const void* d_digests, size_t n,
+                                 void* d_out_pub, void* d_status, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  return secp_recover_dev(c, 0, d_sigs, d_digests, n, d_out_pub, d_status, (hipStream_t)stream);
+}
+
+int sigops_secp256r1_recover_dev(sigops_ctx* c, const void* d_sigs, const void* d_digests, size_t n,
+                                 void* d_out_pub, void* d_status, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  return secp_recover_dev(c, 1, d_sigs, d_digests, n, d_out_pub, d_status, (hipStream_t)stream);
+}
+
+int sigops_ed25519_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
+                              const uint64_t* d_msg_off, size_t n, void* d_status, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (n == 0) return SIGOPS_OK;
+  HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_msgs, d_msg_off, d_status, (uint32_t)n, c->d_btab,
+                                          c->d_lane_tab, c->slots, c->grid, (hipStream_t)stream));
+  return SIGOPS_OK;
+}
+
+int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  if (n == 0) return SIGOPS_OK;
+  if (!sigs || !pubkeys || !msg_off || !valid) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  const uint64_t base = msg_off[0];
+  for (size_t i = 0; i < n; i++)
+    if (msg_off[i + 1] < msg_off[i] || msg_off[i + 1] - msg_off[i] > 0xFFFFFFFFull)
+      return fail(c, SIGOPS_E_ARG, "msg_off must be non-decreasing with lengths < 2^32");
+  const uint64_t total = msg_off[n] - base;
+  if (total && !msgs) return fail(c, SIGOPS_E_ARG, "NULL msgs");
+  int rc;
+  if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) ||
+      (rc = ensure(c, c->msg, total + 16)) || (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->st, n)))
+    return rc;
+  c->host_off.resize(n + 1);
+  for (size_t i = 0; i <= n; i++) c->host_off[i] = msg_off[i] - base;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->aux.p, pubkeys, n * 32, hipMemcpyHostToDevice, s));
+  if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, msgs + base, total, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
+  HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = sigops_ed25519_verify_dev(c, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->st.p, s)))
+    return rc;
+  uint8_t* st = status;
+  if (!st) {
+    c->host_status.resize(n);
+    st = c->host_status.data();
+  }
+  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; i++) {
+    if (st[i] > SIGOPS_ED_MISMATCH) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
+    valid[i] = st[i] == SIGOPS_ED_OK;
+  }
+  return SIGOPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+"""Mirror of the reference's secp256r1 (P-256) recovery API (src/secp256r1_ecdsa.rs).
+
+``ecrecover(signatures, messages, table_limbs, log_limb_size)`` and
+``ecrecover_single_shader(signatures, messages, log_limb_size)`` keep the reference's argument
+meaning and results: fuel-format 64-byte signatures (recovery parity in bit 7 of byte 32), 32-byte
+message digests, and a list of 64-byte X||Y public keys. Like the reference they raise
+``ShaderFailureError`` when the GPU did not produce the results, fail on a length mismatch
+(src/secp256r1_ecdsa.rs:22) and return [] for no input (:71-73). An invalid signature yields 64 zero
+bytes (the reference has no per-item error channel; ``recover_with_status`` exposes the codes).
+"""
+import numpy as np
+
+from . import precompute
+from ._secp_common import ecrecover_impl
+
+CURVE = 1
+
+
+def ecrecover(signatures, messages, table_limbs, log_limb_size, ctx=None):
+    precompute.check_table(table_limbs, 2, log_limb_size)
+    return ecrecover_impl(CURVE, signatures, messages, ctx)[0]
+
+
+def ecrecover_single_shader(signatures, messages, log_limb_size, ctx=None):
+    precompute.check_log_limb_size(log_limb_size)
+    return ecrecover_impl(CURVE, signatures, messages, ctx)[0]
+
+
+ecrecover_single = ecrecover_single_shader
+
+
+def recover_with_status(signatures, messages, ctx=None):
+    """(list of 64-byte keys, numpy status codes)"""
+    return ecrecover_impl(CURVE, signatures, messages, ctx)
+
+
+__all__ = ["ecrecover", "ecrecover_single_shader", "ecrecover_single", "recover_with_status", "np"]
