@@ -53,9 +53,12 @@ typedef struct sigops_ctx sigops_ctx;
 
 typedef struct {
   int device;               /* HIP device ordinal */
-  uint32_t blocks_per_cu;   /* resident 256-thread blocks per CU for the persistent kernels (0: default) */
+  uint32_t blocks_per_cu;   /* resident 256-thread blocks per CU for the persistent kernels (0: default 2) */
   uint32_t max_batch;       /* largest n accepted per call (0: 1 << 22, the reference's limit,
                                src/benchmarks/mod.rs:45) */
+  uint32_t tile_rounds;     /* mixed batch: a wavefront takes 64 * tile_rounds items of one bucket per
+                               queue pop (1..4, 0: default 1) */
+  uint32_t no_graph;        /* mixed batch: 1 = plain stream launches instead of the captured graph */
 } sigops_config;
 
 /* Creates a context on one GPU: uploads the fixed-base generator tables to HBM once (the
@@ -99,6 +102,36 @@ int sigops_secp256r1_recover_dev(sigops_ctx* ctx, const void* d_sigs, const void
 /* d_msgs must stay readable for 16 bytes past its last message byte */
 int sigops_ed25519_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
                               const uint64_t* d_msg_off, size_t n, void* d_status, void* stream);
+
+/* ---- mixed-scheme batch: one call for secp256k1 / secp256r1 recoveries and Ed25519 verifications
+   in any order. No single counterpart in the reference (one entry point per scheme, 32-byte
+   messages); item i is processed exactly as the per-scheme entry points would process it.
+   schemes: n bytes (SIGOPS_SCHEME_*); sigs: n x 64; aux: n x 32 (the digest of a secp item, the
+   public key of an Ed25519 item); message of item i = msgs[msg_off[i] .. msg_off[i+1]) (empty for
+   secp items); out: n x 64 (recovered key of a secp item, zeros otherwise); status: n bytes
+   (SIGOPS_ST_* for secp items, SIGOPS_ED_* for Ed25519 items). An unknown scheme byte fails the
+   call with SIGOPS_E_ARG. ---- */
+int sigops_batch(sigops_ctx* ctx, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
+                 const uint8_t* msgs, const uint64_t* msg_off, size_t n, uint8_t* out, uint8_t* status);
+/* device pointers (d_msgs readable 16 bytes past the last message), enqueued on `stream` */
+int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs, const void* d_aux,
+                     const void* d_msgs, const uint64_t* d_msg_off, size_t n, void* d_out, void* d_status,
+                     void* stream);
+
+/* ---- synthetic workload generation on the GPU. No counterpart in the reference, which signs its
+   benchmark inputs serially on the CPU with fuel-crypto / p256 / dalek
+   (src/benchmarks/secp256k1_ecdsa.rs:150-172, src/benchmarks/ed25519_eddsa.rs:140-166); this signs
+   millions of inputs in milliseconds for benchmarks and serving experiments. Item i is a pure
+   function of (seed, i): sk and nonce uniform in [1, n-1], digest 32 uniform bytes, fuel-format
+   low-s signature; Ed25519 keys from a 32-byte seed per item (RFC 8032). corrupt_ppm items (per
+   million) get one uniform bit flipped in r/R, s or the digest/message; d_corrupted (n bytes, may
+   be NULL) marks them. Device pointers, enqueued on `stream`. ---- */
+int sigops_gen_secp_dev(sigops_ctx* ctx, int curve, uint64_t seed, size_t n, uint32_t corrupt_ppm, void* d_sigs,
+                        void* d_digests, void* d_pubs, void* d_corrupted, void* stream);
+/* messages are written at d_msgs + d_msg_off[i] (lengths from the offsets); d_msgs must be 4-byte
+   aligned with 16 readable bytes after the last message */
+int sigops_gen_ed25519_dev(sigops_ctx* ctx, uint64_t seed, size_t n, uint32_t corrupt_ppm, const uint64_t* d_msg_off,
+                           void* d_msgs, void* d_sigs, void* d_pubkeys, void* d_corrupted, void* stream);
 
 #ifdef __cplusplus
 }

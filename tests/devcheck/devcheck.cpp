@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "ed25519.hpp"
+#include "gen.hpp"
 #include "secp.hpp"
 
 using namespace sigops;
@@ -137,5 +138,36 @@ void dc_sha512_ed(const unsigned char* hdr64, const unsigned char* msg, unsigned
   sha512_ed(hdr, mem, 0, len, h);
   memcpy(out, h, 64);
   delete[] buf;
+}
+
+void dc_gen_secp(int curve, unsigned long long seed, unsigned long long i, unsigned ppm, unsigned char* sig,
+                 unsigned char* dig, unsigned char* pub, unsigned* corrupted) {
+  u32 s[16], dg[8], p[16], cor;
+  if (curve == 0) gen_secp_item<CurveK1>(seed, i, ppm, GTabHost{K1_GTABLE}, s, dg, p, cor);
+  else gen_secp_item<CurveR1>(seed, i, ppm, GTabHost{R1_GTABLE}, s, dg, p, cor);
+  memcpy(sig, s, 64);
+  memcpy(dig, dg, 32);
+  memcpy(pub, p, 64);
+  *corrupted = cor;
+}
+
+struct PutHost {
+  unsigned char* p;
+  void operator()(u64 off, u32 v) const { p[off] = (unsigned char)v; }
+};
+
+// msgbuf: 4-byte aligned with slack; generates message bytes then signs, applies the flip
+void dc_gen_ed(unsigned long long seed, unsigned long long i, unsigned ppm, unsigned long long off, unsigned len,
+               unsigned char* msgbuf, unsigned char* sig, unsigned char* pk, unsigned* corrupted) {
+  PutHost put{msgbuf};
+  gen_ed_message(seed, i, off, len, put);
+  MemHost mem{(const u32*)msgbuf};
+  u32 s[16], p[8], cor, mask;
+  u64 foff;
+  gen_ed_item(seed, i, ppm, off, len, BTabHost{}, mem, s, p, cor, foff, mask);
+  if (mask) msgbuf[foff] ^= (unsigned char)mask;
+  memcpy(sig, s, 64);
+  memcpy(pk, p, 32);
+  *corrupted = cor;
 }
 }

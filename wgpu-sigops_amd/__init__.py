@@ -38,7 +38,8 @@ class SigopsError(RuntimeError):
 
 
 class _Config(ctypes.Structure):
-    _fields_ = [("device", ctypes.c_int), ("blocks_per_cu", ctypes.c_uint32), ("max_batch", ctypes.c_uint32)]
+    _fields_ = [("device", ctypes.c_int), ("blocks_per_cu", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("tile_rounds", ctypes.c_uint32), ("no_graph", ctypes.c_uint32)]
 
 
 _lib = None
@@ -74,6 +75,11 @@ def lib():
             for name in ("sigops_secp256k1_recover_dev", "sigops_secp256r1_recover_dev"):
                 getattr(L, name).argtypes = [vp, vp, vp, sz, vp, vp, vp]
             L.sigops_ed25519_verify_dev.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp]
+            L.sigops_batch.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, sz, c_p, c_p]
+            L.sigops_batch_dev.argtypes = [vp, vp, vp, vp, vp, vp, sz, vp, vp, vp]
+            L.sigops_gen_secp_dev.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp,
+                                              vp]
+            L.sigops_gen_ed25519_dev.argtypes = [vp, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
             _lib = L
         return _lib
 
@@ -94,8 +100,8 @@ def _u8(a, shape_tail, name):
 class Context:
     """One GPU context: resident generator tables, per-lane scratch and reusable I/O buffers."""
 
-    def __init__(self, device=0, blocks_per_cu=0, max_batch=0):
-        cfg = _Config(device, blocks_per_cu, max_batch)
+    def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True):
+        cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1)
         self._h = lib().sigops_create(ctypes.byref(cfg))
         if not self._h:
             raise SigopsError("sigops_create failed (no usable GPU?)")
@@ -163,6 +169,35 @@ class Context:
         self._check(lib().sigops_ed25519_verify(self._h, _ptr(sigs), _ptr(pubkeys), _ptr(msgs) if msgs.size else None,
                                                 _ptr(msg_off), n, _ptr(valid), _ptr(st)))
         return valid.astype(bool), st
+
+    def batch(self, schemes, sigs, aux, msgs, msg_off):
+        """mixed batch: (n,) scheme bytes, (n,64) sigs, (n,32) digest-or-key, messages + (n+1) offsets
+        -> ((n,64) recovered keys (zeros for Ed25519 items), (n,) status)"""
+        schemes = np.ascontiguousarray(schemes, dtype=np.uint8).reshape(-1)
+        sigs = _u8(sigs, 64, "sigs")
+        aux = _u8(aux, 32, "aux")
+        n = schemes.shape[0]
+        msg_off = np.ascontiguousarray(msg_off, dtype=np.uint64)
+        if sigs.shape[0] != n or aux.shape[0] != n or msg_off.shape[0] != n + 1:
+            raise ValueError("schemes, sigs, aux and msg_off (n+1) differ in length")
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint8).reshape(-1)
+        out = np.zeros((n, 64), np.uint8)
+        st = np.zeros(n, np.uint8)
+        self._check(lib().sigops_batch(self._h, _ptr(schemes), _ptr(sigs), _ptr(aux), _ptr(msgs) if msgs.size else None,
+                                       _ptr(msg_off), n, _ptr(out), _ptr(st)))
+        return out, st
+
+    def batch_dev(self, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status, stream=0):
+        self._check(lib().sigops_batch_dev(self._h, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status,
+                                           stream or None))
+
+    def gen_secp_dev(self, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs, d_corrupted=None, stream=0):
+        self._check(lib().sigops_gen_secp_dev(self._h, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs,
+                                              d_corrupted, stream or None))
+
+    def gen_ed25519_dev(self, seed, n, corrupt_ppm, d_msg_off, d_msgs, d_sigs, d_pubkeys, d_corrupted=None, stream=0):
+        self._check(lib().sigops_gen_ed25519_dev(self._h, seed, n, corrupt_ppm, d_msg_off, d_msgs, d_sigs, d_pubkeys,
+                                                 d_corrupted, stream or None))
 
     # device-resident variants (torch tensors or raw ints as device pointers)
     def secp_recover_dev(self, curve, d_sigs, d_digests, n, d_out, d_status, stream=0):

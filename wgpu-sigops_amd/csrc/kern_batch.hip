@@ -1,0 +1,251 @@
+// kern_batch.hip -- mixed-scheme batch: device-side bucketing plus one persistent kernel.
+//
+// A batch mixes secp256k1 recoveries, secp256r1 recoveries and Ed25519 verifications with messages
+// of any length. The reference has one kernel chain per scheme and fixed 32-byte messages; here:
+//   1. k_bucket_hist / k_bucket_scan / k_bucket_scatter sort item indices into buckets (k1, r1, and
+//      Ed25519 by SHA-512 block count, so the lanes of a wavefront hash equally long messages), each
+//      bucket padded to whole tiles; tiles are ordered by estimated cost, longest first;
+//   2. k_batch_run is persistent: every wavefront pulls the next tile (64 x tile_rounds items of
+//      one bucket) from an atomic counter, so the scheme is uniform across the wavefront and load
+//      balances dynamically; it runs the same per-item code as the plain kernels;
+//   3. k_batch_careful finishes secp items whose fast ladder met an exceptional addition.
+// The n of the batch is read from device memory, so the captured graph of these launches replays
+// for any n up to the context capacity.
+#include "kernels_impl.hpp"
+
+namespace sigops {
+
+constexpr u32 DUMMY = 0xFFFFFFFFu;
+
+struct BatchArgs {
+  const u8* scheme;
+  const uint4* sigs;
+  const uint4* aux;  // digest (secp) or public key (Ed25519), 32 bytes per item
+  const u32* msgs;
+  const u64* msg_off;
+  uint4* out;
+  u8* status;
+  const u32* n_ptr;
+  u32* counts;      // [NB]
+  u32* cursor;      // [NB]
+  u32* tile_start;  // [NB + 1] in processing order
+  u32* order;       // padded item lists, bucket-major in processing order
+  u32* tile_counter;
+  u32* pend_list;
+  u32* pend_count;
+  const u8* rank;   // bucket -> processing rank
+  u32 tile_items;   // 64 * tile_rounds
+};
+
+DEV u32 item_bucket(const BatchArgs& a, u32 i) {
+  const u32 s = a.scheme[i];
+  if (s == 0) return BUCKET_K1;
+  if (s == 1) return BUCKET_R1;
+  const u64 len = a.msg_off[i + 1] - a.msg_off[i];
+  u64 nb = (64 + len + 17 + 127) >> 7;
+  if (nb > ED_BINS) nb = ED_BINS;
+  return BUCKET_ED0 + (u32)nb - 1;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_hist(BatchArgs a) {
+  __shared__ u32 h[NBUCKETS];
+  for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x) h[t] = 0;
+  __syncthreads();
+  const u32 n = *a.n_ptr;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (a.scheme[i] > 2) continue;
+    atomicAdd(&h[item_bucket(a, i)], 1u);
+  }
+  __syncthreads();
+  for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x)
+    if (h[t]) atomicAdd(&a.counts[t], h[t]);
+}
+
+// one block: tile offsets in processing order, cursors, and DUMMY padding of each bucket's tail
+__global__ __launch_bounds__(64) void k_bucket_scan(BatchArgs a) {
+  __shared__ u32 start[NBUCKETS];
+  if (threadIdx.x == 0) {
+    u32 pos_tiles = 0;
+    for (u32 r = 0; r < NBUCKETS; r++) {
+      u32 b = 0;
+      for (u32 q = 0; q < NBUCKETS; q++)
+        if (a.rank[q] == r) b = q;
+      const u32 c = a.counts[b];
+      const u32 tiles = (c + a.tile_items - 1) / a.tile_items;
+      a.tile_start[r] = pos_tiles;
+      start[b] = pos_tiles * a.tile_items;
+      pos_tiles += tiles;
+    }
+    a.tile_start[NBUCKETS] = pos_tiles;
+  }
+  __syncthreads();
+  for (u32 b = 0; b < NBUCKETS; b++) {
+    const u32 c = a.counts[b];
+    const u32 padded = (c + a.tile_items - 1) / a.tile_items * a.tile_items;
+    for (u32 q = c + threadIdx.x; q < padded; q += blockDim.x) a.order[start[b] + q] = DUMMY;
+    if (threadIdx.x == 0) a.cursor[b] = start[b];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
+  const u32 n = *a.n_ptr;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (a.scheme[i] > 2) {
+      a.status[i] = 0xFD;  // unknown scheme: reported as a canary failure by the host
+      continue;
+    }
+    const u32 pos = atomicAdd(&a.cursor[item_bucket(a, i)], 1u);
+    a.order[pos] = i;
+  }
+}
+
+DEV void store16(uint4* p, const u32 o[16]) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) p[c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
+}
+
+template <class C, class GT>
+DEV void batch_secp_item(const BatchArgs& a, u32 i, JTabDev& tab, const GT& gt) {
+  u32 sr[16], dr[8], o[16];
+  load_words(a.sigs + 4 * (u64)i, sr, 4);
+  load_words(a.aux + 2 * (u64)i, dr, 2);
+  secp_pro pro;
+  secp_prologue<C>(sr, dr, pro);
+  u32 st = pro.status;
+  if (st == ST_OK) {
+    bool exc = false;
+    const jac Q = secp_ladder<C>(pro, tab, gt, exc);
+    if (exc) {
+      st = ST_PENDING;
+      a.pend_list[atomicAdd(a.pend_count, 1u)] = i;
+    } else {
+      secp_output<C>(Q, o);
+    }
+  }
+  if (st != ST_OK) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) o[k] = 0;
+  }
+  store16(a.out + 4 * (u64)i, o);
+  a.status[i] = (u8)st;
+}
+
+__global__ __launch_bounds__(256) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
+                                                   const uint4* __restrict__ gr1, const uint4* __restrict__ gb,
+                                                   uint4* __restrict__ lane_tab, u32 slots) {
+  __shared__ uint4 s_k1[GTAB_ENTRIES * 5];
+  __shared__ uint4 s_r1[GTAB_ENTRIES * 5];
+  __shared__ uint4 s_b[BTAB_ENTRIES * 7];
+  for (int t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) {
+    s_k1[t] = gk1[t];
+    s_r1[t] = gr1[t];
+  }
+  for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
+  __syncthreads();
+  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 lane = threadIdx.x & 63u;
+  JTabDev jt;
+  jt.base = lane_tab;
+  jt.slots = slots;
+  jt.slot = slot;
+  CTabDev ct;
+  ct.base = lane_tab;
+  ct.slots = slots;
+  ct.slot = slot;
+  const u32 ntiles = a.tile_start[NBUCKETS];
+  const MemDev mem{a.msgs};
+  for (;;) {
+    u32 t = 0;
+    if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
+    t = __shfl(t, 0);
+    if (t >= ntiles) break;
+    u32 r = 0;
+    while (r + 1 < NBUCKETS && a.tile_start[r + 1] <= t) r++;
+    u32 b = 0;
+    for (u32 q = 0; q < NBUCKETS; q++)
+      if (a.rank[q] == r) b = q;
+    b = __builtin_amdgcn_readfirstlane(b);
+    for (u32 k = 0; k < a.tile_items; k += 64) {
+      const u32 i = a.order[(u64)t * a.tile_items + k + lane];
+      if (i == DUMMY) continue;
+      if (b == BUCKET_K1) {
+        batch_secp_item<CurveK1>(a, i, jt, GTabLds{s_k1});
+      } else if (b == BUCKET_R1) {
+        batch_secp_item<CurveR1>(a, i, jt, GTabLds{s_r1});
+      } else {
+        u32 sr[16], pr[8];
+        load_words(a.sigs + 4 * (u64)i, sr, 4);
+        load_words(a.aux + 2 * (u64)i, pr, 2);
+        const u64 off = a.msg_off[i];
+        const u32 len = (u32)(a.msg_off[i + 1] - off);
+        a.status[i] = (u8)ed25519_verify_item(sr, pr, mem, off, len, ct, BTabLds{s_b});
+        const u32 zero[16] = {0};
+        store16(a.out + 4 * (u64)i, zero);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_batch_careful(BatchArgs a, const uint4* __restrict__ gk1,
+                                                      const uint4* __restrict__ gr1) {
+  const u32 cnt = *a.pend_count;
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const u32 i = a.pend_list[j];
+    u32 sr[16], dr[8], o[16];
+    load_words(a.sigs + 4 * (u64)i, sr, 4);
+    load_words(a.aux + 2 * (u64)i, dr, 2);
+    secp_pro pro;
+    jacf Q;
+    if (a.scheme[i] == 0) {
+      secp_prologue<CurveK1>(sr, dr, pro);
+      Q = secp_ladder_careful<CurveK1>(pro, GTabGlobal{gk1});
+      if (!Q.inf) secp_output<CurveK1>(Q.P, o);
+    } else {
+      secp_prologue<CurveR1>(sr, dr, pro);
+      Q = secp_ladder_careful<CurveR1>(pro, GTabGlobal{gr1});
+      if (!Q.inf) secp_output<CurveR1>(Q.P, o);
+    }
+    if (Q.inf) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) o[k] = 0;
+    }
+    store16(a.out + 4 * (u64)i, o);
+    a.status[i] = (u8)(Q.inf ? ST_INFINITY : ST_OK);
+  }
+}
+
+static __global__ void k_zero_u32(u32* p, u32 n) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
+  BatchArgs a;
+  a.scheme = (const u8*)L.scheme;
+  a.sigs = (const uint4*)L.sigs;
+  a.aux = (const uint4*)L.aux;
+  a.msgs = (const u32*)L.msgs;
+  a.msg_off = L.msg_off;
+  a.out = (uint4*)L.out;
+  a.status = (u8*)L.status;
+  a.n_ptr = L.n_ptr;
+  a.counts = L.work;                    // NBUCKETS
+  a.cursor = L.work + NBUCKETS;         // NBUCKETS
+  a.tile_start = L.work + 2 * NBUCKETS;  // NBUCKETS + 1
+  a.tile_counter = L.work + 3 * NBUCKETS + 1;
+  a.pend_count = L.work + 3 * NBUCKETS + 2;
+  a.pend_list = L.pend_list;
+  a.order = L.order;
+  a.rank = L.rank;
+  a.tile_items = 64u * L.tile_rounds;
+  // counters are cleared with a kernel (not a memset node) so the sequence is safe to capture
+  k_zero_u32<<<1, 256, 0, s>>>(L.work, BATCH_WORK_WORDS);
+  k_bucket_hist<<<L.aux_grid, 256, 0, s>>>(a);
+  k_bucket_scan<<<1, 64, 0, s>>>(a);
+  k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
+  k_batch_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+                                     (uint4*)L.lane_tab, L.slots);
+  k_batch_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);
+  return hipGetLastError();
+}
+
+}  // namespace sigops

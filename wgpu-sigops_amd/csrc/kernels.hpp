@@ -23,4 +23,38 @@ hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* 
                                  void* status, uint32_t n, const void* btab, void* ctab, uint32_t slots,
                                  uint32_t grid, hipStream_t s);
 
+// ---- mixed batch ----
+constexpr uint32_t BUCKET_K1 = 0, BUCKET_R1 = 1, BUCKET_ED0 = 2;
+constexpr uint32_t ED_BINS = 40;  // Ed25519 buckets by SHA-512 block count 1..40 (40 = "40 or more")
+constexpr uint32_t NBUCKETS = 2 + ED_BINS;
+constexpr uint32_t BATCH_WORK_WORDS = 3 * NBUCKETS + 3;  // counts, cursors, tile starts, tile ctr, pending ctr
+constexpr uint32_t MAX_TILE_ROUNDS = 4;
+
+struct BatchLaunch {
+  const void* scheme;
+  const void* sigs;
+  const void* aux;
+  const void* msgs;
+  const uint64_t* msg_off;
+  void* out;
+  void* status;
+  const uint32_t* n_ptr;
+  uint32_t* work;       // BATCH_WORK_WORDS
+  uint32_t* order;      // capacity max_batch + NBUCKETS * 64 * tile_rounds
+  uint32_t* pend_list;  // capacity max_batch
+  const uint8_t* rank;  // NBUCKETS
+  const void* gk1;
+  const void* gr1;
+  const void* gb;
+  void* lane_tab;
+  uint32_t slots, grid, aux_grid, tile_rounds;
+};
+
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
+
+hipError_t launch_gen_secp(int curve, uint64_t seed, uint32_t n, uint32_t ppm, const void* gtab, void* sigs,
+                           void* digs, void* pubs, void* corrupted, uint32_t grid, hipStream_t s);
+hipError_t launch_gen_ed25519(uint64_t seed, uint32_t n, uint32_t ppm, const void* btab, const uint64_t* msg_off,
+                              void* msgs, void* sigs, void* pks, void* corrupted, uint32_t grid, hipStream_t s);
+
 }  // namespace sigops

@@ -55,14 +55,15 @@ DEV u64 load8_le(const Mem& mem, u64 off) {
   return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
 }
 
-// SHA-512(hdr64 || msg[0..len)) where hdr is 16 raw u32 words (R then A, as loaded from memory).
-// Output: the 64 digest bytes as 16 little-endian u32 words.
-template <class Mem>
-DEV void sha512_ed(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 out[16]) {
+// SHA-512(hdr[0..HL) || msg[0..len)) for a header of HL = 32 or 64 bytes, given as 16 raw u32
+// words (only the first HL/4 are used). Output: the 64 digest bytes as 16 little-endian u32 words.
+template <int HL, class Mem>
+DEV void sha512_hdr(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 out[16]) {
+  static_assert(HL == 32 || HL == 64, "header is 32 or 64 bytes");
   u64 h[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) h[i] = SHA512_IV[i];
-  const u64 total = 64ull + len;
+  const u64 total = (u64)HL + len;
   const u32 nblk = (u32)((total + 1 + 16 + 127) >> 7);
   for (u32 b = 0; b < nblk; b++) {
     u64 w[16];
@@ -70,11 +71,11 @@ DEV void sha512_ed(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 
     for (int q = 0; q < 16; q++) {
       const u64 o = (u64)b * 128 + 8 * q;  // byte offset in the preimage
       u64 word;
-      if (o < 64) {
+      if (o < (u64)HL) {
         const int hq = (int)(o >> 3);
         word = bswap64((u64)hdr[2 * hq] | ((u64)hdr[2 * hq + 1] << 32));
       } else {
-        const u64 mo = o - 64;
+        const u64 mo = o - HL;
         u64 le = 0;
         if (mo < len) le = load8_le(mem, msg_off + mo);
         const i64 keep = (i64)len - (i64)mo;  // message bytes in this word
@@ -96,6 +97,12 @@ DEV void sha512_ed(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 
     out[2 * i] = (u32)le;
     out[2 * i + 1] = (u32)(le >> 32);
   }
+}
+
+// SHA-512(R || A || M): hdr = R then A (16 raw words)
+template <class Mem>
+DEV void sha512_ed(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 out[16]) {
+  sha512_hdr<64>(hdr, mem, msg_off, len, out);
 }
 
 }  // namespace sigops

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
+#include <tuple>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,7 +40,17 @@ struct sigops_ctx {
   void* d_btab = nullptr;
   void* d_lane_tab = nullptr;
   uint32_t* d_pend = nullptr;  // [0] = count, [1 ..] = list
-  DevBuf in, aux, msg, off, out, st;
+  // mixed batch
+  uint32_t tile_rounds = 1;
+  bool use_graph = true;
+  uint8_t* d_rank = nullptr;
+  uint32_t* d_work = nullptr;
+  uint32_t* d_order = nullptr;
+  uint32_t* d_n = nullptr;
+  std::map<std::tuple<const void*, const void*, const void*, const void*, const void*, void*, void*>,
+           hipGraphExec_t>
+      graphs;
+  DevBuf in, aux, msg, off, out, st, sch;
   std::vector<uint8_t> host_status;
   std::vector<uint64_t> host_off;
   std::string err;
@@ -77,6 +89,8 @@ __global__ void k_fill_bytes(uint8_t* p, uint8_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 int secp_recover_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_digs, size_t n, void* d_out,
                      void* d_status, hipStream_t s) {
@@ -162,6 +176,28 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
   if ((e = hipMalloc(&c->d_lane_tab, lane_bytes)) != hipSuccess) return bad(e, "hipMalloc lane tables");
   if ((e = hipMalloc((void**)&c->d_pend, ((size_t)c->max_batch + 1) * sizeof(uint32_t))) != hipSuccess)
     return bad(e, "hipMalloc pending list");
+  if (cfg->tile_rounds) c->tile_rounds = std::min<uint32_t>(cfg->tile_rounds, sigops::MAX_TILE_ROUNDS);
+  c->use_graph = cfg->no_graph == 0;
+  {
+    // processing order of the buckets: estimated cost per item, most expensive first (an Ed25519
+    // verification costs ~0.85 of a recovery plus ~0.6% per SHA-512 block)
+    std::vector<std::pair<double, uint32_t>> cost;
+    cost.push_back({1.0, sigops::BUCKET_K1});
+    cost.push_back({1.01, sigops::BUCKET_R1});
+    for (uint32_t b = 0; b < sigops::ED_BINS; b++) cost.push_back({0.85 + 0.006 * (b + 1), sigops::BUCKET_ED0 + b});
+    std::stable_sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
+    uint8_t rank[sigops::NBUCKETS];
+    for (uint32_t r = 0; r < sigops::NBUCKETS; r++) rank[cost[r].second] = (uint8_t)r;
+    if ((e = hipMalloc((void**)&c->d_rank, sizeof rank)) != hipSuccess) return bad(e, "hipMalloc rank");
+    if ((e = hipMemcpy(c->d_rank, rank, sizeof rank, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "upload");
+  }
+  if ((e = hipMalloc((void**)&c->d_work, sigops::BATCH_WORK_WORDS * sizeof(uint32_t))) != hipSuccess)
+    return bad(e, "hipMalloc batch work");
+  if ((e = hipMalloc((void**)&c->d_order, ((size_t)c->max_batch + (size_t)sigops::NBUCKETS * 64 *
+                                                                      sigops::MAX_TILE_ROUNDS) * sizeof(uint32_t))) !=
+      hipSuccess)
+    return bad(e, "hipMalloc batch order");
+  if ((e = hipMalloc((void**)&c->d_n, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc n");
   return c;
 }
 
@@ -169,8 +205,10 @@ void sigops_destroy(sigops_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
-                  c->msg.p, c->off.p, c->out.p, c->st.p})
+                  c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
+                  (void*)c->d_order, (void*)c->d_n})
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -249,6 +287,132 @@ int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
   for (size_t i = 0; i < n; i++) {
     if (st[i] > SIGOPS_ED_MISMATCH) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
     valid[i] = st[i] == SIGOPS_ED_OK;
+  }
+  return SIGOPS_OK;
+}
+
+int sigops_gen_secp_dev(sigops_ctx* c, int curve, uint64_t seed, size_t n, uint32_t corrupt_ppm, void* d_sigs,
+                        void* d_digests, void* d_pubs, void* d_corrupted, void* stream) {
+  if (!c || (curve != 0 && curve != 1)) return SIGOPS_E_ARG;
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (n && (!d_sigs || !d_digests || !d_pubs)) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  HIPCHK(c, sigops::launch_gen_secp(curve, seed, (uint32_t)n, corrupt_ppm, curve == 0 ? c->d_gtab_k1 : c->d_gtab_r1,
+                                    d_sigs, d_digests, d_pubs, d_corrupted, c->grid, (hipStream_t)stream));
+  return SIGOPS_OK;
+}
+
+int sigops_gen_ed25519_dev(sigops_ctx* c, uint64_t seed, size_t n, uint32_t corrupt_ppm, const uint64_t* d_msg_off,
+                           void* d_msgs, void* d_sigs, void* d_pubkeys, void* d_corrupted, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (n && (!d_msg_off || !d_msgs || !d_sigs || !d_pubkeys)) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (((uintptr_t)d_msgs & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_msgs must be 4-byte aligned");
+  HIPCHK(c, sigops::launch_gen_ed25519(seed, (uint32_t)n, corrupt_ppm, c->d_btab, d_msg_off, d_msgs, d_sigs, d_pubkeys,
+                                       d_corrupted, c->grid, (hipStream_t)stream));
+  return SIGOPS_OK;
+}
+
+int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, const void* d_aux, const void* d_msgs,
+                     const uint64_t* d_msg_off, size_t n, void* d_out, void* d_status, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (n == 0) return SIGOPS_OK;
+  if (!d_schemes || !d_sigs || !d_aux || !d_msg_off || !d_out || !d_status) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  sigops::BatchLaunch L;
+  L.scheme = d_schemes;
+  L.sigs = d_sigs;
+  L.aux = d_aux;
+  L.msgs = d_msgs;
+  L.msg_off = d_msg_off;
+  L.out = d_out;
+  L.status = d_status;
+  L.n_ptr = c->d_n;
+  L.work = c->d_work;
+  L.order = c->d_order;
+  L.pend_list = c->d_pend + 1;
+  L.rank = c->d_rank;
+  L.gk1 = c->d_gtab_k1;
+  L.gr1 = c->d_gtab_r1;
+  L.gb = c->d_btab;
+  L.lane_tab = c->d_lane_tab;
+  L.slots = c->slots;
+  L.grid = c->grid;
+  L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
+  L.tile_rounds = c->tile_rounds;
+  // n goes to device memory first (stream-ordered, by value at enqueue time), so one captured graph
+  // serves every n and back-to-back enqueues do not race
+  k_set_u32<<<1, 1, 0, s>>>(c->d_n, (uint32_t)n);
+  HIPCHK(c, hipGetLastError());
+  if (!c->use_graph) {
+    HIPCHK(c, sigops::launch_batch(L, s));
+    return SIGOPS_OK;
+  }
+  L.aux_grid = c->grid;  // the graph is shape-independent of n
+  auto key = std::make_tuple(d_schemes, d_sigs, d_aux, d_msgs, (const void*)d_msg_off, d_out, d_status);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = sigops::launch_batch(L, c->stream);
+    hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (le != hipSuccess) return hip_fail(c, le, "launch_batch under capture");
+    if (ce != hipSuccess) return hip_fail(c, ce, "hipStreamEndCapture");
+    hipGraphExec_t ex;
+    hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return hip_fail(c, ie, "hipGraphInstantiate");
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIPCHK(c, hipGraphLaunch(it->second, s));
+  return SIGOPS_OK;
+}
+
+int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux, const uint8_t* msgs,
+                 const uint64_t* msg_off, size_t n, uint8_t* out, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  if (n == 0) return SIGOPS_OK;
+  if (!schemes || !sigs || !aux || !msg_off || !out) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  for (size_t i = 0; i < n; i++)
+    if (schemes[i] > SIGOPS_SCHEME_ED25519) return fail(c, SIGOPS_E_ARG, "unknown scheme byte");
+  const uint64_t base = msg_off[0];
+  for (size_t i = 0; i < n; i++)
+    if (msg_off[i + 1] < msg_off[i] || msg_off[i + 1] - msg_off[i] > 0xFFFFFFFFull)
+      return fail(c, SIGOPS_E_ARG, "msg_off must be non-decreasing with lengths < 2^32");
+  const uint64_t total = msg_off[n] - base;
+  if (total && !msgs) return fail(c, SIGOPS_E_ARG, "NULL msgs");
+  int rc;
+  if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) || (rc = ensure(c, c->msg, total + 16)) ||
+      (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->out, n * 64)) || (rc = ensure(c, c->st, n)) ||
+      (rc = ensure(c, c->sch, n)))
+    return rc;
+  c->host_off.resize(n + 1);
+  for (size_t i = 0; i <= n; i++) c->host_off[i] = msg_off[i] - base;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->sch.p, schemes, n, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->aux.p, aux, n * 32, hipMemcpyHostToDevice, s));
+  if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, msgs + base, total, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
+  HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = sigops_batch_dev(c, c->sch.p, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->out.p,
+                             c->st.p, s)))
+    return rc;
+  uint8_t* st = status;
+  if (!st) {
+    c->host_status.resize(n);
+    st = c->host_status.data();
+  }
+  HIPCHK(c, hipMemcpyAsync(out, c->out.p, n * 64, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; i++) {
+    const uint8_t lim = schemes[i] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY;
+    if (st[i] > lim) return fail(c, SIGOPS_E_CANARY, "a batch result was not written");
   }
   return SIGOPS_OK;
 }
