@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X signature engine (one GPU).
+
+Workload (the headline configuration): a mixed batch of 2^20 signatures -- 50 % secp256k1
+recoveries, 30 % secp256r1 recoveries, 20 % Ed25519 verifications with message lengths
+log-uniform on 0..4096 bytes, schemes randomly interleaved, 5 % corrupted -- generated on the GPU
+by the engine's seeded signer and resident in HBM when the timed region starts.
+
+One step = one batched, graph-captured pass over the whole batch (device-side bucketing + one
+persistent kernel + careful pass). `value` is signatures per second over the K timed steps.
+Also reported, each on the same mixed workload:
+  * plain: the three per-scheme kernels (gather by scheme, one launch per scheme, scatter back),
+    and the batched/plain ratio;
+  * transfers: the batched path through the host API (H2D + graph + D2H), the reference's
+    benchmark method (src/benchmarks/secp256k1_ecdsa.rs:126-138);
+  * roofline: algorithmic u32 multiply-accumulates per signature (canonical schedule, see
+    DESIGN.md) divided by the launch time, against the measured v_mad_u64_u32 issue rate;
+  * cpu_baseline: the CPU oracle timed on a bounded sample of the same items on this host's cores,
+    also used to check the GPU outputs of that sample bit for bit.
+Prints ONE JSON line.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+# canonical multiply-accumulates per signature (u32 x u32 -> u64), see DESIGN.md section "Roofline"
+MAC_PER_SIG = {"secp256k1": 595072, "secp256r1": 627456, "ed25519_per_block": 24 * 128, "ed25519_base": 3850 * 128}
+LIMBS_MAC = 128  # MACs per canonical 256-bit modular multiplication
+
+
+def mixed_mac_per_sig(schemes, lens):
+    """average canonical MACs per signature of this exact batch"""
+    k1 = (schemes == 0).sum()
+    r1 = (schemes == 1).sum()
+    ed = schemes == 2
+    blocks = ((64 + lens[ed] + 17 + 127) // 128).sum()
+    total = k1 * MAC_PER_SIG["secp256k1"] + r1 * MAC_PER_SIG["secp256r1"] + \
+        ed.sum() * MAC_PER_SIG["ed25519_base"] + blocks * MAC_PER_SIG["ed25519_per_block"]
+    return float(total) / len(schemes)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--tile-rounds", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every signature is independent, "
+              "so more GPUs would shard the batch without any collective (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    sigops = importlib.import_module("wgpu-sigops_amd")
+    W = importlib.import_module("wgpu-sigops_amd.workload")
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n = args.n
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    mad_per_s, fma_per_s = sigops.probe_valu(0)
+    log(f"probe: v_mad_u64_u32 {mad_per_s / 1e12:.2f} T/s, v_fma_f64 {fma_per_s / 1e12:.2f} T/s")
+    ctx = sigops.Context(blocks_per_cu=args.blocks_per_cu, tile_rounds=args.tile_rounds, max_batch=max(n, 1 << 22))
+    t0 = time.time()
+    mx = W.mixed(ctx, n, seed=args.seed)
+    torch.cuda.synchronize()
+    log(f"workload: {n} mixed items generated on the GPU in {time.time() - t0:.2f} s "
+        f"({int(mx.msg_off_host[-1])} message bytes)")
+    stream = torch.cuda.current_stream()
+    s_ptr = stream.cuda_stream
+    out = torch.zeros((n, 64), dtype=torch.uint8, device=dev)
+    st = torch.zeros(n, dtype=torch.uint8, device=dev)
+
+    def batched_step():
+        ctx.batch_dev(mx.schemes.data_ptr(), mx.sigs.data_ptr(), mx.aux.data_ptr(), mx.msgs.data_ptr(),
+                      mx.msg_off.data_ptr(), n, out.data_ptr(), st.data_ptr(), s_ptr)
+
+    for _ in range(args.warmup):
+        batched_step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    ev[0].record(stream)
+    for k in range(args.steps):
+        batched_step()
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
+    total_ms = ev[0].elapsed_time(ev[-1])
+    ms_per_step = total_ms / args.steps
+    value = n / (ms_per_step / 1e3)
+    log(f"batched: {ms_per_step:.2f} ms/step, {value / 1e6:.2f} M sigs/s (steps {min(step_ms):.2f}..{max(step_ms):.2f})")
+
+    # correctness of the timed output by the round-trip property (all 2^20 items)
+    out_h, st_h, cor = out.cpu().numpy(), st.cpu().numpy(), mx.corrupted.cpu().numpy()
+    sch = mx.schemes_host
+    roundtrip_ok = True
+    for s in (0, 1):
+        ii = mx.idx[s]
+        pubs = mx.parts[s].pubs.cpu().numpy()
+        ok = cor[ii] == 0
+        roundtrip_ok &= bool(np.array_equal(out_h[ii][ok], pubs[ok]) and not st_h[ii][ok].any())
+    roundtrip_ok &= bool(np.array_equal(st_h[mx.idx[2]] == 0, cor[mx.idx[2]] == 0))
+
+    # plain per-scheme kernels on the same mixed workload
+    plain = None
+    if not args.no_plain:
+        d_idx = mx.d_idx
+        parts = []
+        for s, di in enumerate(d_idx):
+            m = len(di)
+            if s < 2:
+                parts.append(dict(sigs=mx.sigs[di].contiguous(), aux=mx.aux[di].contiguous(),
+                                  out=torch.zeros((m, 64), dtype=torch.uint8, device=dev),
+                                  st=torch.zeros(m, dtype=torch.uint8, device=dev)))
+            else:
+                ed = mx.parts[2]
+                parts.append(dict(sigs=mx.sigs[di].contiguous(), aux=mx.aux[di].contiguous(), msgs=ed.msgs,
+                                  off=ed.msg_off, st=torch.zeros(m, dtype=torch.uint8, device=dev)))
+
+        def plain_step(gather):
+            if gather:  # the mixed layout must be split by scheme first, and merged back after
+                for s, di in enumerate(d_idx):
+                    parts[s]["sigs"].copy_(mx.sigs[di])
+                    parts[s]["aux"].copy_(mx.aux[di])
+            for s in (0, 1):
+                p = parts[s]
+                ctx.secp_recover_dev(s, p["sigs"].data_ptr(), p["aux"].data_ptr(), len(d_idx[s]), p["out"].data_ptr(),
+                                     p["st"].data_ptr(), s_ptr)
+            p = parts[2]
+            ctx.ed25519_verify_dev(p["sigs"].data_ptr(), p["aux"].data_ptr(), p["msgs"].data_ptr(), p["off"].data_ptr(),
+                                   len(d_idx[2]), p["st"].data_ptr(), s_ptr)
+            if gather:
+                for s in (0, 1):
+                    out[d_idx[s]] = parts[s]["out"]
+                    st[d_idx[s]] = parts[s]["st"]
+                st[d_idx[2]] = parts[2]["st"]
+
+        res = {}
+        for gather in (False, True):
+            for _ in range(max(1, args.warmup)):
+                plain_step(gather)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                plain_step(gather)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            res[gather] = e0.elapsed_time(e1) / args.steps
+        plain = dict(ms_per_step_kernels_only=res[False], ms_per_step_with_split=res[True],
+                     sigs_per_s_kernels_only=n / (res[False] / 1e3), sigs_per_s_with_split=n / (res[True] / 1e3),
+                     batched_over_plain_kernels_only=res[False] / ms_per_step,
+                     batched_over_plain_with_split=res[True] / ms_per_step)
+        log(f"plain: {res[False]:.2f} ms kernels only, {res[True]:.2f} ms with split/merge; "
+            f"batched/plain = {res[False] / ms_per_step:.2f}x / {res[True] / ms_per_step:.2f}x")
+
+    # transfers included: host arrays in, host arrays out (pageable memory, as a caller has it)
+    h_sch, h_sigs, h_aux = mx.schemes_host, mx.sigs.cpu().numpy(), mx.aux.cpu().numpy()
+    h_msgs = mx.msgs.cpu().numpy()[: int(mx.msg_off_host[-1])]
+    h_off = mx.msg_off_host.astype(np.uint64)
+    ctx.batch(h_sch, h_sigs, h_aux, h_msgs, h_off)
+    t0 = time.perf_counter()
+    reps = max(1, min(3, args.steps))
+    for _ in range(reps):
+        h_out, h_st = ctx.batch(h_sch, h_sigs, h_aux, h_msgs, h_off)
+    incl_ms = (time.perf_counter() - t0) / reps * 1e3
+    incl_ok = bool(np.array_equal(h_out, out_h) and np.array_equal(h_st, st_h))
+    log(f"with transfers: {incl_ms:.2f} ms per batch, {n / incl_ms * 1e3 / 1e6:.2f} M sigs/s")
+
+    # roofline: dominant kernel = the persistent batch kernel; its launch time is the step time
+    # less the small bucketing kernels (measured by rocprofv3 in profiles/), bounded here by the step
+    mac_sig = mixed_mac_per_sig(sch, mx.lens)
+    achieved = value * mac_sig / 1e12
+    peak = mad_per_s / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
+                "unit": "TMAC/s (u32 x u32 -> u64 multiply-accumulate)", "frac": round(achieved / peak, 4),
+                "traffic": None, "mac_per_sig": round(mac_sig), "peak_source": "sigops_probe_valu (v_mad_u64_u32)"}
+
+    # CPU baseline: the oracle on a bounded sample of the same items (rank 0, N=1)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as O
+
+        threads = min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(1)
+        sample = np.sort(rng.choice(n, min(args.cpu_sample, n), replace=False))
+        t0 = time.perf_counter()
+        exact = True
+        for s in (0, 1):
+            ss = sample[sch[sample] == s]
+            o, t = O.recover_batch(s, h_sigs[ss], h_aux[ss], nthreads=threads)
+            exact &= bool(np.array_equal(o, out_h[ss]) and np.array_equal(t, st_h[ss]))
+        ss = sample[sch[sample] == 2]
+        lens = h_off[ss + 1] - h_off[ss]
+        sub_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+        sub_msgs = np.concatenate([h_msgs[h_off[i]:h_off[i + 1]] for i in ss]) if len(ss) else np.zeros(0, np.uint8)
+        v = O.verify_batch(h_sigs[ss], h_aux[ss], sub_msgs, sub_off, nthreads=threads)
+        exact &= bool(np.array_equal(v.astype(bool), st_h[ss] == 0))
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(len(sample) / cpu_s, 1), "unit": "sigs/s", "cores": threads, "kind": "port",
+               "sample": f"{len(sample)} items drawn uniformly from the same 2^20 mixed batch, C oracle "
+                         f"(oracle/), {threads} threads; outputs also compared with the GPU's",
+               "bit_exact_vs_gpu": exact}
+        log(f"cpu oracle: {cpu['value']:.0f} sigs/s on {threads} threads; GPU sample bit-exact: {exact}")
+
+    line = {
+        "metric": "sigs/sec at 2^20 (batched vs plain, bit-exact) + p50/p99 latency served",
+        "value": round(value, 1),
+        "unit": "sigs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded, signed on the GPU)",
+        "config": {"workload": "mixed 2^20: 50% secp256k1 recover, 30% secp256r1 recover, 20% ed25519 verify "
+                               "(messages log-uniform 0..4096 B), interleaved, 5% corrupted; one graph-captured "
+                               "persistent launch per step",
+                   "n": n, "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "incl_transfers": {"ms_per_batch": round(incl_ms, 3), "sigs_per_s": round(n / incl_ms * 1e3, 1),
+                           "identical_to_device_path": incl_ok},
+        "roundtrip_all_items_ok": roundtrip_ok,
+        "probe": {"mad_per_s": mad_per_s, "fma64_per_s": fma_per_s},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

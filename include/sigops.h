@@ -118,6 +118,44 @@ int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs,
                      const void* d_msgs, const uint64_t* d_msg_off, size_t n, void* d_out, void* d_status,
                      void* stream);
 
+/* ---- serving: continuous batching of verification requests. No counterpart in the reference.
+   Requests (each a small mixed batch in the sigops_batch layout) are queued on submit and packed,
+   in arrival order, into the next free slot (a context with its own stream and scratch) while the
+   batch stays within max_batch items and all batches in flight within the HBM budget; arrivals
+   during a launch go out with the next batch, and consecutive batches overlap on the device.
+   Thread-safe: any thread may submit or wait. ---- */
+typedef struct sigops_server sigops_server;
+typedef struct {
+  int device;
+  uint32_t slots;          /* batches in flight at once (0: 2) */
+  uint64_t hbm_budget;     /* device bytes for the request data of all batches in flight (0: 1 GiB) */
+  uint32_t max_batch;      /* items per launched batch (0: 1 << 20) */
+  uint32_t blocks_per_cu;  /* per-slot context setting (0: default) */
+} sigops_server_config;
+
+sigops_server* sigops_server_create(const sigops_server_config* cfg);
+/* finishes the queued and in-flight work, then releases everything */
+void sigops_server_destroy(sigops_server* srv);
+/* queues one request (inputs are copied before returning); returns a ticket >= 0, or SIGOPS_E_ARG
+   for a bad request or one larger than max_batch items or the HBM budget */
+int64_t sigops_server_submit(sigops_server* srv, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
+                             const uint8_t* msgs, const uint64_t* msg_off, size_t n);
+/* waits for a ticket (timeout_ms < 0: no limit); on completion copies out (n x 64) and status (n),
+   gives the submit and completion times (steady clock, ns) and releases the ticket.
+   Returns 0, 1 on timeout (the ticket stays valid), or a negative SIGOPS_E_* code. */
+int sigops_server_wait(sigops_server* srv, int64_t ticket, uint8_t* out, uint8_t* status, int timeout_ms,
+                       uint64_t* t_submit_ns, uint64_t* t_done_ns);
+/* batches launched and items processed so far */
+int sigops_server_stats(sigops_server* srv, uint64_t* batches, uint64_t* items);
+/* Load replay for latency measurement: a native client submits request r (items req_off[r] ..
+   req_off[r+1] of the given arrays, messages through the global msg_off) at arrival_ns[r] after the
+   start, collects every result into out / status (may be NULL) and writes latency_ns[r] =
+   completion - submission. */
+int sigops_server_replay(sigops_server* srv, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
+                         const uint8_t* msgs, const uint64_t* msg_off, const uint64_t* req_off,
+                         const uint64_t* arrival_ns, size_t n_requests, uint64_t* latency_ns, uint8_t* out,
+                         uint8_t* status);
+
 /* ---- synthetic workload generation on the GPU. No counterpart in the reference, which signs its
    benchmark inputs serially on the CPU with fuel-crypto / p256 / dalek
    (src/benchmarks/secp256k1_ecdsa.rs:150-172, src/benchmarks/ed25519_eddsa.rs:140-166); this signs
@@ -132,6 +170,12 @@ int sigops_gen_secp_dev(sigops_ctx* ctx, int curve, uint64_t seed, size_t n, uin
    aligned with 16 readable bytes after the last message */
 int sigops_gen_ed25519_dev(sigops_ctx* ctx, uint64_t seed, size_t n, uint32_t corrupt_ppm, const uint64_t* d_msg_off,
                            void* d_msgs, void* d_sigs, void* d_pubkeys, void* d_corrupted, void* stream);
+
+/* ---- VALU roofline probe. No counterpart in the reference. Measures, on `device`, the
+   whole-GPU issue rate of v_mad_u64_u32 (the multiply-accumulate of the field arithmetic) and of
+   v_fma_f64, in operations per second: the peak against which the signature kernels are
+   reported. Takes a few milliseconds. ---- */
+int sigops_probe_valu(int device, double* mad_per_s, double* fma64_per_s);
 
 #ifdef __cplusplus
 }

@@ -80,8 +80,19 @@ def lib():
             L.sigops_gen_secp_dev.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp,
                                               vp]
             L.sigops_gen_ed25519_dev.argtypes = [vp, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
+            L.sigops_probe_valu.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]
             _lib = L
         return _lib
+
+
+def probe_valu(device=0):
+    """(v_mad_u64_u32 ops/s, v_fma_f64 ops/s) measured over the whole GPU"""
+    mad, fma = ctypes.c_double(), ctypes.c_double()
+    rc = lib().sigops_probe_valu(device, ctypes.byref(mad), ctypes.byref(fma))
+    if rc:
+        raise SigopsError(f"sigops_probe_valu failed ({rc})")
+    return mad.value, fma.value
 
 
 def _ptr(a):

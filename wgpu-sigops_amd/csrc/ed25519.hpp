@@ -118,10 +118,26 @@ DEV bool sqrt_ratio_i(const fe& u, const fe& v, fe& r_out) {
 enum : u32 { ED_OK = 0, ED_BAD_S = 1, ED_BAD_KEY = 2, ED_MISMATCH = 3 };
 
 // sig_raw: 16 raw words (R || s); pk_raw: 8 raw words; message in mem at msg_off, len bytes.
+// k = SHA-512(R || A || M) mod l as 8 little-endian words
+template <class Mem>
+DEV void ed25519_challenge(const u32 sig_raw[16], const u32 pk_raw[8], const Mem& mem, u64 msg_off, u32 len,
+                           u32 k[8]) {
+  u32 hdr[16], h[16];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    hdr[i] = sig_raw[i];
+    hdr[8 + i] = pk_raw[i];
+  }
+  sha512_ed(hdr, mem, msg_off, len, h);
+  const fe hi = mont_mul<Fl>(fe_from_words(h + 8), fe_from_const(L_C256));
+  const fe lo = mont_mul<Fl>(fe_from_words(h), fe_from_const(L_C0));
+  fe_to_words(fcanon<Fl>(fadd<Fl>(hi, lo)), k);
+}
+
+// verification given the challenge k (from ed25519_challenge).
 // BTab: load(idx, ypx, ymx, t2d) for (2 idx + 1) B. Tab: per-lane store(e, cached) / load(e).
-template <class Tab, class BTab, class Mem>
-DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Mem& mem, u64 msg_off,
-                            u32 len, Tab& tab, const BTab& bt) {
+template <class Tab, class BTab>
+DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u32 k[8], Tab& tab, const BTab& bt) {
   u32 status = ED_OK;
   // s (little-endian bytes = little-endian words)
   u32 s[8];
@@ -149,20 +165,6 @@ DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Me
   const bool ok = sqrt_ratio_i(u, v, x);
   if (!ok && status == ED_OK) status = ED_BAD_KEY;
   if (x_sign) x = fneg<Fe>(x);
-  // k = SHA-512(R || A || M) mod l
-  u32 hdr[16], h[16];
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    hdr[i] = sig_raw[i];
-    hdr[8 + i] = pk_raw[i];
-  }
-  sha512_ed(hdr, mem, msg_off, len, h);
-  u32 k[8];
-  {
-    const fe hi = mont_mul<Fl>(fe_from_words(h + 8), fe_from_const(L_C256));
-    const fe lo = mont_mul<Fl>(fe_from_words(h), fe_from_const(L_C0));
-    fe_to_words(fcanon<Fl>(fadd<Fl>(hi, lo)), k);
-  }
   // odd representatives: s even -> l - s with B negated; k even -> k + 1 and a final +A
   const bool b_neg = (s[0] & 1u) == 0;
   const bool k_even = (k[0] & 1u) == 0;
@@ -202,25 +204,22 @@ DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Me
   ext acc = eadd_cached(ext_identity(), tab.load(nibble(c2, 63) - 8), true);
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
+    // issue the HBM load of this window's table entry before the doublings, which hide its latency
+    const u32 c = nibble(c2, i);
+    const bool neg = c < 8;
+    cached E = tab.load((neg ? ~c : c) & 7u);
     acc = edbl(acc, false);
     acc = edbl(acc, false);
     acc = edbl(acc, false);
     acc = edbl(acc, true);
-    {
-      const u32 c = nibble(c2, i);
-      const bool neg = c < 8;
-      const u32 idx = (neg ? ~c : c) & 7u;
-      cached E = tab.load(idx);
-      if (neg) E = neg_cached(E);
-      acc = eadd_cached(acc, E, true);
-    }
+    if (neg) E = neg_cached(E);
+    acc = eadd_cached(acc, E, true);
     if ((i & 1) == 0) {
-      const u32 c = byte8(c1, i >> 1);
-      const bool neg = (c < 128) != b_neg;
-      const u32 idx = ((c < 128) ? ~c : c) & 127u;
+      const u32 cb = byte8(c1, i >> 1);
+      const bool bneg = (cb < 128) != b_neg;
       fe ypx, ymx, t2d;
-      bt.load(idx, ypx, ymx, t2d);
-      if (neg) {
+      bt.load(((cb < 128) ? ~cb : cb) & 127u, ypx, ymx, t2d);
+      if (bneg) {
         const fe tmp = ypx;
         ypx = ymx;
         ymx = tmp;
@@ -252,6 +251,14 @@ DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Me
   for (int i = 0; i < 8; i++) diff |= enc[i] ^ sig_raw[i];
   if (status == ED_OK && diff != 0) status = ED_MISMATCH;
   return status;
+}
+
+template <class Tab, class BTab, class Mem>
+DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Mem& mem, u64 msg_off, u32 len,
+                            Tab& tab, const BTab& bt) {
+  u32 k[8];
+  ed25519_challenge(sig_raw, pk_raw, mem, msg_off, len, k);
+  return ed25519_verify_core(sig_raw, pk_raw, k, tab, bt);
 }
 
 }  // namespace sigops

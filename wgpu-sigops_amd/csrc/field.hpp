@@ -19,10 +19,18 @@
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
 #define DEV __device__ __forceinline__
+// The modular multiplication is inlined: an out-of-line copy (SIGOPS_OUTLINE_MUL=1) bounds the live
+// registers around each call but pays for call frames in scratch memory at every call site.
+#if defined(SIGOPS_OUTLINE_MUL) && SIGOPS_OUTLINE_MUL
+#define DEV_MUL __device__ __noinline__
+#else
+#define DEV_MUL __device__ __forceinline__
+#endif
 #else
 // Host build of the same arithmetic (tests/devcheck compiles these headers with g++ to check the
 // device algorithms against the oracle without a GPU).
 #define DEV inline
+#define DEV_MUL inline
 #endif
 
 #include "consts.hpp"
@@ -60,7 +68,7 @@ DEV fe fe_zero() {
 // (it only lacks the carry of column i-1), so m_i = col_i * N0 mod 2^29 and adding m_i*M zeroes
 // its low 29 bits.
 template <class F>
-DEV fe mont_mul(const fe& a, const fe& b) {
+DEV_MUL fe mont_mul(const fe& a, const fe& b) {
   u64 acc[2 * NL - 1];
 #pragma unroll
   for (int k = 0; k < 2 * NL - 1; k++) acc[k] = 0;
@@ -88,7 +96,7 @@ DEV fe mont_mul(const fe& a, const fe& b) {
 // Montgomery square: 45 instead of 81 partial products (cross terms use a pre-doubled operand,
 // < 2^30, so a cross product is < 2^59 and a column still stays below 2^63).
 template <class F>
-DEV fe mont_sqr(const fe& a) {
+DEV_MUL fe mont_sqr(const fe& a) {
   u32 a2[NL];
 #pragma unroll
   for (int k = 0; k < NL; k++) a2[k] = a.v[k] << 1;

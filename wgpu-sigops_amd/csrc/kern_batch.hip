@@ -35,6 +35,7 @@ struct BatchArgs {
   u32* pend_count;
   const u8* rank;   // bucket -> processing rank
   u32 tile_items;   // 64 * tile_rounds
+  const uint4* kbuf;  // Ed25519 challenges, written by the hash pass
 };
 
 DEV u32 item_bucket(const BatchArgs& a, u32 i) {
@@ -99,6 +100,23 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_batch_hash(BatchArgs a) {
+  const MemDev mem{a.msgs};
+  const u32 total = a.tile_start[NBUCKETS] * a.tile_items;
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+    const u32 i = a.order[j];
+    if (i == DUMMY || a.scheme[i] != 2) continue;
+    u32 sr[16], pr[8], k[8];
+    load_words(a.sigs + 4 * (u64)i, sr, 4);
+    load_words(a.aux + 2 * (u64)i, pr, 2);
+    const u64 off = a.msg_off[i];
+    ed25519_challenge(sr, pr, mem, off, (u32)(a.msg_off[i + 1] - off), k);
+    uint4* kb = (uint4*)a.kbuf;
+    kb[2 * (u64)i] = make_uint4(k[0], k[1], k[2], k[3]);
+    kb[2 * (u64)i + 1] = make_uint4(k[4], k[5], k[6], k[7]);
+  }
+}
+
 DEV void store16(uint4* p, const u32 o[16]) {
 #pragma unroll
   for (int c = 0; c < 4; c++) p[c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
@@ -130,7 +148,7 @@ DEV void batch_secp_item(const BatchArgs& a, u32 i, JTabDev& tab, const GT& gt) 
   a.status[i] = (u8)st;
 }
 
-__global__ __launch_bounds__(256) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
+__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
                                                    const uint4* __restrict__ gr1, const uint4* __restrict__ gb,
                                                    uint4* __restrict__ lane_tab, u32 slots) {
   __shared__ uint4 s_k1[GTAB_ENTRIES * 5];
@@ -153,7 +171,6 @@ __global__ __launch_bounds__(256) void k_batch_run(BatchArgs a, const uint4* __r
   ct.slots = slots;
   ct.slot = slot;
   const u32 ntiles = a.tile_start[NBUCKETS];
-  const MemDev mem{a.msgs};
   for (;;) {
     u32 t = 0;
     if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
@@ -173,12 +190,11 @@ __global__ __launch_bounds__(256) void k_batch_run(BatchArgs a, const uint4* __r
       } else if (b == BUCKET_R1) {
         batch_secp_item<CurveR1>(a, i, jt, GTabLds{s_r1});
       } else {
-        u32 sr[16], pr[8];
+        u32 sr[16], pr[8], kk[8];
         load_words(a.sigs + 4 * (u64)i, sr, 4);
         load_words(a.aux + 2 * (u64)i, pr, 2);
-        const u64 off = a.msg_off[i];
-        const u32 len = (u32)(a.msg_off[i + 1] - off);
-        a.status[i] = (u8)ed25519_verify_item(sr, pr, mem, off, len, ct, BTabLds{s_b});
+        load_words(a.kbuf + 2 * (u64)i, kk, 2);
+        a.status[i] = (u8)ed25519_verify_core(sr, pr, kk, ct, BTabLds{s_b});
         const u32 zero[16] = {0};
         store16(a.out + 4 * (u64)i, zero);
       }
@@ -237,11 +253,15 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   a.order = L.order;
   a.rank = L.rank;
   a.tile_items = 64u * L.tile_rounds;
+  a.kbuf = (const uint4*)L.kbuf;
   // counters are cleared with a kernel (not a memset node) so the sequence is safe to capture
   k_zero_u32<<<1, 256, 0, s>>>(L.work, BATCH_WORK_WORDS);
   k_bucket_hist<<<L.aux_grid, 256, 0, s>>>(a);
   k_bucket_scan<<<1, 64, 0, s>>>(a);
   k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
+  // Ed25519 challenges, hashing the items in bucket order (equal SHA-512 block counts per wavefront);
+  // secp entries of the order array are skipped by the scheme test inside
+  k_batch_hash<<<L.grid, 256, 0, s>>>(a);
   k_batch_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                      (uint4*)L.lane_tab, L.slots);
   k_batch_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);

@@ -20,7 +20,7 @@ hipError_t launch_secp_recover_CurveR1(const void* sigs, const void* digs, void*
                                       uint32_t* pend_count, uint32_t grid, hipStream_t s);
 
 hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* msgs, const uint64_t* msg_off,
-                                 void* status, uint32_t n, const void* btab, void* ctab, uint32_t slots,
+                                 void* status, uint32_t n, const void* btab, void* ctab, uint32_t slots, void* kbuf,
                                  uint32_t grid, hipStream_t s);
 
 // ---- mixed batch ----
@@ -47,6 +47,7 @@ struct BatchLaunch {
   const void* gr1;
   const void* gb;
   void* lane_tab;
+  void* kbuf;  // n x 32 bytes: Ed25519 challenges
   uint32_t slots, grid, aux_grid, tile_rounds;
 };
 

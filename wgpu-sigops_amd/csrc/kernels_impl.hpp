@@ -17,6 +17,11 @@
 #include "kernels.hpp"
 #include "secp.hpp"
 
+// Waves per SIMD the heavy kernels are compiled for (the register budget per lane is 512 / this).
+#ifndef SIGOPS_WAVES_PER_EU
+#define SIGOPS_WAVES_PER_EU 2
+#endif
+
 namespace sigops {
 
 // ---- table accessors ----
@@ -125,7 +130,7 @@ DEV void load_words(const uint4* p, u32* w, int nq) {
 
 // ---- kernels ----
 template <class C>
-__global__ __launch_bounds__(256) void k_secp_recover(const uint4* __restrict__ sigs,
+__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_recover(const uint4* __restrict__ sigs,
                                                       const uint4* __restrict__ digs, uint4* __restrict__ out,
                                                       u8* __restrict__ status, u32 n,
                                                       const uint4* __restrict__ gtab, uint4* __restrict__ rtab,
@@ -198,12 +203,33 @@ __global__ __launch_bounds__(64) void k_secp_careful(const uint4* __restrict__ s
   }
 }
 
-static __global__ __launch_bounds__(256) void k_ed25519_verify(const uint4* __restrict__ sigs,
-                                                        const uint4* __restrict__ pks,
-                                                        const u32* __restrict__ msgs,
-                                                        const u64* __restrict__ msg_off, u8* __restrict__ status,
-                                                        u32 n, const uint4* __restrict__ btab,
-                                                        uint4* __restrict__ ctab, u32 slots) {
+// k = SHA-512(R || A || M) mod l per item; a pass of its own so that the verifier's register
+// budget carries no hash state (the reference also hashes in a separate stage,
+// src/wgsl/main/ed25519_eddsa_main_1.wgsl). `order` (may be NULL) lists the items to hash.
+static __global__ __launch_bounds__(256) void k_ed25519_hash(const uint4* __restrict__ sigs,
+                                                             const uint4* __restrict__ pks,
+                                                             const u32* __restrict__ msgs,
+                                                             const u64* __restrict__ msg_off,
+                                                             const u32* __restrict__ order, const u32* n_ptr, u32 n_arg,
+                                                             uint4* __restrict__ kbuf) {
+  const MemDev mem{msgs};
+  const u32 n = n_ptr ? *n_ptr : n_arg;
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const u32 i = order ? order[j] : j;
+    if (i == 0xFFFFFFFFu) continue;
+    u32 sr[16], pr[8], k[8];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(pks + 2 * (u64)i, pr, 2);
+    const u64 off = msg_off[i];
+    ed25519_challenge(sr, pr, mem, off, (u32)(msg_off[i + 1] - off), k);
+    kbuf[2 * (u64)i] = make_uint4(k[0], k[1], k[2], k[3]);
+    kbuf[2 * (u64)i + 1] = make_uint4(k[4], k[5], k[6], k[7]);
+  }
+}
+
+static __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_ed25519_verify(
+    const uint4* __restrict__ sigs, const uint4* __restrict__ pks, const uint4* __restrict__ kbuf,
+    u8* __restrict__ status, u32 n, const uint4* __restrict__ btab, uint4* __restrict__ ctab, u32 slots) {
   __shared__ uint4 sb[BTAB_ENTRIES * 7];
   for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) sb[t] = btab[t];
   __syncthreads();
@@ -213,15 +239,13 @@ static __global__ __launch_bounds__(256) void k_ed25519_verify(const uint4* __re
   tab.base = ctab;
   tab.slots = slots;
   tab.slot = slot;
-  const MemDev mem{msgs};
   const u32 stride = gridDim.x * blockDim.x;
   for (u32 i = slot; i < n; i += stride) {
-    u32 sr[16], pr[8];
+    u32 sr[16], pr[8], k[8];
     load_words(sigs + 4 * (u64)i, sr, 4);
     load_words(pks + 2 * (u64)i, pr, 2);
-    const u64 off = msg_off[i];
-    const u32 len = (u32)(msg_off[i + 1] - off);
-    status[i] = (u8)ed25519_verify_item(sr, pr, mem, off, len, tab, bt);
+    load_words(kbuf + 2 * (u64)i, k, 2);
+    status[i] = (u8)ed25519_verify_core(sr, pr, k, tab, bt);
   }
 }
 

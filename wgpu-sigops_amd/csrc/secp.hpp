@@ -254,23 +254,20 @@ DEV jac secp_ladder(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
   jac acc = tab.load(nibble(o.c2, 63) - 8);
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
+    // issue the HBM load of this window's table entry before the doublings, which hide its latency
+    const u32 c = nibble(o.c2, i);
+    const bool neg = c < 8;
+    jac E = tab.load((neg ? ~c : c) & 7u);
 #pragma unroll 1
     for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
-    {
-      const u32 c = nibble(o.c2, i);
-      const bool neg = c < 8;
-      const u32 idx = (neg ? ~c : c) & 7u;
-      jac E = tab.load(idx);
-      if (neg) E.Y = fneg<F>(E.Y);
-      acc = jadd<C>(acc, E, exc);
-    }
+    if (neg) E.Y = fneg<F>(E.Y);
+    acc = jadd<C>(acc, E, exc);
     if ((i & 1) == 0) {
-      const u32 c = byte8(o.c1, i >> 1);
-      const bool neg = (c < 128) != o.g_neg;
-      const u32 idx = ((c < 128) ? ~c : c) & 127u;
+      const u32 cg = byte8(o.c1, i >> 1);
+      const bool gneg = (cg < 128) != o.g_neg;
       fe gx, gy;
-      gt.load(idx, gx, gy);
-      if (neg) gy = fneg<F>(gy);
+      gt.load(((cg < 128) ? ~cg : cg) & 127u, gx, gy);
+      if (gneg) gy = fneg<F>(gy);
       acc = jmadd<C>(acc, gx, gy, exc);
     }
   }

@@ -14,33 +14,42 @@ namespace sigops {
 DEV u64 rotr64(u64 x, int n) { return (x >> n) | (x << (64 - n)); }
 DEV u64 bswap64(u64 x) { return __builtin_bswap64(x); }
 
+// 16 rounds per loop trip with the 16-word schedule kept as a circular buffer in registers; the
+// round constants are read with uniform (scalar) loads. Unrolling all 80 rounds lets the compiler
+// hoist the whole message schedule and doubles the register footprint of the verifier.
+DEV void sha512_round(u64 (&s)[8], int j, u64 kw) {
+  const u64 a = s[(8 - j) & 7], b = s[(9 - j) & 7], c = s[(10 - j) & 7];
+  u64& d = s[(11 - j) & 7];
+  const u64 e = s[(12 - j) & 7], f = s[(13 - j) & 7], g = s[(14 - j) & 7];
+  u64& hh = s[(15 - j) & 7];
+  const u64 S1 = rotr64(e, 14) ^ rotr64(e, 18) ^ rotr64(e, 41);
+  const u64 ch = (e & f) ^ (~e & g);
+  const u64 t1 = hh + S1 + ch + kw;
+  const u64 S0 = rotr64(a, 28) ^ rotr64(a, 34) ^ rotr64(a, 39);
+  const u64 maj = (a & b) | (c & (a | b));
+  d += t1;
+  hh = t1 + S0 + maj;  // becomes the new "a": the register names rotate by one per round
+}
+
 DEV void sha512_compress(u64 h[8], u64 w[16]) {
-  u64 a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+  u64 s[8];
 #pragma unroll
-  for (int t = 0; t < 80; t++) {
-    if (t >= 16) {
-      const u64 w15 = w[(t + 1) & 15], w2 = w[(t + 14) & 15];
+  for (int i = 0; i < 8; i++) s[i] = h[i];
+#pragma unroll
+  for (int j = 0; j < 16; j++) sha512_round(s, j & 7, SHA512_K[j] + w[j]);
+#pragma unroll 1
+  for (int t0 = 16; t0 < 80; t0 += 16) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const u64 w15 = w[(j + 1) & 15], w2 = w[(j + 14) & 15];
       const u64 s0 = rotr64(w15, 1) ^ rotr64(w15, 8) ^ (w15 >> 7);
       const u64 s1 = rotr64(w2, 19) ^ rotr64(w2, 61) ^ (w2 >> 6);
-      w[t & 15] += s0 + w[(t + 9) & 15] + s1;
+      w[j] += s0 + w[(j + 9) & 15] + s1;
+      sha512_round(s, j & 7, SHA512_K[t0 + j] + w[j]);
     }
-    const u64 S1 = rotr64(e, 14) ^ rotr64(e, 18) ^ rotr64(e, 41);
-    const u64 ch = (e & f) ^ (~e & g);
-    const u64 t1 = hh + S1 + ch + SHA512_K[t] + w[t & 15];
-    const u64 S0 = rotr64(a, 28) ^ rotr64(a, 34) ^ rotr64(a, 39);
-    const u64 maj = (a & b) | (c & (a | b));
-    const u64 t2 = S0 + maj;
-    hh = g;
-    g = f;
-    f = e;
-    e = d + t1;
-    d = c;
-    c = b;
-    b = a;
-    a = t1 + t2;
   }
-  h[0] += a; h[1] += b; h[2] += c; h[3] += d;
-  h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+#pragma unroll
+  for (int i = 0; i < 8; i++) h[i] += s[i];
 }
 
 // 8 little-endian bytes at byte offset off of a 4-byte-aligned buffer (reads 12 bytes starting at

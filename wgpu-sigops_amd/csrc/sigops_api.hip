@@ -51,6 +51,7 @@ struct sigops_ctx {
            hipGraphExec_t>
       graphs;
   DevBuf in, aux, msg, off, out, st, sch;
+  void* d_kbuf = nullptr;  // max_batch x 32 bytes: Ed25519 challenges
   std::vector<uint8_t> host_status;
   std::vector<uint64_t> host_off;
   std::string err;
@@ -198,6 +199,7 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
       hipSuccess)
     return bad(e, "hipMalloc batch order");
   if ((e = hipMalloc((void**)&c->d_n, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc n");
+  if ((e = hipMalloc(&c->d_kbuf, (size_t)c->max_batch * 32)) != hipSuccess) return bad(e, "hipMalloc challenges");
   return c;
 }
 
@@ -208,7 +210,7 @@ void sigops_destroy(sigops_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
                   c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
-                  (void*)c->d_order, (void*)c->d_n})
+                  (void*)c->d_order, (void*)c->d_n, c->d_kbuf})
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -244,7 +246,7 @@ int sigops_ed25519_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_p
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
   HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_msgs, d_msg_off, d_status, (uint32_t)n, c->d_btab,
-                                          c->d_lane_tab, c->slots, c->grid, (hipStream_t)stream));
+                                          c->d_lane_tab, c->slots, c->d_kbuf, c->grid, (hipStream_t)stream));
   return SIGOPS_OK;
 }
 
@@ -336,6 +338,7 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
   L.gr1 = c->d_gtab_r1;
   L.gb = c->d_btab;
   L.lane_tab = c->d_lane_tab;
+  L.kbuf = c->d_kbuf;
   L.slots = c->slots;
   L.grid = c->grid;
   L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));

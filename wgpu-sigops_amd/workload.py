@@ -95,7 +95,7 @@ def mixed(ctx, n, seed=1, corrupt_ppm=50000, max_len=4096, device="cuda:0"):
     total = int(off[-1])
     msgs = torch.zeros(((total + 16 + 3) // 4) * 4, dtype=torch.uint8, device=dev)
     # place each Ed25519 message at its offset in the mixed layout (one gather of byte positions)
-    src_pos = np.concatenate([np.arange(s, e) for s, e in zip(ed.msg_off_host[:-1], ed.msg_off_host[1:])]) \\
+    src_pos = np.concatenate([np.arange(s, e) for s, e in zip(ed.msg_off_host[:-1], ed.msg_off_host[1:])]) \
         if total else np.zeros(0, np.int64)
     dst_pos = np.concatenate([np.arange(off[i], off[i + 1]) for i in idx[2]]) if total else np.zeros(0, np.int64)
     if total:
