@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--tile-rounds", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every signature is independent, "
@@ -183,6 +185,50 @@ def main():
     incl_ok = bool(np.array_equal(h_out, out_h) and np.array_equal(h_st, st_h))
     log(f"with transfers: {incl_ms:.2f} ms per batch, {n / incl_ms * 1e3 / 1e6:.2f} M sigs/s")
 
+    # served: Poisson arrivals of mixed requests (1..4096 items, log-uniform) at three offered loads,
+    # replayed by the library's native client against the continuous-batching server
+    served = None
+    if not args.no_serve:
+        served = []
+        rng = np.random.default_rng(7)
+        # serving moves every request over PCIe, so its capacity is the transfer-inclusive batch rate
+        capacity = n / (incl_ms / 1e3)
+        with sigops.Server(slots=3) as srv:
+            warm = True
+            for load in (0.5, 0.25, 0.5, 0.8):  # the first pass warms up (buffers, graphs) and is dropped
+                rate_items = load * capacity
+                mean_req = (4096 - 1) / np.log(4096)
+                nreq = int(max(2000, (0.3 if warm else args.serve_seconds) * rate_items / mean_req))
+                sizes = np.floor(np.exp(rng.random(nreq) * np.log(4096))).astype(np.int64).clip(1, 4096)
+                starts = rng.integers(0, n - 4096, nreq)
+                # requests are contiguous slices of the pool; the replay takes one offsets array, so
+                # lay the requests out back to back in a gathered copy
+                idx = np.concatenate([np.arange(a, a + k) for a, k in zip(starts, sizes)])
+                req_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+                lens = (h_off[idx + 1] - h_off[idx]).astype(np.int64)
+                g_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+                g_msgs = np.concatenate([h_msgs[h_off[i]:h_off[i + 1]] for i in idx[h_sch[idx] == 2]]) \
+                    if (h_sch[idx] == 2).any() else np.zeros(0, np.uint8)
+                # messages of non-Ed25519 items are empty, so the Ed25519 messages concatenate in order
+                arrivals = np.cumsum(rng.exponential(1e9 * mean_req / rate_items, nreq)).astype(np.uint64)
+                t0 = time.perf_counter()
+                lat, s_out, s_st = srv.replay(h_sch[idx], h_sigs[idx], h_aux[idx], g_msgs, g_off, req_off, arrivals)
+                wall = time.perf_counter() - t0
+                same = bool(np.array_equal(s_out, out_h[idx]) and np.array_equal(s_st, st_h[idx]))
+                lat_ms = lat.astype(np.float64) / 1e6
+                if warm:
+                    warm = False
+                    continue
+                served.append({"load": load, "capacity_basis": "batched incl. transfers",
+                               "offered_sigs_per_s": round(rate_items, 1), "requests": nreq,
+                               "items": int(req_off[-1]), "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                               "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                               "sustained_sigs_per_s": round(float(req_off[-1]) / wall, 1),
+                               "results_identical_to_batch": same})
+                log(f"served load {load}: {nreq} requests, p50 {served[-1]['p50_ms']} ms, p99 {served[-1]['p99_ms']} ms, "
+                    f"{served[-1]['sustained_sigs_per_s'] / 1e6:.2f} M sigs/s sustained, identical: {same}")
+            served_batches = srv.stats()
+
     # roofline: dominant kernel = the persistent batch kernel; its launch time is the step time
     # less the small bucketing kernels (measured by rocprofv3 in profiles/), bounded here by the step
     mac_sig = mixed_mac_per_sig(sch, mx.lens)
@@ -243,6 +289,7 @@ def main():
         "incl_transfers": {"ms_per_batch": round(incl_ms, 3), "sigs_per_s": round(n / incl_ms * 1e3, 1),
                            "identical_to_device_path": incl_ok},
         "roundtrip_all_items_ok": roundtrip_ok,
+        "served": served,
         "probe": {"mad_per_s": mad_per_s, "fma64_per_s": fma_per_s},
     }
     print(json.dumps(line), flush=True)

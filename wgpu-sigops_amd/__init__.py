@@ -42,6 +42,11 @@ class _Config(ctypes.Structure):
                 ("tile_rounds", ctypes.c_uint32), ("no_graph", ctypes.c_uint32)]
 
 
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+                ("max_batch", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32)]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -82,6 +87,15 @@ def lib():
             L.sigops_gen_ed25519_dev.argtypes = [vp, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
             L.sigops_probe_valu.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)]
+            L.sigops_server_create.restype = vp
+            L.sigops_server_create.argtypes = [ctypes.POINTER(_ServerConfig)]
+            L.sigops_server_destroy.argtypes = [vp]
+            L.sigops_server_submit.restype = ctypes.c_int64
+            L.sigops_server_submit.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, sz]
+            L.sigops_server_wait.argtypes = [vp, ctypes.c_int64, c_p, c_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+            L.sigops_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+            L.sigops_server_replay.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, c_p, c_p, sz, c_p, c_p, c_p]
             _lib = L
         return _lib
 
@@ -218,6 +232,85 @@ class Context:
     def ed25519_verify_dev(self, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream=0):
         self._check(lib().sigops_ed25519_verify_dev(self._h, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status,
                                                     stream or None))
+
+
+class Server:
+    """Continuous-batching request server (sigops_server_*): submit() returns a ticket at once,
+    wait() returns (out, status, t_submit_ns, t_done_ns)."""
+
+    def __init__(self, device=0, slots=0, hbm_budget=0, max_batch=0, blocks_per_cu=0):
+        cfg = _ServerConfig(device, slots, hbm_budget, max_batch, blocks_per_cu)
+        self._h = lib().sigops_server_create(ctypes.byref(cfg))
+        if not self._h:
+            raise SigopsError("sigops_server_create failed")
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sigops_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, schemes, sigs, aux, msgs, msg_off):
+        schemes = np.ascontiguousarray(schemes, dtype=np.uint8).reshape(-1)
+        n = schemes.shape[0]
+        sigs = _u8(sigs, 64, "sigs")
+        aux = _u8(aux, 32, "aux")
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint8).reshape(-1)
+        msg_off = np.ascontiguousarray(msg_off, dtype=np.uint64)
+        t = lib().sigops_server_submit(self._h, _ptr(schemes), _ptr(sigs), _ptr(aux), _ptr(msgs) if msgs.size else None,
+                                       _ptr(msg_off), n)
+        if t < 0:
+            raise SigopsError(f"submit failed ({t})")
+        self._n[t] = n
+        return t
+
+    def wait(self, ticket, timeout_ms=-1):
+        n = self._n[ticket]
+        out = np.zeros((n, 64), np.uint8)
+        st = np.zeros(n, np.uint8)
+        ts, td = ctypes.c_uint64(), ctypes.c_uint64()
+        rc = lib().sigops_server_wait(self._h, ticket, _ptr(out), _ptr(st), timeout_ms, ctypes.byref(ts),
+                                      ctypes.byref(td))
+        if rc == 1:
+            return None
+        del self._n[ticket]
+        if rc == E_CANARY:
+            raise ShaderFailureError("request results were not written")
+        if rc:
+            raise SigopsError(f"request failed ({rc})")
+        return out, st, ts.value, td.value
+
+    def stats(self):
+        b, i = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().sigops_server_stats(self._h, ctypes.byref(b), ctypes.byref(i))
+        return b.value, i.value
+
+    def replay(self, schemes, sigs, aux, msgs, msg_off, req_off, arrival_ns):
+        """native client: request r = items req_off[r]..req_off[r+1], submitted at arrival_ns[r];
+        returns (latency_ns per request, out, status)"""
+        schemes = np.ascontiguousarray(schemes, dtype=np.uint8)
+        sigs = _u8(sigs, 64, "sigs")
+        aux = _u8(aux, 32, "aux")
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint8).reshape(-1)
+        msg_off = np.ascontiguousarray(msg_off, dtype=np.uint64)
+        req_off = np.ascontiguousarray(req_off, dtype=np.uint64)
+        arrival_ns = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        nreq = arrival_ns.shape[0]
+        lat = np.zeros(nreq, np.uint64)
+        out = np.zeros((schemes.shape[0], 64), np.uint8)
+        st = np.zeros(schemes.shape[0], np.uint8)
+        rc = lib().sigops_server_replay(self._h, _ptr(schemes), _ptr(sigs), _ptr(aux), _ptr(msgs) if msgs.size else None,
+                                        _ptr(msg_off), _ptr(req_off), _ptr(arrival_ns), nreq, _ptr(lat), _ptr(out),
+                                        _ptr(st))
+        if rc:
+            raise SigopsError(f"replay failed ({rc})")
+        return lat, out, st
 
 
 _default_ctx = None

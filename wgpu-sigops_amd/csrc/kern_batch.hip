@@ -88,15 +88,32 @@ __global__ __launch_bounds__(64) void k_bucket_scan(BatchArgs a) {
   }
 }
 
+// Each workgroup ranks its chunk of 256 items per bucket with LDS atomics, reserves one range per
+// bucket with a single global atomic, then writes the indices: one global atomic per bucket and
+// chunk instead of one per item on a handful of hot counters.
 __global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
+  __shared__ u32 cnt[NBUCKETS];
+  __shared__ u32 base[NBUCKETS];
   const u32 n = *a.n_ptr;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (a.scheme[i] > 2) {
-      a.status[i] = 0xFD;  // unknown scheme: reported as a canary failure by the host
-      continue;
+  for (u32 c0 = blockIdx.x * blockDim.x; c0 < n; c0 += gridDim.x * blockDim.x) {
+    for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
+    const u32 i = c0 + threadIdx.x;
+    u32 b = NBUCKETS, rank = 0;
+    if (i < n) {
+      if (a.scheme[i] > 2) {
+        a.status[i] = 0xFD;  // unknown scheme: reported as a canary failure by the host
+      } else {
+        b = item_bucket(a, i);
+        rank = atomicAdd(&cnt[b], 1u);
+      }
     }
-    const u32 pos = atomicAdd(&a.cursor[item_bucket(a, i)], 1u);
-    a.order[pos] = i;
+    __syncthreads();
+    for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x)
+      if (cnt[t]) base[t] = atomicAdd(&a.cursor[t], cnt[t]);
+    __syncthreads();
+    if (b < NBUCKETS) a.order[base[b] + rank] = i;
+    __syncthreads();
   }
 }
 
