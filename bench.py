@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--tile-rounds", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run a plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every signature is independent, "
@@ -76,6 +78,14 @@ def main():
     n = args.n
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 
+    if args.calib_copy_mib:
+        # a plain copy kernel of known size, read by the FETCH_SIZE / WRITE_SIZE counter passes
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mad_per_s, fma_per_s = sigops.probe_valu(0)
     log(f"probe: v_mad_u64_u32 {mad_per_s / 1e12:.2f} T/s, v_fma_f64 {fma_per_s / 1e12:.2f} T/s")
     ctx = sigops.Context(blocks_per_cu=args.blocks_per_cu, tile_rounds=args.tile_rounds, max_batch=max(n, 1 << 22))

@@ -1,0 +1,167 @@
+// sigops.hpp -- header-only C++ mirror of the reference's public API over the C ABI (sigops.h).
+//
+// Same names, argument meaning and results as the reference's async Rust functions:
+//   secp256k1_ecdsa::ecrecover(signatures, messages, table_limbs, log_limb_size)   src/secp256k1_ecdsa.rs:61-66
+//   secp256k1_ecdsa::ecrecover_single_shader(signatures, messages, log_limb_size)  src/secp256k1_ecdsa.rs:215-219
+//   secp256r1_ecdsa::ecrecover / ecrecover_single_shader                         src/secp256r1_ecdsa.rs:62-67, :216-220
+//   ed25519_eddsa::ecverify(signatures, messages, verifying_keys, table_limbs, log_limb_size)
+//   ed25519_eddsa::ecverify_single(signatures, messages, verifying_keys, log_limb_size)
+//                                                                                src/ed25519_eddsa.rs:67-73, :259-264
+// Rust's Result<_, ShaderFailureError> (src/lib.rs:12-14) becomes a thrown sigops::ShaderFailureError;
+// the reference's panics on bad arguments (length mismatch, src/secp256k1_ecdsa.rs:21) become
+// std::invalid_argument. The table arguments are checked for shape only: the library keeps its own
+// generator tables resident in HBM.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sigops.h"
+
+namespace sigops {
+
+using Signature = std::array<uint8_t, 64>;  // fuel Signature / Bytes64: r || s, parity in bit 7 of byte 32
+using Bytes64 = std::array<uint8_t, 64>;
+using Message = std::array<uint8_t, 32>;    // fuel Message: a 32-byte digest
+using VerifyingKey = std::array<uint8_t, 32>;
+using Ed25519Signature = std::array<uint8_t, 64>;  // R || s
+
+struct ShaderFailureError : std::runtime_error {
+  explicit ShaderFailureError(const std::string& w) : std::runtime_error(w) {}
+};
+
+class Context {
+ public:
+  explicit Context(const sigops_config* cfg = nullptr) : c_(sigops_create(cfg)) {
+    if (!c_) throw std::runtime_error("sigops_create failed");
+  }
+  ~Context() { sigops_destroy(c_); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  sigops_ctx* get() const { return c_; }
+  void check(int rc) const {
+    if (rc == SIGOPS_OK) return;
+    const std::string msg = sigops_last_error(c_);
+    if (rc == SIGOPS_E_CANARY) throw ShaderFailureError(msg);
+    throw std::runtime_error("libsigops error " + std::to_string(rc) + ": " + msg);
+  }
+
+ private:
+  sigops_ctx* c_;
+};
+
+inline Context& default_context() {
+  static Context ctx;
+  return ctx;
+}
+
+namespace detail {
+inline void check_log_limb_size(uint32_t l) {
+  if (l < 11 || l > 15) throw std::invalid_argument("log_limb_size must be in 11..15");
+}
+inline void check_table(const std::vector<uint32_t>& t, size_t coords, uint32_t l) {
+  check_log_limb_size(l);
+  if (t.size() != 16 * coords * (256 / l + 1)) throw std::invalid_argument("table_limbs has the wrong size");
+}
+template <class Sig>
+std::vector<std::vector<uint8_t>> recover(int curve, const std::vector<Sig>& signatures,
+                                          const std::vector<Message>& messages, Context& ctx) {
+  if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
+  const size_t n = signatures.size();
+  if (n == 0) return {};
+  std::vector<uint8_t> sigs(n * 64), digs(n * 32), out(n * 64);
+  for (size_t i = 0; i < n; i++) {
+    memcpy(&sigs[i * 64], signatures[i].data(), 64);
+    memcpy(&digs[i * 32], messages[i].data(), 32);
+  }
+  ctx.check(curve == 0 ? sigops_secp256k1_recover(ctx.get(), sigs.data(), digs.data(), n, out.data(), nullptr)
+                       : sigops_secp256r1_recover(ctx.get(), sigs.data(), digs.data(), n, out.data(), nullptr));
+  std::vector<std::vector<uint8_t>> res(n);
+  for (size_t i = 0; i < n; i++) res[i].assign(out.begin() + i * 64, out.begin() + i * 64 + 64);
+  return res;
+}
+}  // namespace detail
+
+namespace secp256k1_ecdsa {
+inline std::vector<std::vector<uint8_t>> ecrecover(const std::vector<Signature>& signatures,
+                                                   const std::vector<Message>& messages,
+                                                   const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
+                                                   Context& ctx = default_context()) {
+  detail::check_table(table_limbs, 2, log_limb_size);
+  return detail::recover(0, signatures, messages, ctx);
+}
+inline std::vector<std::vector<uint8_t>> ecrecover_single_shader(const std::vector<Signature>& signatures,
+                                                                 const std::vector<Message>& messages,
+                                                                 uint32_t log_limb_size,
+                                                                 Context& ctx = default_context()) {
+  detail::check_log_limb_size(log_limb_size);
+  return detail::recover(0, signatures, messages, ctx);
+}
+inline std::vector<std::vector<uint8_t>> ecrecover_single(const std::vector<Signature>& signatures,
+                                                          const std::vector<Message>& messages, uint32_t log_limb_size,
+                                                          Context& ctx = default_context()) {
+  return ecrecover_single_shader(signatures, messages, log_limb_size, ctx);
+}
+}  // namespace secp256k1_ecdsa
+
+namespace secp256r1_ecdsa {
+inline std::vector<std::vector<uint8_t>> ecrecover(const std::vector<Bytes64>& signatures,
+                                                   const std::vector<Message>& messages,
+                                                   const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
+                                                   Context& ctx = default_context()) {
+  detail::check_table(table_limbs, 2, log_limb_size);
+  return detail::recover(1, signatures, messages, ctx);
+}
+inline std::vector<std::vector<uint8_t>> ecrecover_single_shader(const std::vector<Bytes64>& signatures,
+                                                                 const std::vector<Message>& messages,
+                                                                 uint32_t log_limb_size,
+                                                                 Context& ctx = default_context()) {
+  detail::check_log_limb_size(log_limb_size);
+  return detail::recover(1, signatures, messages, ctx);
+}
+}  // namespace secp256r1_ecdsa
+
+namespace ed25519_eddsa {
+namespace detail_ed {
+inline std::vector<bool> verify(const std::vector<Ed25519Signature>& signatures, const std::vector<Message>& messages,
+                                const std::vector<VerifyingKey>& verifying_keys, Context& ctx) {
+  if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
+  const size_t n = signatures.size();
+  if (n == 0) return {};
+  if (verifying_keys.size() < n) throw std::out_of_range("fewer verifying keys than signatures");
+  std::vector<uint8_t> sigs(n * 64), pks(n * 32), msgs(n * 32), valid(n);
+  std::vector<uint64_t> off(n + 1);
+  for (size_t i = 0; i < n; i++) {
+    memcpy(&sigs[i * 64], signatures[i].data(), 64);
+    memcpy(&pks[i * 32], verifying_keys[i].data(), 32);
+    memcpy(&msgs[i * 32], messages[i].data(), 32);
+    off[i] = 32 * i;
+  }
+  off[n] = 32 * n;
+  ctx.check(sigops_ed25519_verify(ctx.get(), sigs.data(), pks.data(), msgs.data(), off.data(), n, valid.data(), nullptr));
+  std::vector<bool> res(n);
+  for (size_t i = 0; i < n; i++) res[i] = valid[i] == 1;
+  return res;
+}
+}  // namespace detail_ed
+
+inline std::vector<bool> ecverify(const std::vector<Ed25519Signature>& signatures, const std::vector<Message>& messages,
+                                  const std::vector<VerifyingKey>& verifying_keys,
+                                  const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
+                                  Context& ctx = default_context()) {
+  detail::check_table(table_limbs, 3, log_limb_size);
+  return detail_ed::verify(signatures, messages, verifying_keys, ctx);
+}
+inline std::vector<bool> ecverify_single(const std::vector<Ed25519Signature>& signatures,
+                                         const std::vector<Message>& messages,
+                                         const std::vector<VerifyingKey>& verifying_keys, uint32_t log_limb_size,
+                                         Context& ctx = default_context()) {
+  detail::check_log_limb_size(log_limb_size);
+  return detail_ed::verify(signatures, messages, verifying_keys, ctx);
+}
+}  // namespace ed25519_eddsa
+
+}  // namespace sigops

@@ -1,0 +1,66 @@
+// mirror_test.cpp -- drives the C++ mirror of the reference API (include/sigops.hpp) on a GPU.
+// Reads vectors written by tests/test_gpu_mirror.py, one per line:
+//   k1|r1 <sig hex> <digest hex> <expected pub hex>
+//   ed <sig hex> <pubkey hex> <message hex (32 bytes)> <0|1>
+// and checks every result; also checks the error behaviour (length mismatch, bad limb size).
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "sigops.hpp"
+
+template <size_t N>
+static std::array<uint8_t, N> hex(const std::string& s) {
+  std::array<uint8_t, N> a{};
+  for (size_t i = 0; i < N; i++) a[i] = (uint8_t)std::stoul(s.substr(2 * i, 2), nullptr, 16);
+  return a;
+}
+static std::vector<uint8_t> hexv(const std::string& s) {
+  std::vector<uint8_t> v(s.size() / 2);
+  for (size_t i = 0; i < v.size(); i++) v[i] = (uint8_t)std::stoul(s.substr(2 * i, 2), nullptr, 16);
+  return v;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) return 2;
+  std::ifstream in(argv[1]);
+  std::vector<sigops::Signature> k1s, r1s;
+  std::vector<sigops::Message> k1d, r1d, edm;
+  std::vector<std::vector<uint8_t>> k1e, r1e;
+  std::vector<sigops::Ed25519Signature> eds;
+  std::vector<sigops::VerifyingKey> edk;
+  std::vector<bool> ede;
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    std::string kind, a, b, c, d;
+    ss >> kind >> a >> b >> c;
+    if (kind == "k1") { k1s.push_back(hex<64>(a)); k1d.push_back(hex<32>(b)); k1e.push_back(hexv(c)); }
+    if (kind == "r1") { r1s.push_back(hex<64>(a)); r1d.push_back(hex<32>(b)); r1e.push_back(hexv(c)); }
+    if (kind == "ed") { ss >> d; eds.push_back(hex<64>(a)); edk.push_back(hex<32>(b)); edm.push_back(hex<32>(c)); ede.push_back(d == "1"); }
+  }
+  // the tables as the reference builds them: 16 points x {x,y} (x {x,y,t}) x 20 limbs at 13 bits
+  std::vector<uint32_t> t2(16 * 2 * 20), t3(16 * 3 * 20);
+  int bad = 0;
+  auto r = sigops::secp256k1_ecdsa::ecrecover(k1s, k1d, t2, 13);
+  for (size_t i = 0; i < r.size(); i++) bad += r[i] != k1e[i];
+  auto r2 = sigops::secp256k1_ecdsa::ecrecover_single(k1s, k1d, 13);
+  bad += r2 != r;
+  auto q = sigops::secp256r1_ecdsa::ecrecover(r1s, r1d, t2, 13);
+  for (size_t i = 0; i < q.size(); i++) bad += q[i] != r1e[i];
+  auto v = sigops::ed25519_eddsa::ecverify(eds, edm, edk, t3, 13);
+  for (size_t i = 0; i < v.size(); i++) bad += v[i] != ede[i];
+  bad += sigops::ed25519_eddsa::ecverify_single(eds, edm, edk, 13) != v;
+  bad += !sigops::secp256k1_ecdsa::ecrecover({}, {}, t2, 13).empty();
+  int thrown = 0;
+  try { sigops::secp256k1_ecdsa::ecrecover(k1s, {}, t2, 13); } catch (const std::invalid_argument&) { thrown++; }
+  try { sigops::secp256k1_ecdsa::ecrecover_single_shader(k1s, k1d, 9); } catch (const std::invalid_argument&) { thrown++; }
+  try { sigops::secp256r1_ecdsa::ecrecover(r1s, r1d, std::vector<uint32_t>(3), 13); } catch (const std::invalid_argument&) { thrown++; }
+  bad += thrown != 3;
+  std::cout << "mirror: k1 " << r.size() << " r1 " << q.size() << " ed " << v.size() << " mismatches " << bad
+            << std::endl;
+  return bad ? 1 : 0;
+}

@@ -1,0 +1,42 @@
+"""The C++ mirror of the reference API (include/sigops.hpp) run on the GPU through the C ABI:
+the reference's secp256k1 known-answer vector, generated signatures of both curves (valid and
+corrupted, expected values from the oracle) and Ed25519 with 32-byte messages as the reference
+passes them (RFC-style signing of SHA-256 digests), plus the mirror's error behaviour."""
+import hashlib
+import json
+import os
+import subprocess
+
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_cpp_mirror(tmp_path):
+    exe = os.path.join(HERE, "cpp", "mirror_test")
+    assert os.path.exists(exe), "tests/cpp/mirror_test is built by __graft_entry__.build()"
+    lines = []
+    kat = json.load(open(os.path.join(HERE, "golden", "secp256k1_kat.json")))
+    lines.append(f"k1 {kat['signature']} {kat['digest']} {kat['pubkey']}")
+    for curve, tag in ((0, "k1"), (1, "r1")):
+        sigs, dig, pubs, cor = O.gen_secp(curve, 77, 300, corrupt_ppm=100000)
+        out, st = O.recover_batch(curve, sigs, dig)
+        for i in range(300):
+            lines.append(f"{tag} {bytes(sigs[i]).hex()} {bytes(dig[i]).hex()} {bytes(out[i]).hex()}")
+    for i in range(60):
+        seed = hashlib.sha256(b"seed%d" % i).digest()
+        msg = hashlib.sha256(b"message%d" % i).digest()  # fuel Message::new: SHA-256 of the data
+        sig = bytearray(O.ed25519_sign(seed, msg))
+        if i % 7 == 3:
+            sig[40] ^= 1
+        pk = O.ed25519_pubkey(seed)
+        ok = O.ed25519_verify(bytes(sig), pk, msg)
+        lines.append(f"ed {bytes(sig).hex()} {pk.hex()} {msg.hex()} {int(ok)}")
+    p = tmp_path / "vectors.txt"
+    p.write_text("\n".join(lines) + "\n")
+    res = subprocess.run([exe, str(p)], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stdout + res.stderr
