@@ -47,6 +47,21 @@ def mixed_mac_per_sig(schemes, lens):
     return float(total) / len(schemes)
 
 
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hbm_traffic_and_valu_k_batch_run_mixed_2p20.md")
+
+
+def read_traffic():
+    """HBM bytes per launch of the persistent kernel on this workload, from the committed summary of
+    the FETCH_SIZE / WRITE_SIZE counter passes (each scaled by a copy of known size)."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), os.path.relpath(TRAFFIC_PROFILE, REPO)
+    except OSError:
+        pass
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +132,18 @@ def main():
     ms_per_step = total_ms / args.steps
     value = n / (ms_per_step / 1e3)
     log(f"batched: {ms_per_step:.2f} ms/step, {value / 1e6:.2f} M sigs/s (steps {min(step_ms):.2f}..{max(step_ms):.2f})")
+
+    # the dominant kernel alone: eager launches with HIP events recorded by the library on the
+    # launch stream around each stage; averaged over the same number of steps
+    ctx.set_stage_timing(True)
+    batched_step()
+    stage_runs = []
+    for _ in range(args.steps):
+        batched_step()
+        stage_runs.append(ctx.batch_stage_ms())
+    ctx.set_stage_timing(False)
+    stage_ms = {k: sum(r[k] for r in stage_runs) / len(stage_runs) for k in stage_runs[0]}
+    log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
 
     # correctness of the timed output by the round-trip property (all 2^20 items)
     out_h, st_h, cor = out.cpu().numpy(), st.cpu().numpy(), mx.corrupted.cpu().numpy()
@@ -239,14 +266,18 @@ def main():
                     f"{served[-1]['sustained_sigs_per_s'] / 1e6:.2f} M sigs/s sustained, identical: {same}")
             served_batches = srv.stats()
 
-    # roofline: dominant kernel = the persistent batch kernel; its launch time is the step time
-    # less the small bucketing kernels (measured by rocprofv3 in profiles/), bounded here by the step
+    # roofline of the dominant kernel (the persistent verification kernel): algorithmic MACs of the
+    # launch over its own event-timed duration; traffic from the calibrated counter passes
     mac_sig = mixed_mac_per_sig(sch, mx.lens)
-    achieved = value * mac_sig / 1e12
+    kern_ms = stage_ms["verify_kernel"]
+    achieved = n * mac_sig / (kern_ms / 1e3) / 1e12
     peak = mad_per_s / 1e12
+    traffic, traffic_src = read_traffic()
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
                 "unit": "TMAC/s (u32 x u32 -> u64 multiply-accumulate)", "frac": round(achieved / peak, 4),
-                "traffic": None, "mac_per_sig": round(mac_sig), "peak_source": "sigops_probe_valu (v_mad_u64_u32)"}
+                "traffic": traffic, "traffic_unit": "B per launch", "traffic_source": traffic_src,
+                "kernel": "k_batch_run", "kernel_ms": round(kern_ms, 3), "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+                "mac_per_sig": round(mac_sig), "peak_source": "sigops_probe_valu (v_mad_u64_u32)"}
 
     # CPU baseline: the oracle on a bounded sample of the same items (rank 0, N=1)
     cpu = None

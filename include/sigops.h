@@ -118,6 +118,16 @@ int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs,
                      const void* d_msgs, const uint64_t* d_msg_off, size_t n, void* d_out, void* d_status,
                      void* stream);
 
+/* ---- per-stage timing of the mixed batch (no counterpart in the reference; for profiling).
+   With timing on, sigops_batch / sigops_batch_dev launch the stages eagerly (no graph) with HIP
+   events recorded on the launch stream between them. sigops_batch_stage_ms waits for the last
+   batch and writes SIGOPS_STAGES durations in milliseconds: [0] bucketing (clear, histogram,
+   scan, scatter), [1] Ed25519 challenge hashing, [2] the persistent verification kernel,
+   [3] the careful pass for exceptional additions. ---- */
+#define SIGOPS_STAGES 4
+int sigops_set_stage_timing(sigops_ctx* ctx, int on);
+int sigops_batch_stage_ms(sigops_ctx* ctx, float* ms);
+
 /* ---- serving: continuous batching of verification requests. No counterpart in the reference.
    Requests (each a small mixed batch in the sigops_batch layout) are queued on submit and packed,
    in arrival order, into the next free slot (a context with its own stream and scratch) while the

@@ -82,6 +82,8 @@ def lib():
             L.sigops_ed25519_verify_dev.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp]
             L.sigops_batch.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, sz, c_p, c_p]
             L.sigops_batch_dev.argtypes = [vp, vp, vp, vp, vp, vp, sz, vp, vp, vp]
+            L.sigops_set_stage_timing.argtypes = [vp, ctypes.c_int]
+            L.sigops_batch_stage_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
             L.sigops_gen_secp_dev.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp,
                                               vp]
             L.sigops_gen_ed25519_dev.argtypes = [vp, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
@@ -215,6 +217,18 @@ class Context:
     def batch_dev(self, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status, stream=0):
         self._check(lib().sigops_batch_dev(self._h, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status,
                                            stream or None))
+
+    STAGES = ("bucketing", "ed25519_hash", "verify_kernel", "careful")
+
+    def set_stage_timing(self, on):
+        """Eager batch launches with HIP events between the stages (see batch_stage_ms)."""
+        self._check(lib().sigops_set_stage_timing(self._h, 1 if on else 0))
+
+    def batch_stage_ms(self):
+        """Stage durations (ms) of the last batch launched with stage timing on, keyed by STAGES."""
+        ms = (ctypes.c_float * len(self.STAGES))()
+        self._check(lib().sigops_batch_stage_ms(self._h, ms))
+        return dict(zip(self.STAGES, (float(x) for x in ms)))
 
     def gen_secp_dev(self, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs, d_corrupted=None, stream=0):
         self._check(lib().sigops_gen_secp_dev(self._h, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs,

@@ -271,17 +271,25 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   a.rank = L.rank;
   a.tile_items = 64u * L.tile_rounds;
   a.kbuf = (const uint4*)L.kbuf;
+  // stage events exist only on the eager (timed) path, never under graph capture
+  auto mark = [&](int i) -> hipError_t { return L.stage_ev ? hipEventRecord(L.stage_ev[i], s) : hipSuccess; };
+  hipError_t e;
+  if ((e = mark(0)) != hipSuccess) return e;
   // counters are cleared with a kernel (not a memset node) so the sequence is safe to capture
   k_zero_u32<<<1, 256, 0, s>>>(L.work, BATCH_WORK_WORDS);
   k_bucket_hist<<<L.aux_grid, 256, 0, s>>>(a);
   k_bucket_scan<<<1, 64, 0, s>>>(a);
   k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
+  if ((e = mark(1)) != hipSuccess) return e;
   // Ed25519 challenges, hashing the items in bucket order (equal SHA-512 block counts per wavefront);
   // secp entries of the order array are skipped by the scheme test inside
   k_batch_hash<<<L.grid, 256, 0, s>>>(a);
+  if ((e = mark(2)) != hipSuccess) return e;
   k_batch_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                      (uint4*)L.lane_tab, L.slots);
+  if ((e = mark(3)) != hipSuccess) return e;
   k_batch_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);
+  if ((e = mark(4)) != hipSuccess) return e;
   return hipGetLastError();
 }
 

@@ -49,6 +49,7 @@ struct BatchLaunch {
   void* lane_tab;
   void* kbuf;  // n x 32 bytes: Ed25519 challenges
   uint32_t slots, grid, aux_grid, tile_rounds;
+  hipEvent_t* stage_ev;  // null, or 5 events recorded before, between and after the 4 stages
 };
 
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);

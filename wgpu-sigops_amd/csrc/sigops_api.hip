@@ -52,6 +52,9 @@ struct sigops_ctx {
       graphs;
   DevBuf in, aux, msg, off, out, st, sch;
   void* d_kbuf = nullptr;  // max_batch x 32 bytes: Ed25519 challenges
+  bool stage_timing = false;
+  bool stage_recorded = false;
+  hipEvent_t stage_ev[SIGOPS_STAGES + 1] = {};
   std::vector<uint8_t> host_status;
   std::vector<uint64_t> host_off;
   std::string err;
@@ -208,6 +211,8 @@ void sigops_destroy(sigops_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (hipEvent_t e : c->stage_ev)
+    if (e) (void)hipEventDestroy(e);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
                   c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
                   (void*)c->d_order, (void*)c->d_n, c->d_kbuf})
@@ -343,10 +348,17 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
   L.grid = c->grid;
   L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
   L.tile_rounds = c->tile_rounds;
+  L.stage_ev = nullptr;
   // n goes to device memory first (stream-ordered, by value at enqueue time), so one captured graph
   // serves every n and back-to-back enqueues do not race
   k_set_u32<<<1, 1, 0, s>>>(c->d_n, (uint32_t)n);
   HIPCHK(c, hipGetLastError());
+  if (c->stage_timing) {
+    L.stage_ev = c->stage_ev;
+    HIPCHK(c, sigops::launch_batch(L, s));
+    c->stage_recorded = true;
+    return SIGOPS_OK;
+  }
   if (!c->use_graph) {
     HIPCHK(c, sigops::launch_batch(L, s));
     return SIGOPS_OK;
@@ -368,6 +380,25 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
     it = c->graphs.emplace(key, ex).first;
   }
   HIPCHK(c, hipGraphLaunch(it->second, s));
+  return SIGOPS_OK;
+}
+
+int sigops_set_stage_timing(sigops_ctx* c, int on) {
+  if (!c) return SIGOPS_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (on)
+    for (hipEvent_t& e : c->stage_ev)
+      if (!e) HIPCHK(c, hipEventCreate(&e));
+  c->stage_timing = on != 0;
+  c->stage_recorded = false;
+  return SIGOPS_OK;
+}
+
+int sigops_batch_stage_ms(sigops_ctx* c, float* ms) {
+  if (!c || !ms) return SIGOPS_E_ARG;
+  if (!c->stage_recorded) return fail(c, SIGOPS_E_ARG, "no timed batch since stage timing was switched on");
+  HIPCHK(c, hipEventSynchronize(c->stage_ev[SIGOPS_STAGES]));
+  for (int i = 0; i < SIGOPS_STAGES; i++) HIPCHK(c, hipEventElapsedTime(&ms[i], c->stage_ev[i], c->stage_ev[i + 1]));
   return SIGOPS_OK;
 }
 
