@@ -10,6 +10,17 @@
 using namespace sigops;
 
 namespace {
+// the generator tables hold canonical values (< p); the bound checker is told so
+template <class F>
+void canonical_bounds(fe& x) {
+#if SIGOPS_BOUNDS
+  BOUNDS_SET(x, LMASK, F::M[NL - 1], mod_ld<F>());
+  bcheck_actual(x, "table entry");
+#else
+  (void)x;
+#endif
+}
+template <class F>
 struct GTabHost {
   const u32* base;
   void load(u32 idx, fe& x, fe& y) const {
@@ -18,6 +29,8 @@ struct GTabHost {
       x.v[k] = e[k];
       y.v[k] = e[9 + k];
     }
+    canonical_bounds<F>(x);
+    canonical_bounds<F>(y);
   }
 };
 struct BTabHost {
@@ -28,6 +41,9 @@ struct BTabHost {
       ymx.v[k] = e[9 + k];
       t2d.v[k] = e[18 + k];
     }
+    canonical_bounds<ModPed>(ypx);
+    canonical_bounds<ModPed>(ymx);
+    canonical_bounds<ModPed>(t2d);
   }
 };
 struct JTabHost {
@@ -53,7 +69,7 @@ int recover_one(const unsigned char* sig, const unsigned char* dig, unsigned cha
   memcpy(dr, dig, 32);
   secp_pro pro;
   secp_prologue<C>(sr, dr, pro);
-  GTabHost gt{gtab};
+  GTabHost<typename C::Fp> gt{gtab};
   memset(out, 0, 64);
   if (pro.status != ST_OK) return (int)pro.status;
   bool exc = false;
@@ -97,6 +113,29 @@ int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const u
 }
 
 // field op self-check: op 0 mul, 1 sqr, 2 inv, 3 add, 4 sub; mod 0..5; values as 32 BE bytes (plain)
+// Raw Montgomery operations on arbitrary limb vectors (no conversion, no canonicalisation):
+// op 0 = mont_mul(a, b), 1 = mont_sqr(a). Inputs and output are 9 u32 limbs.
+void dc_mont_raw(int op, int mod, const u32* a, const u32* b, u32* out) {
+  bounds_enabled() = false;  // raw limb vectors may lie outside the invariants the checker enforces
+  fe x, y, r;
+  for (int k = 0; k < NL; k++) {
+    x.v[k] = a[k];
+    y.v[k] = b[k];
+  }
+#define RAW(F) r = op == 0 ? mont_mul<F>(x, y) : mont_sqr<F>(x);
+  switch (mod) {
+    case 0: RAW(ModPk1) break;
+    case 1: RAW(ModNk1) break;
+    case 2: RAW(ModPr1) break;
+    case 3: RAW(ModNr1) break;
+    case 4: RAW(ModPed) break;
+    default: RAW(ModL) break;
+  }
+#undef RAW
+  for (int k = 0; k < NL; k++) out[k] = r.v[k];
+  bounds_enabled() = true;
+}
+
 void dc_field(int op, int mod, const unsigned char* a, const unsigned char* b, unsigned char* out) {
   u32 ra[8], rb[8], wa[8], wb[8], wo[8], ro[8];
   memcpy(ra, a, 32);
@@ -143,8 +182,8 @@ void dc_sha512_ed(const unsigned char* hdr64, const unsigned char* msg, unsigned
 void dc_gen_secp(int curve, unsigned long long seed, unsigned long long i, unsigned ppm, unsigned char* sig,
                  unsigned char* dig, unsigned char* pub, unsigned* corrupted) {
   u32 s[16], dg[8], p[16], cor;
-  if (curve == 0) gen_secp_item<CurveK1>(seed, i, ppm, GTabHost{K1_GTABLE}, s, dg, p, cor);
-  else gen_secp_item<CurveR1>(seed, i, ppm, GTabHost{R1_GTABLE}, s, dg, p, cor);
+  if (curve == 0) gen_secp_item<CurveK1>(seed, i, ppm, GTabHost<ModPk1>{K1_GTABLE}, s, dg, p, cor);
+  else gen_secp_item<CurveR1>(seed, i, ppm, GTabHost<ModPr1>{R1_GTABLE}, s, dg, p, cor);
   memcpy(sig, s, 64);
   memcpy(dig, dg, 32);
   memcpy(pub, p, 64);

@@ -16,7 +16,7 @@ def build():
     deps = [SRC] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
     if os.path.exists(SO) and all(os.path.getmtime(SO) >= os.path.getmtime(d) for d in deps):
         return SO
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-I" + CSRC,
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-DSIGOPS_HOST_CHECK=1", "-I" + CSRC,
                     "-o", SO, SRC], check=True)
     return SO
 
@@ -30,6 +30,8 @@ def lib():
         L.dc_ed25519_verify.argtypes = [c_p, c_p, c_p, ctypes.c_uint]
         L.dc_field.argtypes = [ctypes.c_int, ctypes.c_int, c_p, c_p, c_p]
         L.dc_sha512_ed.argtypes = [c_p, c_p, ctypes.c_uint, c_p]
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.dc_mont_raw.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p]
         _lib = L
     return _lib
 
@@ -54,3 +56,12 @@ def sha512_ed(hdr, msg):
     out = ctypes.create_string_buffer(64)
     lib().dc_sha512_ed(hdr, msg, len(msg), out)
     return out.raw
+
+
+def mont_raw(op, mod, a_limbs, b_limbs):
+    """mont_mul (op 0) or mont_sqr (op 1) of raw 9-limb vectors; returns the 9 output limbs"""
+    A = (ctypes.c_uint32 * 9)(*a_limbs)
+    B = (ctypes.c_uint32 * 9)(*b_limbs)
+    O = (ctypes.c_uint32 * 9)()
+    lib().dc_mont_raw(op, mod, A, B, O)
+    return list(O)

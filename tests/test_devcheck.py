@@ -28,6 +28,40 @@ def test_field_ops(mod):
             assert D.field(2, mod, a) == pow(a, -1, m)
 
 
+def _val(limbs):
+    return sum(v << (29 * i) for i, v in enumerate(limbs))
+
+
+def _limbs(x):
+    return [(x >> (29 * i)) & (2**29 - 1) for i in range(9)]
+
+
+@pytest.mark.parametrize("mod", range(6))
+def test_montgomery_raw_limbs(mod):
+    """mont_mul / mont_sqr on raw limb vectors: the output is congruent to a*b/2^261, normalised,
+    and below 2M whenever a, b < 2M. Includes the column-sum extremes (every limb 2^29-1), where
+    an overflowing or sign-confused column accumulator would show."""
+    m = MODULI[mod]
+    R = 2**261
+    rinv = pow(R, -1, m)
+    rnd = random.Random(100 + mod)
+    cases = [(2 * m - 1, 2 * m - 1), (0, 2 * m - 1), (1, 1), (m, m), (m - 1, 2 * m - 1)]
+    cases += [(rnd.randrange(2 * m), rnd.randrange(2 * m)) for _ in range(200)]
+    # limbs all at the maximum: value 2^261 - 1 (outside [0, 2M), checked for congruence only)
+    top = [2**29 - 1] * 9
+    raw = [(_limbs(a), _limbs(b), True) for a, b in cases] + [(top, top, False), (top, _limbs(2 * m - 1), False)]
+    for al, bl, in_range in raw:
+        a, b = _val(al), _val(bl)
+        for op in (0, 1):
+            rb = al if op == 1 else bl
+            out = D.mont_raw(op, mod, al, rb)
+            v = _val(out)
+            assert v % m == a * _val(rb) * rinv % m, (mod, op, al, rb)
+            assert all(x < 2**29 for x in out[:8])
+            if in_range:
+                assert v < 2 * m and out[8] < 2**29
+
+
 def test_sha512_preimage_stream():
     rnd = random.Random(2)
     for L in list(range(0, 260)) + [1000, 4096]:

@@ -40,31 +40,37 @@ DEV ext ext_identity() {
   return r;
 }
 
+// Group law on lazy field arithmetic (field.hpp). Inputs: coordinates normalised with value < 2M;
+// cached ypx / ymx normalised, z and t2d products (t2d possibly a lazy negation [2, 2]).
+// Bounds of intermediates as [limb bound in units of 2^29, value bound in units of M]; a product
+// needs La*Lb < 7.1 and Va*Vb < 64 (M = 2^255 - 19, R = 2^261).
 DEV cached to_cached(const ext& P) {
   cached c;
-  c.ypx = fadd<Fe>(P.Y, P.X);
-  c.ymx = fsub<Fe>(P.Y, P.X);
+  c.ypx = fnorm_red<Fe>(fadd_nr(P.Y, P.X));
+  c.ymx = fnorm_red<Fe>(fsub_nr<Fe, 3, 1>(P.Y, P.X));  // P.X may be any value < 2M
   c.z = P.Z;
   c.t2d = mont_mul<Fe>(P.T, fe_from_const(ED_D2));
   return c;
 }
 
-DEV cached neg_cached(const cached& c) {
+// -Q when neg: swap Y+X and Y-X and negate 2dT (lazily)
+DEV cached neg_cached_if(bool neg, const cached& c) {
+  const u32 m = lane_mask(neg);
   cached r;
-  r.ypx = c.ymx;
-  r.ymx = c.ypx;
+  r.ypx = fsel_mask(m, c.ymx, c.ypx);
+  r.ymx = fsel_mask(m, c.ypx, c.ymx);
   r.z = c.z;
-  r.t2d = fneg<Fe>(c.t2d);
+  r.t2d = fsel_mask(m, fsub_nr<Fe, 2, 1>(fe_zero(), c.t2d), c.t2d);
   return r;
 }
 
-// completed point (X:Y:Z:T) with x = X/Z, y = Y/T -> extended
-DEV ext completed_to_ext(const fe& X, const fe& Y, const fe& Z, const fe& T, bool want_t) {
+// completed point (E:G:H:Tc) with x = E/Tc, y = G/H -> extended; each pair product within bounds
+DEV ext completed_to_ext(const fe& E, const fe& G, const fe& H, const fe& Tc, bool want_t) {
   ext r;
-  r.X = mont_mul<Fe>(X, T);
-  r.Y = mont_mul<Fe>(Y, Z);
-  r.Z = mont_mul<Fe>(Z, T);
-  r.T = want_t ? mont_mul<Fe>(X, Y) : fe_zero();
+  r.X = mont_mul<Fe>(E, Tc);
+  r.Y = mont_mul<Fe>(G, H);
+  r.Z = mont_mul<Fe>(H, Tc);
+  r.T = want_t ? mont_mul<Fe>(E, G) : fe_zero();
   return r;
 }
 
@@ -72,28 +78,38 @@ DEV ext completed_to_ext(const fe& X, const fe& Y, const fe& Z, const fe& T, boo
 DEV ext edbl(const ext& P, bool want_t) {
   const fe XX = mont_sqr<Fe>(P.X);
   const fe YY = mont_sqr<Fe>(P.Y);
-  const fe ZZ2 = fdbl<Fe>(mont_sqr<Fe>(P.Z));
-  const fe XpY2 = mont_sqr<Fe>(fadd<Fe>(P.X, P.Y));
-  const fe YYpXX = fadd<Fe>(YY, XX);
-  const fe YYmXX = fsub<Fe>(YY, XX);
-  return completed_to_ext(fsub<Fe>(XpY2, YYpXX), YYpXX, YYmXX, fsub<Fe>(ZZ2, YYmXX), want_t);
+  const fe ZZ = mont_sqr<Fe>(P.Z);
+  const fe XpY2 = mont_sqr<Fe>(fadd_nr(P.X, P.Y));           // [2, 4]^2
+  const fe G = fadd_nr(YY, XX);                               // YY + XX: [2, 2.2]
+  const fe H = fsub_nr<Fe, 2, 1>(YY, XX);                     // YY - XX: [3, 3.1]
+  const fe E = fnorm_red<Fe>(fsub_nr<Fe, 3, 2>(XpY2, G));     // 2XY
+  const fe Tc = fnorm_red<Fe>(fsub_nr<Fe, 2, 1>(fadd_nr(fscale_nr(ZZ, 2), XX), YY));  // 2ZZ - (YY - XX)
+  return completed_to_ext(E, G, H, Tc, want_t);               // [1]x[1], [2]x[3], [3]x[1], [1]x[2]
 }
 
 DEV ext eadd_cached(const ext& P, const cached& Q, bool want_t) {
-  const fe PP = mont_mul<Fe>(fadd<Fe>(P.Y, P.X), Q.ypx);
-  const fe MM = mont_mul<Fe>(fsub<Fe>(P.Y, P.X), Q.ymx);
+  const fe PP = mont_mul<Fe>(fadd_nr(P.Y, P.X), Q.ypx);
+  const fe MM = mont_mul<Fe>(fsub_nr<Fe, 3, 1>(P.Y, P.X), Q.ymx);  // P.X: any value < 2M
   const fe TT2d = mont_mul<Fe>(P.T, Q.t2d);
-  const fe ZZ2 = fdbl<Fe>(mont_mul<Fe>(P.Z, Q.z));
-  return completed_to_ext(fsub<Fe>(PP, MM), fadd<Fe>(PP, MM), fadd<Fe>(ZZ2, TT2d), fsub<Fe>(ZZ2, TT2d), want_t);
+  const fe ZZ2 = mont_mul<Fe>(fscale_nr(P.Z, 2), Q.z);
+  const fe E = fnorm_red<Fe>(fsub_nr<Fe, 2, 1>(PP, MM));
+  const fe G = fadd_nr(PP, MM);                               // [2, 2.x]
+  const fe H = fadd_nr(ZZ2, TT2d);                            // [2, 2.x]
+  const fe Tc = fsub_nr<Fe, 2, 1>(ZZ2, TT2d);                 // [3, 3.x]
+  return completed_to_ext(E, G, H, Tc, want_t);               // [1]x[3], [2]x[2], [2]x[3], [1]x[2]
 }
 
-// affine Niels (y+x, y-x, 2d*x*y), optionally negated
+// affine Niels (y+x, y-x, 2d*x*y); t2d possibly a lazy negation
 DEV ext eadd_niels(const ext& P, const fe& ypx, const fe& ymx, const fe& t2d, bool want_t) {
-  const fe PP = mont_mul<Fe>(fadd<Fe>(P.Y, P.X), ypx);
-  const fe MM = mont_mul<Fe>(fsub<Fe>(P.Y, P.X), ymx);
+  const fe PP = mont_mul<Fe>(fadd_nr(P.Y, P.X), ypx);
+  const fe MM = mont_mul<Fe>(fsub_nr<Fe, 3, 1>(P.Y, P.X), ymx);
   const fe TT2d = mont_mul<Fe>(P.T, t2d);
-  const fe ZZ2 = fdbl<Fe>(P.Z);
-  return completed_to_ext(fsub<Fe>(PP, MM), fadd<Fe>(PP, MM), fadd<Fe>(ZZ2, TT2d), fsub<Fe>(ZZ2, TT2d), want_t);
+  const fe ZZ2 = fscale_nr(P.Z, 2);                           // [2, 4]
+  const fe E = fnorm_red<Fe>(fsub_nr<Fe, 2, 1>(PP, MM));
+  const fe G = fadd_nr(PP, MM);                               // [2, 2.x]
+  const fe H = fadd_nr(ZZ2, TT2d);                            // [3, 5.x]
+  const fe Tc = fnorm_red<Fe>(fsub_nr<Fe, 2, 1>(ZZ2, TT2d));
+  return completed_to_ext(E, G, H, Tc, want_t);               // [1]x[1], [2]x[3], [3]x[1], [1]x[2]
 }
 
 // dalek FieldElement::sqrt_ratio_i (field.rs): returns was_nonzero_square, r = nonnegative root
@@ -212,25 +228,21 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
     acc = edbl(acc, false);
     acc = edbl(acc, false);
     acc = edbl(acc, true);
-    if (neg) E = neg_cached(E);
+    E = neg_cached_if(neg, E);
     acc = eadd_cached(acc, E, true);
     if ((i & 1) == 0) {
       const u32 cb = byte8(c1, i >> 1);
       const bool bneg = (cb < 128) != b_neg;
       fe ypx, ymx, t2d;
       bt.load(((cb < 128) ? ~cb : cb) & 127u, ypx, ymx, t2d);
-      if (bneg) {
-        const fe tmp = ypx;
-        ypx = ymx;
-        ymx = tmp;
-        t2d = fneg<Fe>(t2d);
-      }
-      acc = eadd_niels(acc, ypx, ymx, t2d, true);
+      const u32 m = lane_mask(bneg);
+      acc = eadd_niels(acc, fsel_mask(m, ymx, ypx), fsel_mask(m, ypx, ymx),
+                       fsel_mask(m, fsub_nr<Fe, 2, 1>(fe_zero(), t2d), t2d), true);
     }
   }
   {
     // + A when k was made odd by adding one, else + identity (complete formulas)
-    cached corr = neg_cached(tab.load(0));
+    cached corr = neg_cached_if(true, tab.load(0));
     if (!k_even) {
       corr.ypx = one;
       corr.ymx = one;

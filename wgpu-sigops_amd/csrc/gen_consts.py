@@ -88,9 +88,38 @@ def ed_base():
     return (x, y)
 
 
+# Sparse signed forms m = sum coef * 2^(29 * off) of the three field primes: a Montgomery step then
+# adds coef * m_i at column i + off for each term (2 to 5 terms instead of 9 limb products).
+SPARSE = {
+    P_K1: [(0, -977), (1, -8), (8, 2**24)],                                # 2^256 - 2^32 - 977
+    # 2^256 - 2^224 + 2^192 + 2^96 - 1, with -2^224 = (2^29 - 2^21) 2^203 - 2^232 so that every term
+    # but the column-0 -1 (which needs no instruction) is non-negative
+    P_R1: [(0, -1), (3, 2**9), (6, 2**18), (7, 2**29 - 2**21), (8, 2**24 - 1)],
+    P_ED: [(0, -19), (8, 2**23)],                                          # 2^255 - 19
+}
+
+
+def sparse_block(m):
+    terms = SPARSE.get(m)
+    if terms is None:
+        return "  static constexpr bool SPARSE = false;\n  static constexpr int SP_N = 1;\n" \
+               "  static constexpr int SP_OFF[1] = {0};\n  static constexpr int32_t SP_COEF[1] = {0};\n"
+    assert sum(c * 2**(LIMB_BITS * o) for o, c in terms) == m
+    assert terms[0][0] == 0 and all(o > 0 for o, _ in terms[1:])
+    assert all(abs(c) < 2**31 for _, c in terms)
+    # the Montgomery digit only sees the column-0 coefficient: m = coef0 (mod 2^29)
+    assert (m - terms[0][1]) % 2**LIMB_BITS == 0
+    offs = ", ".join(str(o) for o, _ in terms)
+    coefs = ", ".join(str(c) for _, c in terms)
+    return (f"  static constexpr bool SPARSE = true;\n  static constexpr int SP_N = {len(terms)};\n"
+            f"  static constexpr int SP_OFF[{len(terms)}] = {{{offs}}};\n"
+            f"  static constexpr int32_t SP_COEF[{len(terms)}] = {{{coefs}}};\n")
+
+
 def modulus_block(tag, m):
     out = f"// ---- {tag}: m = 0x{m:064x}\n"
     out += f"struct {tag} {{\n"
+    out += sparse_block(m)
     out += "  " + arr("M", limbs(m))
     out += "  " + arr("M2", limbs(2 * m))
     out += f"  static constexpr uint32_t N0 = 0x{(-pow(m, -1, 2**LIMB_BITS)) % 2**LIMB_BITS:08x}u;\n"

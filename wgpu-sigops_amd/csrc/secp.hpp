@@ -51,41 +51,44 @@ struct jac {
   fe X, Y, Z;
 };
 
+// Point formulas on lazy field arithmetic (field.hpp). Inputs: coordinates normalised with value
+// < 2M. Outputs: X3, Y3 from fnorm_red (value < (1+2^-19) M), Z3 from a multiplication (< 2M).
+// Intermediate bounds are noted as [limb bound in units of 2^29, value bound in units of M];
+// a product needs La*Lb < 7.1 and Va*Vb < 32.
+
 template <class C>
 DEV jac jdbl(const jac& P) {
   typedef typename C::Fp F;
   jac R;
   if (!C::A_IS_M3) {
-    // dbl-2009-l: 2M + 5S
-    const fe A = mont_sqr<F>(P.X);
-    const fe B = mont_sqr<F>(P.Y);
-    const fe Cc = mont_sqr<F>(B);
-    fe D = fsub<F>(fsub<F>(mont_sqr<F>(fadd<F>(P.X, B)), A), Cc);
-    D = fdbl<F>(D);
-    const fe E = fadd<F>(fdbl<F>(A), A);
-    const fe Fv = mont_sqr<F>(E);
-    R.X = fsub<F>(Fv, fdbl<F>(D));
-    const fe C8 = fdbl<F>(fdbl<F>(fdbl<F>(Cc)));
-    R.Y = fsub<F>(mont_mul<F>(E, fsub<F>(D, R.X)), C8);
-    R.Z = fdbl<F>(mont_mul<F>(P.Y, P.Z));
+    // a = 0, 3M + 4S: S = 4XY^2, M = 3X^2, X3 = M^2 - 2S, Y3 = M(S - X3) - 8Y^4, Z3 = 2YZ
+    const fe XX = mont_sqr<F>(P.X);
+    const fe YY = mont_sqr<F>(P.Y);                           // [1, 1.13]
+    const fe S = mont_mul<F>(fscale_nr(P.X, 4), YY);          // [4, 8] x [1, 1.13] -> [1, 1.29]
+    const fe Mn = fnorm_red<F>(fscale_nr(XX, 3));              // [1, 1+]
+    R.X = fnorm_red<F>(fsub_nr<F, 3, 2>(mont_sqr<F>(Mn), fscale_nr(S, 2)));
+    const fe Y4x4 = mont_sqr<F>(fscale_nr(YY, 2));            // 4 Y^4: [2, 2.26]^2 -> [1, 1.16]
+    const fe MSX = mont_mul<F>(Mn, fsub_nr<F, 2, 1>(S, R.X));  // [1, 1+] x [3, 3.29]
+    R.Y = fnorm_red<F>(fsub_nr<F, 3, 2>(MSX, fscale_nr(Y4x4, 2)));
+    R.Z = mont_mul<F>(fscale_nr(P.Y, 2), P.Z);                // [2, 4] x [1, 2]
   } else {
-    // dbl-2001-b (a = -3): 3M + 5S
+    // a = -3, dbl-2001-b with 4beta and 2YZ taken as products:
+    //   alpha = 3(X - Z^2)(X + Z^2), X3 = alpha^2 - 8beta, Y3 = alpha(4beta - X3) - 8Y^4, Z3 = 2YZ
     const fe delta = mont_sqr<F>(P.Z);
     const fe gamma = mont_sqr<F>(P.Y);
-    const fe beta = mont_mul<F>(P.X, gamma);
-    const fe t = mont_mul<F>(fsub<F>(P.X, delta), fadd<F>(P.X, delta));
-    const fe alpha = fadd<F>(fdbl<F>(t), t);
-    const fe beta4 = fdbl<F>(fdbl<F>(beta));
-    R.X = fsub<F>(mont_sqr<F>(alpha), fdbl<F>(beta4));
-    R.Z = fsub<F>(fsub<F>(mont_sqr<F>(fadd<F>(P.Y, P.Z)), gamma), delta);
-    const fe g2 = mont_sqr<F>(gamma);
-    const fe g8 = fdbl<F>(fdbl<F>(fdbl<F>(g2)));
-    R.Y = fsub<F>(mont_mul<F>(alpha, fsub<F>(beta4, R.X)), g8);
+    const fe beta4 = mont_mul<F>(fscale_nr(P.X, 4), gamma);   // [4, 8] x [1, 1.13]
+    const fe t = mont_mul<F>(fsub_nr<F, 2, 1>(P.X, delta), fadd_nr(P.X, delta));  // [3, 4] x [2, 3.13]
+    const fe alpha = fnorm_red<F>(fscale_nr(t, 3));
+    R.X = fnorm_red<F>(fsub_nr<F, 3, 2>(mont_sqr<F>(alpha), fscale_nr(beta4, 2)));
+    const fe g4 = mont_sqr<F>(fscale_nr(gamma, 2));            // 4 gamma^2
+    const fe aBX = mont_mul<F>(alpha, fsub_nr<F, 2, 1>(beta4, R.X));
+    R.Y = fnorm_red<F>(fsub_nr<F, 3, 2>(aBX, fscale_nr(g4, 2)));
+    R.Z = mont_mul<F>(fscale_nr(P.Y, 2), P.Z);
   }
   return R;
 }
 
-// add-2007-bl: 11M + 5S. exc is set when H == 0 (P == +-Q).
+// add-2007-bl: 11M + 5S. exc is set when H == 0 (P == +-Q). Q.Y may be a lazy negation [2, 3].
 template <class C>
 DEV jac jadd(const jac& P, const jac& Q, bool& exc) {
   typedef typename C::Fp F;
@@ -95,38 +98,48 @@ DEV jac jadd(const jac& P, const jac& Q, bool& exc) {
   const fe U2 = mont_mul<F>(Q.X, Z1Z1);
   const fe S1 = mont_mul<F>(mont_mul<F>(P.Y, Q.Z), Z2Z2);
   const fe S2 = mont_mul<F>(mont_mul<F>(Q.Y, P.Z), Z1Z1);
-  const fe H = fsub<F>(U2, U1);
+  const fe H = fnorm_red<F>(fsub_nr<F, 2, 1>(U2, U1));
   exc = exc || fis_zero<F>(H);
-  const fe I = mont_sqr<F>(fdbl<F>(H));
+  const fe I = mont_sqr<F>(fscale_nr(H, 2));
   const fe J = mont_mul<F>(H, I);
-  const fe r = fdbl<F>(fsub<F>(S2, S1));
+  const fe r = fnorm_red<F>(fscale_nr(fsub_nr<F, 2, 1>(S2, S1), 2));  // [6, 6.x] -> [1, 1+]
   const fe V = mont_mul<F>(U1, I);
   jac R;
-  R.X = fsub<F>(fsub<F>(mont_sqr<F>(r), J), fdbl<F>(V));
-  R.Y = fsub<F>(mont_mul<F>(r, fsub<F>(V, R.X)), fdbl<F>(mont_mul<F>(S1, J)));
-  R.Z = mont_mul<F>(fsub<F>(fsub<F>(mont_sqr<F>(fadd<F>(P.Z, Q.Z)), Z1Z1), Z2Z2), H);
+  R.X = fnorm_red<F>(fsub_nr<F, 4, 3>(mont_sqr<F>(r), fadd_nr(J, fscale_nr(V, 2))));
+  const fe rVX = mont_mul<F>(r, fsub_nr<F, 2, 1>(V, R.X));
+  R.Y = fnorm_red<F>(fsub_nr<F, 2, 1>(rVX, mont_mul<F>(fscale_nr(S1, 2), J)));
+  // ((Z1 + Z2)^2 - Z1Z1 - Z2Z2) H = 2 Z1 Z2 H
+  const fe Zs = mont_sqr<F>(fadd_nr(P.Z, Q.Z));             // [2, 4]^2
+  R.Z = mont_mul<F>(fsub_nr<F, 3, 2>(Zs, fadd_nr(Z1Z1, Z2Z2)), H);  // [4, 4.x] x [1, 1+]
   return R;
 }
 
-// madd-2007-bl (Q affine): 7M + 4S
+// madd-2007-bl (Q affine, y2 possibly a lazy negation [2, 3]): 8M + 3S
 template <class C>
 DEV jac jmadd(const jac& P, const fe& x2, const fe& y2, bool& exc) {
   typedef typename C::Fp F;
   const fe Z1Z1 = mont_sqr<F>(P.Z);
   const fe U2 = mont_mul<F>(x2, Z1Z1);
   const fe S2 = mont_mul<F>(mont_mul<F>(y2, P.Z), Z1Z1);
-  const fe H = fsub<F>(U2, P.X);
+  const fe H = fnorm_red<F>(fsub_nr<F, 2, 1>(U2, P.X));
   exc = exc || fis_zero<F>(H);
-  const fe HH = mont_sqr<F>(H);
-  const fe I = fdbl<F>(fdbl<F>(HH));
-  const fe J = mont_mul<F>(H, I);
-  const fe r = fdbl<F>(fsub<F>(S2, P.Y));
-  const fe V = mont_mul<F>(P.X, I);
+  const fe I4 = fscale_nr(mont_sqr<F>(H), 4);               // I = 4 H^2: [4, 4.x]
+  const fe J = mont_mul<F>(H, I4);
+  const fe r = fnorm_red<F>(fscale_nr(fsub_nr<F, 2, 1>(S2, P.Y), 2));
+  const fe V = mont_mul<F>(P.X, I4);
   jac R;
-  R.X = fsub<F>(fsub<F>(mont_sqr<F>(r), J), fdbl<F>(V));
-  R.Y = fsub<F>(mont_mul<F>(r, fsub<F>(V, R.X)), fdbl<F>(mont_mul<F>(P.Y, J)));
-  R.Z = fsub<F>(fsub<F>(mont_sqr<F>(fadd<F>(P.Z, H)), Z1Z1), HH);
+  R.X = fnorm_red<F>(fsub_nr<F, 4, 3>(mont_sqr<F>(r), fadd_nr(J, fscale_nr(V, 2))));
+  const fe rVX = mont_mul<F>(r, fsub_nr<F, 2, 1>(V, R.X));
+  R.Y = fnorm_red<F>(fsub_nr<F, 2, 1>(rVX, mont_mul<F>(fscale_nr(P.Y, 2), J)));
+  R.Z = mont_mul<F>(P.Z, fscale_nr(H, 2));                  // (Z1 + H)^2 - Z1Z1 - HH = 2 Z1 H
   return R;
+}
+
+// -y as a lazy value [2, 3] when neg, y otherwise (no carries, no v_cndmask). y < 2M can have a
+// top limb up to (2M >> 232), one above the top limb of 2M's LB = 1 bias, hence K = 3.
+template <class F>
+DEV fe fneg_if(bool neg, const fe& y) {
+  return fsel(neg, fsub_nr<F, 3, 1>(fe_zero(), y), y);
 }
 
 // ---- complete (branching) group law for the careful path ----
@@ -260,14 +273,14 @@ DEV jac secp_ladder(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
     jac E = tab.load((neg ? ~c : c) & 7u);
 #pragma unroll 1
     for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
-    if (neg) E.Y = fneg<F>(E.Y);
+    E.Y = fneg_if<F>(neg, E.Y);
     acc = jadd<C>(acc, E, exc);
     if ((i & 1) == 0) {
       const u32 cg = byte8(o.c1, i >> 1);
       const bool gneg = (cg < 128) != o.g_neg;
       fe gx, gy;
       gt.load(((cg < 128) ? ~cg : cg) & 127u, gx, gy);
-      if (gneg) gy = fneg<F>(gy);
+      gy = fneg_if<F>(gneg, gy);
       acc = jmadd<C>(acc, gx, gy, exc);
     }
   }
