@@ -136,6 +136,14 @@ void dc_mont_raw(int op, int mod, const u32* a, const u32* b, u32* out) {
   bounds_enabled() = true;
 }
 
+// GLV split of a secp256k1 scalar k (8 little-endian words, k < n): h1, h2 as 5 words each and
+// their signs (bit 0: h1 negative, bit 1: h2 negative)
+int dc_glv_split(const u32* k, u32* h1, u32* h2) {
+  bool n1, n2;
+  glv_split(k, h1, h2, n1, n2);
+  return (int)n1 | ((int)n2 << 1);
+}
+
 void dc_field(int op, int mod, const unsigned char* a, const unsigned char* b, unsigned char* out) {
   u32 ra[8], rb[8], wa[8], wb[8], wo[8], ro[8];
   memcpy(ra, a, 32);

@@ -32,6 +32,7 @@ def lib():
         L.dc_sha512_ed.argtypes = [c_p, c_p, ctypes.c_uint, c_p]
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.dc_mont_raw.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p]
+        L.dc_glv_split.argtypes = [u32p, u32p, u32p]
         _lib = L
     return _lib
 
@@ -65,3 +66,14 @@ def mont_raw(op, mod, a_limbs, b_limbs):
     O = (ctypes.c_uint32 * 9)()
     lib().dc_mont_raw(op, mod, A, B, O)
     return list(O)
+
+
+def glv_split(k):
+    """device GLV split of k < n (secp256k1): returns signed halves (k1, k2)"""
+    K = (ctypes.c_uint32 * 8)(*[(k >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+    H1 = (ctypes.c_uint32 * 5)()
+    H2 = (ctypes.c_uint32 * 5)()
+    sg = lib().dc_glv_split(K, H1, H2)
+    h1 = sum(v << (32 * i) for i, v in enumerate(H1))
+    h2 = sum(v << (32 * i) for i, v in enumerate(H2))
+    return (-h1 if sg & 1 else h1), (-h2 if sg & 2 else h2)

@@ -62,6 +62,28 @@ def test_montgomery_raw_limbs(mod):
                 assert v < 2 * m and out[8] < 2**29
 
 
+def test_glv_split_secp256k1():
+    """k = k1 + k2*lambda (mod n) with both halves odd and below 2^129 in magnitude. lambda is
+    pinned independently: lambda^3 = 1 (mod n) and lambda*G = (beta*Gx, Gy) with beta^3 = 1 (mod p),
+    checked with the oracle's own point multiplication."""
+    c = pyref.CURVES[0]
+    n, p = c["n"], c["p"]
+    lam = 0x5363AD4CC05C30E0A5261C028812645A122E22EA20816678DF02967C1B23BD72
+    beta = 0x7AE96A2B657C07106E64479EAC3434E99CF0497512F58995C1396C28719501EE
+    assert pow(lam, 3, n) == 1 and pow(beta, 3, p) == 1 and lam != 1 and beta != 1
+    G = c["G"][0].to_bytes(32, "big") + c["G"][1].to_bytes(32, "big")
+    rc, lamG = O.secp_mul(0, G, lam)
+    assert rc == 0 and lamG == (beta * c["G"][0] % p).to_bytes(32, "big") + c["G"][1].to_bytes(32, "big")
+    rnd = random.Random(7)
+    ks = [0, 1, 2, 3, n - 1, n - 2, lam, n - lam, (n - 1) // 2, 2**128, 2**128 - 1, 2**255]
+    ks += [rnd.randrange(n) for _ in range(3000)]
+    for k in ks:
+        k1, k2 = D.glv_split(k)
+        assert (k1 + k2 * lam - k) % n == 0, k
+        assert k1 & 1 and k2 & 1, k
+        assert abs(k1) < 2**129 and abs(k2) < 2**129, k
+
+
 def test_sha512_preimage_stream():
     rnd = random.Random(2)
     for L in list(range(0, 260)) + [1000, 4096]:

@@ -116,7 +116,7 @@ DEV ext eadd_niels(const ext& P, const fe& ypx, const fe& ymx, const fe& t2d, bo
 DEV bool sqrt_ratio_i(const fe& u, const fe& v, fe& r_out) {
   const fe v3 = mont_mul<Fe>(mont_sqr<Fe>(v), v);
   const fe v7 = mont_mul<Fe>(mont_sqr<Fe>(v3), v);
-  fe r = mont_mul<Fe>(mont_mul<Fe>(u, v3), fpow_const<Fe>(mont_mul<Fe>(u, v7), ED_P58_EXP));
+  fe r = mont_mul<Fe>(mont_mul<Fe>(u, v3), fpow_chain<Fe, CH_ED_P58>(mont_mul<Fe>(u, v7)));
   const fe check = mont_mul<Fe>(v, mont_sqr<Fe>(r));
   const fe i = fe_from_const(ED_SQRTM1);
   const fe neg_u = fneg<Fe>(u);

@@ -626,45 +626,61 @@ DEV u32 words_sub(const u32 a[8], const u32 b[8], u32 r[8]) {
   return br;
 }
 
-// ---- exponentiation by a fixed (kernel-constant) 256-bit exponent, fixed window of 3 bits ----
-// The window value is uniform across the wavefront, so selecting the table entry is a scalar
-// branch, not divergence.
-template <class F>
-DEV fe fpow_const(const fe& x, const u32 (&e)[8]) {
-  fe t[7];  // t[i] = x^(i+1)
-  t[0] = x;
-  t[1] = mont_sqr<F>(x);
-#pragma unroll
-  for (int i = 2; i < 7; i++) t[i] = mont_mul<F>(t[i - 1], x);
-  fe r = fe_from_const(F::ONE);
-  // 256 bits = 85 windows of 3 + 1 leading bit (bit 255)
-  {
-    const u32 top = (e[7] >> 31) & 1u;
-    if (top) r = x;
+// ---- exponentiation by a fixed exponent along an addition chain ----
+// CH (generated into consts.hpp) lists steps t[DST] = t[SRC]^(2^NSQ) * t[MUL] (MUL < 0: no
+// multiplication) with t[0] = x on entry. The steps are expanded at compile time, so every slot
+// index is a constant and the slots stay in registers; only the squaring runs loop.
+template <class F, class CH, int S>
+DEV void chain_steps(fe* t) {
+  if constexpr (S < CH::N) {
+    fe v = t[CH::SRC[S]];
+#pragma unroll 1
+    for (int k = 0; k < CH::NSQ[S]; k++) v = mont_sqr<F>(v);
+    if constexpr (CH::MUL[S] >= 0) v = mont_mul<F>(v, t[CH::MUL[S]]);
+    t[CH::DST[S]] = v;
+    chain_steps<F, CH, S + 1>(t);
   }
-  for (int w = 84; w >= 0; w--) {
-    r = mont_sqr<F>(r);
-    r = mont_sqr<F>(r);
-    r = mont_sqr<F>(r);
-    const int bit = 3 * w;
-    const int wi = bit >> 5, sh = bit & 31;
-    u32 d = e[wi] >> sh;
-    if (sh > 29 && wi + 1 < 8) d |= e[wi + 1] << (32 - sh);
-    d &= 7u;
-    if (d) {
-      fe m = t[0];
-#pragma unroll
-      for (int i = 1; i < 7; i++)
-        if (d == (u32)(i + 1)) m = t[i];
-      r = mont_mul<F>(r, m);
-    }
-  }
-  return r;
 }
 
+template <class F, class CH>
+DEV fe fpow_chain(const fe& x) {
+  fe t[CH::SLOTS];
+  t[0] = x;
+  chain_steps<F, CH, 0>(t);
+  return t[CH::RESULT];
+}
+
+// x^(M-2) (Fermat inverse) for each modulus
 template <class F>
-DEV fe finv(const fe& x) {  // x != 0; Fermat
-  return fpow_const<F>(x, F::EXP_INV);
+struct InvChain;
+template <>
+struct InvChain<ModPk1> {
+  typedef CH_K1_INV type;
+};
+template <>
+struct InvChain<ModNk1> {
+  typedef CH_NK1_INV type;
+};
+template <>
+struct InvChain<ModPr1> {
+  typedef CH_R1_INV type;
+};
+template <>
+struct InvChain<ModNr1> {
+  typedef CH_NR1_INV type;
+};
+template <>
+struct InvChain<ModPed> {
+  typedef CH_ED_INV type;
+};
+template <>
+struct InvChain<ModL> {
+  typedef CH_L_INV type;
+};
+
+template <class F>
+DEV fe finv(const fe& x) {  // x != 0
+  return fpow_chain<F, typename InvChain<F>::type>(x);
 }
 
 }  // namespace sigops

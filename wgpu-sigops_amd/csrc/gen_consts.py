@@ -125,9 +125,207 @@ def modulus_block(tag, m):
     out += f"  static constexpr uint32_t N0 = 0x{(-pow(m, -1, 2**LIMB_BITS)) % 2**LIMB_BITS:08x}u;\n"
     out += "  " + arr("ONE", limbs(mont(1, m)))
     out += "  " + arr("R2", limbs(2**(2 * RBITS) % m))
-    out += "  " + arr("EXP_INV", [(m - 2) >> (32 * i) & 0xFFFFFFFF for i in range(8)])
     out += "};\n"
     return out
+
+
+def w_mul(k, P, p, a):
+    R, Q = None, P
+    while k:
+        if k & 1:
+            R = w_add(R, Q, p, a)
+        Q = w_add(Q, Q, p, a)
+        k >>= 1
+    return R
+
+
+def glv_k1():
+    """GLV endomorphism of secp256k1: phi(x, y) = (beta x, y) = [lambda](x, y), with a reduced basis
+    (a1, b1), (a2, b2) of the lattice {(u, v): u + v lambda = 0 mod n} from the extended Euclidean
+    algorithm on (n, lambda) (Gallant-Lambert-Vanstone 2001; Hankerson-Menezes-Vanstone, Guide to
+    ECC, algorithms 3.74 and 3.73), and the fixed-point roundings g1 = round(2^384 b2 / n),
+    g2 = round(2^384 |b1| / n) used by the split."""
+    def cube_roots(m):
+        for g in range(2, 100):
+            r = pow(g, (m - 1) // 3, m)
+            if r != 1:
+                return sorted({r, r * r % m})
+    pairs = [(bt, lm) for bt in cube_roots(P_K1) for lm in cube_roots(N_K1)
+             if w_mul(lm, G_K1, P_K1, 0) == (bt * G_K1[0] % P_K1, G_K1[1])]
+    beta, lam = pairs[0]
+    r0, r1, t0, t1 = N_K1, lam, 0, 1
+    seq = [(r0, t0), (r1, t1)]
+    while r1:
+        q = r0 // r1
+        r0, r1 = r1, r0 - q * r1
+        t0, t1 = t1, t0 - q * t1
+        seq.append((r1, t1))
+    sq = _iroot(N_K1, 2)
+    li = max(i for i, (r, _) in enumerate(seq) if r >= sq)
+    a1, b1 = seq[li + 1][0], -seq[li + 1][1]
+    a2, b2 = min([(seq[li][0], -seq[li][1]), (seq[li + 2][0], -seq[li + 2][1])], key=lambda v: v[0] ** 2 + v[1] ** 2)
+    assert (a1 + b1 * lam) % N_K1 == 0 and (a2 + b2 * lam) % N_K1 == 0
+    assert a1 * b2 - a2 * b1 == N_K1 and b1 < 0 < a1 and a2 > 0 and b2 > 0
+    # parities used by the device split to make both halves odd
+    assert a1 & 1 and b1 & 1 and not a2 & 1 and b2 & 1
+    g1 = (2**384 * b2 + N_K1 // 2) // N_K1
+    g2 = (2**384 * -b1 + N_K1 // 2) // N_K1
+    return beta, lam, a1, b1, a2, b2, g1, g2
+
+
+def glv_split_ref(k, c):
+    """the device split (secp.hpp glv_split) restated on integers"""
+    beta, lam, a1, b1, a2, b2, g1, g2 = c
+    def signed(x):
+        x %= 2**256
+        return x - 2**256 if x >> 255 else x
+    c1 = (k * g1 + 2**383) >> 384
+    c2 = (k * g2 + 2**383) >> 384
+    k1 = signed(k - c1 * a1 - c2 * a2)
+    k2 = signed(-c1 * b1 - c2 * b2)
+    e1, e2 = k1 % 2 == 0, k2 % 2 == 0
+    if e1 and e2:
+        k1, k2 = k1 + a1, k2 + b1
+    elif e2:
+        k1, k2 = k1 + a2, k2 + b2
+    elif e1:
+        k1, k2 = k1 + a1 + a2, k2 + b1 + b2
+    return k1, k2
+
+
+def exp_chain(e, w=4):
+    """Addition chain for x -> x^e as steps t[dst] = t[src]^(2^nsq) * t[mul] (mul = -1: none), slot 0
+    holding x on entry. The exponent is scanned from the top: a run of >= w ones becomes one factor
+    x^(2^L - 1), built by the binary method on L (x^(2^2k - 1) = (x^(2^k - 1))^(2^k) x^(2^k - 1),
+    x^(2^(k+1) - 1) = (x^(2^k - 1))^2 x); anything else a sliding window with an odd digit < 2^w,
+    taken from the precomputed odd powers x, x^3, ..., x^(2^w - 1). Slots are reused once dead.
+    Returns (steps, nslots, result_slot, squarings, multiplications)."""
+    bits = bin(e)[2:]
+    nb = len(bits)
+    terms = []  # (kind, value, position of the lowest bit)
+    top_run = len(bits) - len(bits.lstrip("1"))
+    i = 0
+    while i < nb:
+        if bits[i] == "0":
+            i += 1
+            continue
+        j = i
+        while j < nb and bits[j] == "1":
+            j += 1
+        # a run costs no squarings of its own when it is no longer than the top run (its length is
+        # on the way to the top run's); a longer run lower down would, so it takes windows instead
+        if j - i >= w and j - i <= max(top_run, w):
+            terms.append(("run", j - i, nb - j))
+            i = j
+        else:
+            k = min(i + w, nb)
+            while bits[k - 1] == "0":
+                k -= 1
+            terms.append(("odd", int(bits[i:k], 2), nb - k))
+            i = k
+    # symbolic program over named values; names: ("run", L) or ("odd", d); exponent of each
+    prog = []  # (dst_name, src_name, nsq, mul_name)
+    have = {("run", 1): 1, ("odd", 1): 1}
+
+    def alias(name):
+        return ("x",) if name in (("run", 1), ("odd", 1)) else name
+
+    # one chain of run lengths through every needed length: x^(2^(m+c) - 1) = (x^(2^m - 1))^(2^c)
+    # x^(2^c - 1) costs c squarings, so reaching the longest run L costs L - 1 squarings in all
+    lengths = [1]
+    for L in sorted({v for k, v, _ in terms if k == "run"}):
+        while lengths[-1] < L:
+            m = lengths[-1]
+            c = max(x for x in lengths if m + x <= L)
+            prog.append((("run", m + c), ("run", m), c, ("run", c)))
+            have[("run", m + c)] = 2**(m + c) - 1
+            lengths.append(m + c)
+
+    odds = sorted({v for k, v, _ in terms if k == "odd"})
+    if odds and odds[-1] > 1:
+        prog.append((("sq",), ("odd", 1), 1, None))
+        for d in range(3, odds[-1] + 1, 2):
+            prog.append((("odd", d), ("odd", d - 2), 0, ("sq",)))
+    acc = ("acc",)
+    first = terms[0]
+    prev_pos = first[2]
+    src0 = ("run", first[1]) if first[0] == "run" else ("odd", first[1])
+    prog.append((acc, src0, 0, None))
+    for kind, v, pos in terms[1:]:
+        prog.append((acc, acc, prev_pos - pos, ("run", v) if kind == "run" else ("odd", v)))
+        prev_pos = pos
+    if prev_pos:
+        prog.append((acc, acc, prev_pos, None))
+    # drop pure copies (nsq 0, no multiply) by renaming
+    ren = {}
+    prog2 = []
+    for dst, src, nsq, mul in prog:
+        src, mul = ren.get(src, src), ren.get(mul, mul)
+        if nsq == 0 and mul is None:
+            ren[dst] = src
+            continue
+        ren.pop(dst, None)
+        prog2.append((dst, src, nsq, mul))
+    # check the exponent symbolically
+    val = {alias(("run", 1)): 1}
+    for dst, src, nsq, mul in prog2:
+        x = val[alias(src)] << nsq
+        if mul is not None:
+            x += val[alias(mul)]
+        val[alias(dst)] = x
+    res = alias(ren.get(acc, acc))
+    assert val[res] == e, "chain does not compute the exponent"
+    # slot allocation by liveness
+    last_use = {}
+    for idx, (dst, src, nsq, mul) in enumerate(prog2):
+        for nm in (src, mul):
+            if nm is not None:
+                last_use[alias(nm)] = idx
+    last_use[res] = len(prog2)
+    slot = {("x",): 0}
+    free, nslots, steps = [], 1, []
+    for idx, (dst, src, nsq, mul) in enumerate(prog2):
+        s_src = slot[alias(src)]
+        s_mul = slot[alias(mul)] if mul is not None else -1
+        for nm in list(slot):
+            if last_use.get(nm, -1) <= idx and nm != alias(dst):
+                free.append(slot.pop(nm))
+        d = alias(dst)
+        if d in slot:
+            s_dst = slot[d]
+        elif free:
+            s_dst = free.pop()
+        else:
+            s_dst = nslots
+            nslots += 1
+        slot[d] = s_dst
+        steps.append((s_dst, s_src, nsq, s_mul))
+    # replay on slots to verify the allocation
+    reg = {0: 1}
+    for s_dst, s_src, nsq, s_mul in steps:
+        x = reg[s_src] << nsq
+        if s_mul >= 0:
+            x += reg[s_mul]
+        reg[s_dst] = x
+    assert reg[slot[res]] == e, "slot allocation broke the chain"
+    nsqr = sum(st[2] for st in steps)
+    nmul = sum(1 for st in steps if st[3] >= 0)
+    return steps, nslots, slot[res], nsqr, nmul
+
+
+def chain_block(tag, e):
+    steps, nslots, res, nsqr, nmul = exp_chain(e)
+    n = len(steps)
+    out = f"// x^e, e = 0x{e:x}: {nsqr} squarings + {nmul} multiplications\n"
+    out += f"struct {tag} {{\n  static constexpr int N = {n}, SLOTS = {nslots}, RESULT = {res};\n"
+    for nm, col in (("DST", 0), ("SRC", 1), ("NSQ", 2), ("MUL", 3)):
+        out += f"  static constexpr int16_t {nm}[{n}] = {{{', '.join(str(st[col]) for st in steps)}}};\n"
+    out += "};\n"
+    return out
+
+
+def words8(x):
+    return [(x % 2**256) >> (32 * i) & 0xFFFFFFFF for i in range(8)]
 
 
 def _iroot(x, k):
@@ -166,13 +364,10 @@ def main():
         o.write(modulus_block(tag, m))
     o.write("// curve constants (Montgomery form)\n")
     o.write(arr("K1_B", limbs(mont(7, P_K1))))
-    o.write(arr("K1_SQRT_EXP", [((P_K1 + 1) // 4) >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
     o.write(arr("R1_B", limbs(mont(B_R1, P_R1))))
-    o.write(arr("R1_SQRT_EXP", [((P_R1 + 1) // 4) >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
     o.write(arr("ED_D", limbs(mont(D_ED, P_ED))))
     o.write(arr("ED_D2", limbs(mont(2 * D_ED % P_ED, P_ED))))
     o.write(arr("ED_SQRTM1", limbs(mont(SQRTM1, P_ED))))
-    o.write(arr("ED_P58_EXP", [((P_ED - 5) // 8) >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
     # n and l as plain 32-bit words (for range checks and scalar negation)
     o.write(arr("K1_N32", [N_K1 >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
     o.write(arr("R1_N32", [N_R1 >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
@@ -182,6 +377,31 @@ def main():
     #   h = hi*2^256 + lo ->  mont(hi, 2^(2*261+256-261) ...) : we need x*2^256 mod l = mont_mul(x, 2^(256+261) mod l)
     o.write(arr("L_C256", limbs(2**(256 + RBITS) % L_ED)))
     o.write(arr("L_C0", limbs(2**RBITS % L_ED)))
+    # GLV for secp256k1: beta (Montgomery form), split constants as 8 little-endian u32 words
+    # (negative ones in two's complement mod 2^256)
+    glv = glv_k1()
+    beta, lam, a1, b1, a2, b2, g1, g2 = glv
+    import random
+    rnd = random.Random(1)
+    for k in [0, 1, 2, N_K1 - 1, N_K1 - 2, lam, N_K1 - lam] + [rnd.randrange(N_K1) for _ in range(20000)]:
+        h1, h2 = glv_split_ref(k, glv)
+        assert (h1 + h2 * lam - k) % N_K1 == 0 and h1 & 1 and h2 & 1
+        assert abs(h1) < 2**129 and abs(h2) < 2**129
+    o.write(arr("K1_BETA", limbs(mont(beta, P_K1))))
+    o.write(arr("GLV_G1", words8(g1)))
+    o.write(arr("GLV_G2", words8(g2)))
+    o.write(arr("GLV_A1", words8(a1)))
+    o.write(arr("GLV_A2", words8(a2)))
+    o.write(arr("GLV_B1N", words8(-b1)))  # |b1|
+    o.write(arr("GLV_B2", words8(b2)))
+    # parity corrections (k1, k2) += v1, v2 or v1 + v2
+    o.write(arr("GLV_FIX_K1", words8(a1) + words8(a2) + words8(a1 + a2)))
+    o.write(arr("GLV_FIX_K2", words8(b1) + words8(b2) + words8(b1 + b2)))
+    # exponentiation chains: inverses (Fermat), square roots ((p+1)/4) and dalek's pow_p58
+    for tag, e in [("CH_K1_INV", P_K1 - 2), ("CH_K1_SQRT", (P_K1 + 1) // 4), ("CH_R1_INV", P_R1 - 2),
+                   ("CH_R1_SQRT", (P_R1 + 1) // 4), ("CH_ED_INV", P_ED - 2), ("CH_ED_P58", (P_ED - 5) // 8),
+                   ("CH_NK1_INV", N_K1 - 2), ("CH_NR1_INV", N_R1 - 2), ("CH_L_INV", L_ED - 2)]:
+        o.write(chain_block(tag, e))
     K, IV = sha512_constants()
     o.write(arr("SHA512_K", K, "uint64_t"))
     o.write(arr("SHA512_IV", IV, "uint64_t"))

@@ -34,17 +34,19 @@ struct CurveK1 {
   typedef ModPk1 Fp;
   typedef ModNk1 Fn;
   static constexpr bool A_IS_M3 = false;
+  static constexpr bool GLV = true;  // phi(x, y) = (beta x, y) = [lambda](x, y)
   DEV static fe b() { return fe_from_const(K1_B); }
   DEV static const u32 (&n32())[8] { return K1_N32; }
-  DEV static const u32 (&sqrt_exp())[8] { return K1_SQRT_EXP; }
+  typedef CH_K1_SQRT SqrtChain;  // x^((p+1)/4)
 };
 struct CurveR1 {
   typedef ModPr1 Fp;
   typedef ModNr1 Fn;
   static constexpr bool A_IS_M3 = true;
+  static constexpr bool GLV = false;
   DEV static fe b() { return fe_from_const(R1_B); }
   DEV static const u32 (&n32())[8] { return R1_N32; }
-  DEV static const u32 (&sqrt_exp())[8] { return R1_SQRT_EXP; }
+  typedef CH_R1_SQRT SqrtChain;
 };
 
 struct jac {
@@ -174,14 +176,124 @@ DEV jacf jadd_c(const jacf& P, const jacf& Q) {
   return jacf{jadd<C>(P.P, Q.P, exc), false};
 }
 
+// ---- integer words (little-endian u32) for the GLV split ----
+// r = a * b mod 2^(32 NR); every step a[i]*b[j] + r + carry < 2^64
+template <int NA, int NB, int NR>
+DEV void wmul(const u32* a, const u32* b, u32* r) {
+#pragma unroll
+  for (int k = 0; k < NR; k++) r[k] = 0;
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    u64 c = 0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if (i + j < NR) {
+        const u64 t = (u64)a[i] * b[j] + r[i + j] + c;
+        r[i + j] = (u32)t;
+        c = t >> 32;
+      }
+    }
+    if (i + NB < NR) r[i + NB] = (u32)c;
+  }
+}
+
+DEV void wadd8(u32 a[8], const u32 b[8]) {  // a += b mod 2^256
+  u64 c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u64 t = (u64)a[i] + b[i] + c;
+    a[i] = (u32)t;
+    c = t >> 32;
+  }
+}
+
+DEV void wsub8(u32 a[8], const u32 b[8]) {  // a -= b mod 2^256
+  u32 t[8];
+  words_sub(a, b, t);
+#pragma unroll
+  for (int i = 0; i < 8; i++) a[i] = t[i];
+}
+
+// GLV split for secp256k1 (constants and their derivation: gen_consts.py glv_k1). For k < n:
+//   c1 = round(k g1 / 2^384) = round(k b2 / n), c2 = round(k |b1| / n)  (both < 2^128)
+//   (k1, k2) = (k, 0) - c1 (a1, b1) - c2 (a2, b2), so k1 + k2 lambda = k (mod n), |k_i| < 2^128
+//   then one lattice vector v1 = (a1, b1) (flips both parities), v2 = (a2, b2) (flips k2) or
+//   v1 + v2 (flips k1) makes both halves odd: |k_i| < 2^129.
+// Output: magnitudes h1, h2 (5 words) and signs, k = (-1)^n1 h1 + (-1)^n2 h2 lambda (mod n).
+DEV void glv_split(const u32 k[8], u32 h1[5], u32 h2[5], bool& n1, bool& n2) {
+  u32 p[16], c1[4], c2[4];
+#pragma unroll
+  for (int pass = 0; pass < 2; pass++) {
+    wmul<8, 8, 16>(k, pass == 0 ? GLV_G1 : GLV_G2, p);
+    u64 cy = p[11] >> 31;  // + 2^383: round to nearest
+    u32* c = pass == 0 ? c1 : c2;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      cy += p[12 + i];
+      c[i] = (u32)cy;
+      cy >>= 32;
+    }
+  }
+  u32 k1[8], k2[8], t[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) k1[i] = k[i];
+  wmul<4, 8, 8>(c1, GLV_A1, t);
+  wsub8(k1, t);
+  wmul<4, 8, 8>(c2, GLV_A2, t);
+  wsub8(k1, t);
+  wmul<4, 8, 8>(c1, GLV_B1N, k2);
+  wmul<4, 8, 8>(c2, GLV_B2, t);
+  wsub8(k2, t);
+  // parity correction: both even -> v1, k2 even -> v2, k1 even -> v1 + v2
+  const bool e1 = (k1[0] & 1u) == 0, e2 = (k2[0] & 1u) == 0;
+  const int sel = e1 ? (e2 ? 0 : 2) : 1;
+  const u32 any = lane_mask(e1 || e2);
+  u32 f1[8], f2[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u32 a = GLV_FIX_K1[i], b = GLV_FIX_K2[i];
+    if (sel == 1) { a = GLV_FIX_K1[8 + i]; b = GLV_FIX_K2[8 + i]; }
+    if (sel == 2) { a = GLV_FIX_K1[16 + i]; b = GLV_FIX_K2[16 + i]; }
+    f1[i] = a & any;
+    f2[i] = b & any;
+  }
+  wadd8(k1, f1);
+  wadd8(k2, f2);
+  n1 = (k1[7] >> 31) != 0;
+  n2 = (k2[7] >> 31) != 0;
+  u32 z[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m1[8], m2[8];
+  words_sub(z, k1, m1);
+  words_sub(z, k2, m2);
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    h1[i] = n1 ? m1[i] : k1[i];
+    h2[i] = n2 ? m2[i] : k2[i];
+  }
+}
+
+// digit carrier of an odd h < 2^B over 5 words: C = (h >> 1) | 2^(B-1); the w-bit digits c_i of C
+// give h = sum (2 c_i - (2^w - 1)) 2^(w i), every digit odd and the top one positive
+template <int B>
+DEV void glv_carrier(const u32 h[5], u32 c[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) c[i] = (h[i] >> 1) | (i < 4 ? h[i + 1] << 31 : 0u);
+  c[(B - 1) >> 5] |= 1u << ((B - 1) & 31);
+}
+
+constexpr int GLV_BR = 132;  // R streams: 33 windows of 4 bits (halves < 2^129)
+constexpr int GLV_BG = 136;  // G streams: 17 windows of 8 bits
+
 // ---- per-item recovery ----
 // sig_raw: the 64 signature bytes as 16 little-endian u32 loads; dig_raw: 8 words of the digest.
 // Prologue results shared by the fast and careful ladders.
 struct secp_pro {
   u32 status;
-  fe rx, ry;       // R affine, Montgomery form of p (ry already negated when u2 was negated)
-  u32 c1[8], c2[8];  // digit carriers C = (k' >> 1) | 2^255 for u1 (G) and u2 (R)
-  bool g_neg;      // G digits must be negated (u1 replaced by n - u1)
+  fe rx, ry;         // R affine, Montgomery form of p; ry = -y when ry_neg
+  bool ry_neg;
+  u32 c1[8], c2[8];  // digit carriers C = (k' >> 1) | 2^255 for u1 (G) and u2 (R) (no GLV)
+  bool g_neg;        // G digits must be negated (u1 replaced by n - u1) (no GLV)
+  u32 cg1[5], cg2[5], cr1[5], cr2[5];  // GLV: carriers of the halves of u1 (G) and u2 (R)
+  bool sg1, sg2, sr1, sr2;             // GLV: signs of those halves
   u32 u1[8], u2[8];  // plain u1, u2 (for the careful path)
 };
 
@@ -212,7 +324,7 @@ DEV void secp_prologue(const u32 sig_raw[16], const u32 dig_raw[8], secp_pro& o)
   fe rhs = mont_mul<Fp>(mont_sqr<Fp>(x), x);
   if (C::A_IS_M3) rhs = fsub<Fp>(rhs, fadd<Fp>(fdbl<Fp>(x), x));
   rhs = fadd<Fp>(rhs, C::b());
-  fe y = fpow_const<Fp>(rhs, C::sqrt_exp());
+  fe y = fpow_chain<Fp, typename C::SqrtChain>(rhs);
   if (o.status == ST_OK && !feq<Fp>(mont_sqr<Fp>(y), rhs)) o.status = ST_NO_POINT;
   const fe yc = from_mont<Fp>(y);
   if ((bool)(yc.v[0] & 1u) != y_odd) y = fneg<Fp>(y);
@@ -223,6 +335,20 @@ DEV void secp_prologue(const u32 sig_raw[16], const u32 dig_raw[8], secp_pro& o)
   const fe u2m = mont_mul<Fn>(to_mont<Fn>(fe_from_words(s)), rinv);
   fe_to_words(from_mont<Fn>(u1m), o.u1);
   fe_to_words(from_mont<Fn>(u2m), o.u2);
+  o.rx = x;
+  if constexpr (C::GLV) {
+    u32 h1[5], h2[5];
+    glv_split(o.u1, h1, h2, o.sg1, o.sg2);
+    glv_carrier<GLV_BG>(h1, o.cg1);
+    glv_carrier<GLV_BG>(h2, o.cg2);
+    glv_split(o.u2, h1, h2, o.sr1, o.sr2);
+    glv_carrier<GLV_BR>(h1, o.cr1);
+    glv_carrier<GLV_BR>(h2, o.cr2);
+    o.ry = y;
+    o.ry_neg = false;
+    o.g_neg = false;
+    return;
+  }
   // odd representatives
   o.g_neg = (o.u1[0] & 1u) == 0;
   const bool r_neg = (o.u2[0] & 1u) == 0;
@@ -239,12 +365,58 @@ DEV void secp_prologue(const u32 sig_raw[16], const u32 dig_raw[8], secp_pro& o)
     o.c1[i] = (k1[i] >> 1) | (i < 7 ? k1[i + 1] << 31 : 0x80000000u);
     o.c2[i] = (k2[i] >> 1) | (i < 7 ? k2[i + 1] << 31 : 0x80000000u);
   }
-  o.rx = x;
   o.ry = r_neg ? fneg<Fp>(y) : y;
+  o.ry_neg = r_neg;
 }
 
 DEV u32 nibble(const u32 c[8], int i) { return (c[i >> 3] >> ((i & 7) * 4)) & 15u; }
 DEV u32 byte8(const u32 c[8], int j) { return (c[j >> 2] >> ((j & 3) * 8)) & 255u; }
+
+// GLV ladder (secp256k1): Q = h1 G + h2 phi(G) + h3 R + h4 phi(R) on one doubling chain of 132
+// bits, with phi(x, y) = (beta x, y) applied to table entries as they are loaded. R streams use 33
+// windows of 4 bits over the per-lane table of odd multiples of R; G streams 17 windows of 8 bits
+// over the table of odd multiples of G, added at every second R window.
+template <class C, class Tab, class GTab>
+DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
+  typedef typename C::Fp F;
+  const fe beta = fe_from_const(K1_BETA);
+  // top windows: every stream's top digit is positive; only the stream signs apply
+  jac acc = tab.load(nibble(o.cr1, 32) - 8);
+  acc.Y = fneg_if<F>(o.sr1, acc.Y);
+  {
+    jac E = tab.load(nibble(o.cr2, 32) - 8);
+    E.X = mont_mul<F>(E.X, beta);
+    E.Y = fneg_if<F>(o.sr2, E.Y);
+    acc = jadd<C>(acc, E, exc);
+  }
+#pragma unroll 1
+  for (int i = 32; i >= 0; i--) {
+    if (i < 32) {
+      // issue both table loads before the doublings, which hide their latency
+      const u32 c1 = nibble(o.cr1, i), c2 = nibble(o.cr2, i);
+      jac E1 = tab.load((c1 < 8 ? ~c1 : c1) & 7u);
+      jac E2 = tab.load((c2 < 8 ? ~c2 : c2) & 7u);
+#pragma unroll 1
+      for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
+      E1.Y = fneg_if<F>((c1 < 8) != o.sr1, E1.Y);
+      acc = jadd<C>(acc, E1, exc);
+      E2.X = mont_mul<F>(E2.X, beta);
+      E2.Y = fneg_if<F>((c2 < 8) != o.sr2, E2.Y);
+      acc = jadd<C>(acc, E2, exc);
+    }
+    if ((i & 1) == 0) {
+      fe gx, gy;
+      const u32 b1 = byte8(o.cg1, i >> 1), b2 = byte8(o.cg2, i >> 1);
+      gt.load(((b1 < 128) ? ~b1 : b1) & 127u, gx, gy);
+      gy = fneg_if<F>((b1 < 128) != o.sg1, gy);
+      acc = jmadd<C>(acc, gx, gy, exc);
+      gt.load(((b2 < 128) ? ~b2 : b2) & 127u, gx, gy);
+      gy = fneg_if<F>((b2 < 128) != o.sg2, gy);
+      acc = jmadd<C>(acc, mont_mul<F>(gx, beta), gy, exc);
+    }
+  }
+  return acc;
+}
 
 // R-table stored per lane; Tab provides store(e, jac) / load(e) -> jac.
 // GTab provides load(idx, fe& x, fe& y) for the odd multiples (2 idx + 1) G.
@@ -263,6 +435,7 @@ DEV jac secp_ladder(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
     T = jadd<C>(T, P2, exc);
     tab.store(e, T);
   }
+  if constexpr (C::GLV) return secp_ladder_glv<C>(o, tab, gt, exc);
   // top R digit is always positive (C's bit 255 is set)
   jac acc = tab.load(nibble(o.c2, 63) - 8);
 #pragma unroll 1
@@ -295,10 +468,8 @@ DEV jacf secp_ladder_careful(const secp_pro& o, const GTab& gt) {
   fe gx, gy;
   gt.load(0, gx, gy);
   G.P.X = gx; G.P.Y = gy; G.P.Z = fe_from_const(F::ONE); G.inf = false;
-  // o.ry may have been negated together with u2 -> use the original sign consistently:
-  // o.u2 is the true u2 and o.ry = r_neg ? -y : y; undo the negation here.
-  const bool r_neg = (o.u2[0] & 1u) == 0;
-  R.P.X = o.rx; R.P.Y = r_neg ? fneg<F>(o.ry) : o.ry; R.P.Z = fe_from_const(F::ONE); R.inf = false;
+  // o.u2 is the true u2; undo the negation of o.ry that came with the fast ladder's n - u2
+  R.P.X = o.rx; R.P.Y = o.ry_neg ? fneg<F>(o.ry) : o.ry; R.P.Z = fe_from_const(F::ONE); R.inf = false;
   GR = jadd_c<C>(G, R);
   acc.inf = true;
   acc.P = G.P;
