@@ -159,6 +159,7 @@ void dc_field(int op, int mod, const unsigned char* a, const unsigned char* b, u
       case 1: rm = mont_sqr<F>(xm); break;                 \
       case 2: rm = finv<F>(xm); break;                     \
       case 3: rm = fadd<F>(xm, ym); break;                 \
+      case 5: rm = finv_fermat<F>(xm); break;              \
       default: rm = fsub<F>(xm, ym); break;                \
     }                                                      \
     r = from_mont<F>(rm);                                  \

@@ -28,6 +28,20 @@ def test_field_ops(mod):
             assert D.field(2, mod, a) == pow(a, -1, m)
 
 
+@pytest.mark.parametrize("mod", range(6))
+def test_inversion_divsteps_edge_values(mod):
+    """the divstep inversion (op 2) and the Fermat chain (op 5) against pow(x, -1, m) on values that
+    stress the recurrence: tiny, near m, powers of two, halves, and random"""
+    m = MODULI[mod]
+    rnd = random.Random(50 + mod)
+    xs = [1, 2, 3, m - 1, m - 2, (m + 1) // 2, (m - 1) // 2, 2**255 % m, 2**128, 2**64 + 1, 3**100 % m]
+    xs += [pow(2, k, m) for k in range(0, 256, 17)] + [rnd.randrange(1, m) for _ in range(100)]
+    for x in xs:
+        want = pow(x, -1, m)
+        assert D.field(2, mod, x) == want, (mod, x)
+        assert D.field(5, mod, x) == want, (mod, x)
+
+
 def _val(limbs):
     return sum(v << (29 * i) for i, v in enumerate(limbs))
 
@@ -47,6 +61,10 @@ def test_montgomery_raw_limbs(mod):
     rnd = random.Random(100 + mod)
     cases = [(2 * m - 1, 2 * m - 1), (0, 2 * m - 1), (1, 1), (m, m), (m - 1, 2 * m - 1)]
     cases += [(rnd.randrange(2 * m), rnd.randrange(2 * m)) for _ in range(200)]
+    # sparse operands (rows of zero limbs leave the upper columns to the reduction terms and the
+    # carries alone): products with 1 and small powers of two, as from_mont and doublings make
+    cases += [(x, y) for x in (2 * m - 1, m + 1, (m + 1) // 2, 2**255 % (2 * m), rnd.randrange(2 * m))
+              for y in (1, 2, 2**28, 2**29, 2**58 + 1)]
     # limbs all at the maximum: value 2^261 - 1 (outside [0, 2M), checked for congruence only)
     top = [2**29 - 1] * 9
     raw = [(_limbs(a), _limbs(b), True) for a, b in cases] + [(top, top, False), (top, _limbs(2 * m - 1), False)]

@@ -163,23 +163,22 @@ DEV fe fe_zero() {
 //     and N0 = -SP_COEF[0]^-1 (mod 2^29). A coefficient -1 at column i costs nothing: m is then
 //     col_i mod 2^29 and subtracting it only clears the bits the carry shift drops.
 //     Negative terms (-977 m, -8 m for secp256k1 p; -19 m for 2^255-19; magnitude < 2^40) are
-//     absorbed by column offsets: columns 0 .. NEG_TOP, the last one a negative term reaches,
-//     start at COL_E (column 0) or COL_E - COL_E/2^29 and column NEG_TOP + 1 at -COL_E/2^29, so
-//     with the +COL_E/2^29 that each incoming carry brings, every column that receives a negative
-//     term is held as its true value + COL_E >= 0 while it is shifted, and the offsets cancel.
-//     Columns stay unsigned and carries are plain shifts.
+//     absorbed by column offsets: column 0 starts at COL_E, columns 1..15 at COL_E - COL_E/2^29 and
+//     the top column at -COL_E/2^29, so with the +COL_E/2^29 that each incoming carry brings,
+//     every column below the top is held as its true value + COL_E >= 0 while it is shifted (a
+//     negative term or a negative carry from below never exceeds COL_E), and the offsets cancel.
+//     The top column needs none: the result is non-negative, so its final value is too. Columns
+//     stay unsigned and carries are plain shifts. P-256 p has no negative term and no offsets.
 // Column bound: 9 * A * B + (positive reduction terms + offset + carry) < 2^64 for operand limbs
 // <= A, B; the second part is below 2^54 for secp256k1 p and 2^255-19 and below 2^58.1 for P-256.
 // Output: (a*b + m*M) / 2^261 < a*b/2^261 + M, so it is < 2M whenever a*b < M * 2^261.
 constexpr u64 COL_E = 1ull << 40;
 
 template <class F>
-DEV constexpr int neg_top() {  // last column that a negative reduction term reaches (-1: none)
-  int top = -1;
+DEV constexpr bool has_neg_terms() {  // a reduction term other than a column-0 coefficient -1 is negative
   for (int t = 0; t < F::SP_N; t++)
-    if (F::SP_COEF[t] < 0 && !(t == 0 && F::SP_COEF[t] == -1) && NL - 1 + F::SP_OFF[t] > top)
-      top = NL - 1 + F::SP_OFF[t];
-  return top;
+    if (F::SP_COEF[t] < 0 && !(t == 0 && F::SP_COEF[t] == -1)) return true;
+  return false;
 }
 
 template <class F>
@@ -187,9 +186,8 @@ DEV constexpr u64 col_init(int k) {
   if constexpr (!F::SPARSE) {
     return 0;
   } else {
-    constexpr int top = neg_top<F>();
-    if (k > top + 1 || top < 0) return 0;
-    if (k == top + 1) return (u64)0 - (COL_E >> SIGOPS_LIMB_BITS);
+    if (!has_neg_terms<F>()) return 0;
+    if (k == 2 * NL - 2) return (u64)0 - (COL_E >> SIGOPS_LIMB_BITS);
     return k == 0 ? COL_E : COL_E - (COL_E >> SIGOPS_LIMB_BITS);
   }
 }
@@ -678,8 +676,9 @@ struct InvChain<ModL> {
   typedef CH_L_INV type;
 };
 
+// Fermat inverse x^(M-2); the kernels invert with divsteps (inv.hpp), this form checks it
 template <class F>
-DEV fe finv(const fe& x) {  // x != 0
+DEV fe finv_fermat(const fe& x) {  // x != 0
   return fpow_chain<F, typename InvChain<F>::type>(x);
 }
 

@@ -38,12 +38,18 @@ def limbs(x):
     return [(x >> (LIMB_BITS * i)) & MASK for i in range(NLIMBS)]
 
 
+def limbs30(x):
+    assert 0 <= x < 2**270
+    return [(x >> (30 * i)) & (2**30 - 1) for i in range(9)]
+
+
 def mont(x, m):
     return x * 2**RBITS % m
 
 
 def arr(name, vals, ctype="uint32_t"):
-    body = ", ".join(("0x%016xull" if ctype == "uint64_t" else "0x%08xu") % v for v in vals)
+    fmt = {"uint64_t": "0x%016xull", "int32_t": "%d"}.get(ctype, "0x%08xu")
+    body = ", ".join(fmt % v for v in vals)
     return f"static constexpr {ctype} {name}[{len(vals)}] = {{{body}}};\n"
 
 
@@ -125,6 +131,12 @@ def modulus_block(tag, m):
     out += f"  static constexpr uint32_t N0 = 0x{(-pow(m, -1, 2**LIMB_BITS)) % 2**LIMB_BITS:08x}u;\n"
     out += "  " + arr("ONE", limbs(mont(1, m)))
     out += "  " + arr("R2", limbs(2**(2 * RBITS) % m))
+    # binary-GCD inversion (inv.hpp): m and the seeds R mod m, R^2 mod m in signed 30-bit limbs
+    # (all limbs below 2^30 here), and m^-1 mod 2^30
+    out += "  " + arr("M30", limbs30(m), "int32_t")
+    out += "  " + arr("RMOD30", limbs30(2**RBITS % m), "int32_t")
+    out += "  " + arr("R2MOD30", limbs30(2**(2 * RBITS) % m), "int32_t")
+    out += f"  static constexpr uint32_t MINV30 = 0x{pow(m, -1, 2**30):08x}u;\n"
     out += "};\n"
     return out
 

@@ -17,7 +17,7 @@
 //     These are incomplete: an addition of P and +-P is detected (H == 0) and the lane is marked
 //     for the careful path, which redoes the item with complete, branching formulas.
 #pragma once
-#include "field.hpp"
+#include "inv.hpp"
 
 namespace sigops {
 
@@ -392,12 +392,13 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
 #pragma unroll 1
   for (int i = 32; i >= 0; i--) {
     if (i < 32) {
-      // issue both table loads before the doublings, which hide their latency
+      // the first table load is issued before the doublings and the second before the first
+      // addition; each is hidden behind that work without keeping both entries live
       const u32 c1 = nibble(o.cr1, i), c2 = nibble(o.cr2, i);
       jac E1 = tab.load((c1 < 8 ? ~c1 : c1) & 7u);
-      jac E2 = tab.load((c2 < 8 ? ~c2 : c2) & 7u);
 #pragma unroll 1
       for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
+      jac E2 = tab.load((c2 < 8 ? ~c2 : c2) & 7u);
       E1.Y = fneg_if<F>((c1 < 8) != o.sr1, E1.Y);
       acc = jadd<C>(acc, E1, exc);
       E2.X = mont_mul<F>(E2.X, beta);
