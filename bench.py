@@ -50,16 +50,21 @@ def mixed_mac_per_sig(schemes, lens):
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hbm_traffic_and_valu_k_batch_run_mixed_2p20.md")
 
 
-def read_traffic():
-    """HBM bytes per launch of the persistent kernel on this workload, from the committed summary of
-    the FETCH_SIZE / WRITE_SIZE counter passes (each scaled by a copy of known size)."""
+def read_profile():
+    """From the committed summary of the counter passes over this workload: HBM bytes per launch of
+    the persistent kernel (FETCH_SIZE / WRITE_SIZE, each scaled by a copy of known size) and the
+    fraction of its active cycles that issue VALU instructions (SQ_INSTS_VALU x 4 / GRBM_GUI_ACTIVE)."""
+    out = {"traffic": None, "valu_issue_frac": None, "source": None}
     try:
         for line in open(TRAFFIC_PROFILE):
             if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1]), os.path.relpath(TRAFFIC_PROFILE, REPO)
+                out["traffic"] = int(line.split(":")[1])
+            if line.startswith("valu_issue_frac:"):
+                out["valu_issue_frac"] = float(line.split(":")[1])
+        out["source"] = os.path.relpath(TRAFFIC_PROFILE, REPO)
     except OSError:
         pass
-    return None, None
+    return out
 
 
 def main():
@@ -103,7 +108,10 @@ def main():
         del src, dst
     mad_per_s, fma_per_s = sigops.probe_valu(0)
     log(f"probe: v_mad_u64_u32 {mad_per_s / 1e12:.2f} T/s, v_fma_f64 {fma_per_s / 1e12:.2f} T/s")
+    t_setup = time.perf_counter()
     ctx = sigops.Context(blocks_per_cu=args.blocks_per_cu, tile_rounds=args.tile_rounds, max_batch=max(n, 1 << 22))
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t_setup) * 1e3  # context, table upload, buffers (one time)
     t0 = time.time()
     mx = W.mixed(ctx, n, seed=args.seed)
     torch.cuda.synchronize()
@@ -272,10 +280,13 @@ def main():
     kern_ms = stage_ms["verify_kernel"]
     achieved = n * mac_sig / (kern_ms / 1e3) / 1e12
     peak = mad_per_s / 1e12
-    traffic, traffic_src = read_traffic()
+    prof = read_profile()
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
                 "unit": "TMAC/s (u32 x u32 -> u64 multiply-accumulate)", "frac": round(achieved / peak, 4),
-                "traffic": traffic, "traffic_unit": "B per launch", "traffic_source": traffic_src,
+                "work": "canonical textbook schedule (fixed-window Fermat, 252 doublings, dense Montgomery); "
+                        "the kernel executes fewer MACs than that, so frac can exceed 1",
+                "valu_issue_frac": prof["valu_issue_frac"],
+                "traffic": prof["traffic"], "traffic_unit": "B per launch", "traffic_source": prof["source"],
                 "kernel": "k_batch_run", "kernel_ms": round(kern_ms, 3), "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
                 "mac_per_sig": round(mac_sig), "peak_source": "sigops_probe_valu (v_mad_u64_u32)"}
 
@@ -301,11 +312,25 @@ def main():
         v = O.verify_batch(h_sigs[ss], h_aux[ss], sub_msgs, sub_off, nthreads=threads)
         exact &= bool(np.array_equal(v.astype(bool), st_h[ss] == 0))
         cpu_s = time.perf_counter() - t0
+        # serial, one core, on the first 2048 items of the sample (the reference's CPU method)
+        s1 = sample[:2048]
+        t1 = time.perf_counter()
+        for s in (0, 1):
+            ss = s1[sch[s1] == s]
+            O.recover_batch(s, h_sigs[ss], h_aux[ss], nthreads=1)
+        ss = s1[sch[s1] == 2]
+        lens1 = h_off[ss + 1] - h_off[ss]
+        off1 = np.concatenate([[0], np.cumsum(lens1)]).astype(np.uint64)
+        msgs1 = np.concatenate([h_msgs[h_off[i]:h_off[i + 1]] for i in ss]) if len(ss) else np.zeros(0, np.uint8)
+        O.verify_batch(h_sigs[ss], h_aux[ss], msgs1, off1, nthreads=1)
+        serial = len(s1) / (time.perf_counter() - t1)
         cpu = {"value": round(len(sample) / cpu_s, 1), "unit": "sigs/s", "cores": threads, "kind": "port",
                "sample": f"{len(sample)} items drawn uniformly from the same 2^20 mixed batch, C oracle "
                          f"(oracle/), {threads} threads; outputs also compared with the GPU's",
-               "bit_exact_vs_gpu": exact}
-        log(f"cpu oracle: {cpu['value']:.0f} sigs/s on {threads} threads; GPU sample bit-exact: {exact}")
+               "bit_exact_vs_gpu": exact, "serial_1_core": round(serial, 1),
+               "host_cpus": os.cpu_count()}
+        log(f"cpu oracle: {cpu['value']:.0f} sigs/s on {threads} threads, {serial:.0f} sigs/s on 1 core; "
+            f"GPU sample bit-exact: {exact}")
 
     line = {
         "metric": "sigs/sec at 2^20 (batched vs plain, bit-exact) + p50/p99 latency served",
@@ -332,6 +357,7 @@ def main():
         "roundtrip_all_items_ok": roundtrip_ok,
         "served": served,
         "probe": {"mad_per_s": mad_per_s, "fma64_per_s": fma_per_s},
+        "setup_ms": round(setup_ms, 1),
     }
     print(json.dumps(line), flush=True)
     ctx.close()
