@@ -108,6 +108,7 @@ def main():
         del src, dst
     mad_per_s, fma_per_s = sigops.probe_valu(0)
     log(f"probe: v_mad_u64_u32 {mad_per_s / 1e12:.2f} T/s, v_fma_f64 {fma_per_s / 1e12:.2f} T/s")
+    tuned_shape = not args.blocks_per_cu and not args.tile_rounds and sigops._tuned(0) is not None
     t_setup = time.perf_counter()
     ctx = sigops.Context(blocks_per_cu=args.blocks_per_cu, tile_rounds=args.tile_rounds, max_batch=max(n, 1 << 22))
     torch.cuda.synchronize()
@@ -348,7 +349,9 @@ def main():
         "config": {"workload": "mixed 2^20: 50% secp256k1 recover, 30% secp256r1 recover, 20% ed25519 verify "
                                "(messages log-uniform 0..4096 B), interleaved, 5% corrupted; one graph-captured "
                                "persistent launch per step",
-                   "n": n, "seed": args.seed},
+                   "n": n, "seed": args.seed, "blocks_per_cu": ctx.blocks_per_cu or 2,
+                   "tile_rounds": ctx.tile_rounds or 1,
+                   "launch_shape_from": "autotune (tuned.json)" if tuned_shape else "arguments or library defaults"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": plain,

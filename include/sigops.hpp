@@ -71,6 +71,7 @@ std::vector<std::vector<uint8_t>> recover(int curve, const std::vector<Sig>& sig
                                           const std::vector<Message>& messages, Context& ctx) {
   if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
   const size_t n = signatures.size();
+  if (n > (size_t(1) << 30)) throw std::invalid_argument("at most 2^30 signatures");  // init's assert, :21-22
   if (n == 0) return {};
   std::vector<uint8_t> sigs(n * 64), digs(n * 32), out(n * 64);
   for (size_t i = 0; i < n; i++) {
@@ -122,6 +123,11 @@ inline std::vector<std::vector<uint8_t>> ecrecover_single_shader(const std::vect
   detail::check_log_limb_size(log_limb_size);
   return detail::recover(1, signatures, messages, ctx);
 }
+inline std::vector<std::vector<uint8_t>> ecrecover_single(const std::vector<Bytes64>& signatures,
+                                                          const std::vector<Message>& messages, uint32_t log_limb_size,
+                                                          Context& ctx = default_context()) {
+  return ecrecover_single_shader(signatures, messages, log_limb_size, ctx);
+}
 }  // namespace secp256r1_ecdsa
 
 namespace ed25519_eddsa {
@@ -161,6 +167,31 @@ inline std::vector<bool> ecverify_single(const std::vector<Ed25519Signature>& si
                                          Context& ctx = default_context()) {
   detail::check_log_limb_size(log_limb_size);
   return detail_ed::verify(signatures, messages, verifying_keys, ctx);
+}
+// variable-length extension: message i is messages[i] (any length), hashed as given; the
+// reference's Message is fixed at 32 bytes (src/ed25519_eddsa.rs:39)
+inline std::vector<bool> ecverify_bytes(const std::vector<Ed25519Signature>& signatures,
+                                        const std::vector<std::vector<uint8_t>>& messages,
+                                        const std::vector<VerifyingKey>& verifying_keys,
+                                        Context& ctx = default_context()) {
+  if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
+  const size_t n = signatures.size();
+  if (n == 0) return {};
+  if (verifying_keys.size() < n) throw std::out_of_range("fewer verifying keys than signatures");
+  std::vector<uint8_t> sigs(n * 64), pks(n * 32), valid(n), msgs;
+  std::vector<uint64_t> off(n + 1);
+  for (size_t i = 0; i < n; i++) {
+    memcpy(&sigs[i * 64], signatures[i].data(), 64);
+    memcpy(&pks[i * 32], verifying_keys[i].data(), 32);
+    off[i] = msgs.size();
+    msgs.insert(msgs.end(), messages[i].begin(), messages[i].end());
+  }
+  off[n] = msgs.size();
+  ctx.check(sigops_ed25519_verify(ctx.get(), sigs.data(), pks.data(), msgs.empty() ? nullptr : msgs.data(), off.data(),
+                                  n, valid.data(), nullptr));
+  std::vector<bool> res(n);
+  for (size_t i = 0; i < n; i++) res[i] = valid[i] == 1;
+  return res;
 }
 }  // namespace ed25519_eddsa
 

@@ -54,6 +54,11 @@ int main(int argc, char** argv) {
   auto v = sigops::ed25519_eddsa::ecverify(eds, edm, edk, t3, 13);
   for (size_t i = 0; i < v.size(); i++) bad += v[i] != ede[i];
   bad += sigops::ed25519_eddsa::ecverify_single(eds, edm, edk, 13) != v;
+  bad += sigops::secp256r1_ecdsa::ecrecover_single(r1s, r1d, 13) != q;
+  // the variable-length form on the same 32-byte messages gives the same answers
+  std::vector<std::vector<uint8_t>> edv;
+  for (auto& m : edm) edv.emplace_back(m.begin(), m.end());
+  bad += sigops::ed25519_eddsa::ecverify_bytes(eds, edv, edk) != v;
   bad += !sigops::secp256k1_ecdsa::ecrecover({}, {}, t2, 13).empty();
   int thrown = 0;
   try { sigops::secp256k1_ecdsa::ecrecover(k1s, {}, t2, 13); } catch (const std::invalid_argument&) { thrown++; }

@@ -124,10 +124,24 @@ def _u8(a, shape_tail, name):
     return a
 
 
+def _tuned(device):
+    try:
+        from . import autotune
+        return autotune.load(device)
+    except Exception:
+        return None
+
+
 class Context:
     """One GPU context: resident generator tables, per-lane scratch and reusable I/O buffers."""
 
-    def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True):
+    def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True, tuned=True):
+        if tuned and not blocks_per_cu and not tile_rounds:
+            # the launch shape the autotuner measured fastest on this device (autotune.py), if any
+            t = _tuned(device)
+            if t:
+                blocks_per_cu, tile_rounds = t["blocks_per_cu"], t["tile_rounds"]
+        self.blocks_per_cu, self.tile_rounds = blocks_per_cu, tile_rounds
         cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1)
         self._h = lib().sigops_create(ctypes.byref(cfg))
         if not self._h:
