@@ -118,6 +118,25 @@ int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs,
                      const void* d_msgs, const uint64_t* d_msg_off, size_t n, void* d_out, void* d_status,
                      void* stream);
 
+/* ---- Fuel hashing on the device (SHA-256), the steps either side of the path ----
+   Message::new (the step before recovery: fuel `Message::new(data)` = SHA-256(data), as the
+   reference's tests and benchmarks build their digests, src/tests/ed25519_eddsa.rs:19,
+   src/benchmarks/secp256k1_ecdsa.rs:162): digest i = SHA-256(data[off[i] .. off[i+1])).
+   Addresses (the step after it): address i = SHA-256(X || Y) of a 64-byte public key, 32 zero
+   bytes where the recovery failed (status != 0). No counterpart in the reference's API. */
+int sigops_message_new(sigops_ctx* ctx, const uint8_t* data, const uint64_t* off, size_t n, uint8_t* digests);
+/* device pointers, d_data 4-byte aligned and readable 16 bytes past its last byte */
+int sigops_message_new_dev(sigops_ctx* ctx, const void* d_data, const uint64_t* d_off, size_t n, void* d_digests,
+                           void* stream);
+/* d_status may be NULL (every key hashed) */
+int sigops_address_from_pubkeys_dev(sigops_ctx* ctx, const void* d_pubs, const void* d_status, size_t n,
+                                    void* d_addr, void* stream);
+/* recovery followed by the address, both on the device; only n x 32 bytes come back */
+int sigops_secp256k1_recover_address(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                     uint8_t* out_addr, uint8_t* status);
+int sigops_secp256r1_recover_address(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                     uint8_t* out_addr, uint8_t* status);
+
 /* ---- per-stage timing of the mixed batch (no counterpart in the reference; for profiling).
    With timing on, sigops_batch / sigops_batch_dev launch the stages eagerly (no graph) with HIP
    events recorded on the launch stream between them. sigops_batch_stage_ms waits for the last

@@ -6,6 +6,7 @@
 #include "ed25519.hpp"
 #include "gen.hpp"
 #include "secp.hpp"
+#include "sha256.hpp"
 
 using namespace sigops;
 
@@ -134,6 +135,24 @@ void dc_mont_raw(int op, int mod, const u32* a, const u32* b, u32* out) {
 #undef RAW
   for (int k = 0; k < NL; k++) out[k] = r.v[k];
   bounds_enabled() = true;
+}
+
+// SHA-256 of len bytes placed at byte offset `at` of a word buffer (exercises unaligned starts)
+void dc_sha256(const unsigned char* msg, unsigned len, unsigned at, unsigned char* out) {
+  u32* buf = new u32[(at + len + 16) / 4 + 4]();
+  memcpy((unsigned char*)buf + at, msg, len);
+  MemHost mem{buf};
+  u32 d[8];
+  sha256_msg(mem, at, len, d);
+  memcpy(out, d, 32);
+  delete[] buf;
+}
+
+void dc_sha256_64(const unsigned char* in, unsigned char* out) {
+  u32 w[16], d[8];
+  memcpy(w, in, 64);
+  sha256_64(w, d);
+  memcpy(out, d, 32);
 }
 
 // GLV split of a secp256k1 scalar k (8 little-endian words, k < n): h1, h2 as 5 words each and

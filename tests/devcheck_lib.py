@@ -33,6 +33,8 @@ def lib():
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.dc_mont_raw.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p]
         L.dc_glv_split.argtypes = [u32p, u32p, u32p]
+        L.dc_sha256.argtypes = [c_p, ctypes.c_uint, ctypes.c_uint, c_p]
+        L.dc_sha256_64.argtypes = [c_p, c_p]
         _lib = L
     return _lib
 
@@ -77,3 +79,15 @@ def glv_split(k):
     h1 = sum(v << (32 * i) for i, v in enumerate(H1))
     h2 = sum(v << (32 * i) for i, v in enumerate(H2))
     return (-h1 if sg & 1 else h1), (-h2 if sg & 2 else h2)
+
+
+def sha256(msg, at=0):
+    out = ctypes.create_string_buffer(32)
+    lib().dc_sha256(bytes(msg), len(msg), at, out)
+    return out.raw
+
+
+def sha256_64(data):
+    out = ctypes.create_string_buffer(32)
+    lib().dc_sha256_64(bytes(data), out)
+    return out.raw

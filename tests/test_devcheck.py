@@ -102,6 +102,19 @@ def test_glv_split_secp256k1():
         assert abs(k1) < 2**129 and abs(k2) < 2**129, k
 
 
+def test_sha256_message_new_and_address_forms():
+    """the device SHA-256 (built for the host) against hashlib on every length 0..300 and a few
+    long ones, at every byte alignment of the start, and the fixed 64-byte form used for addresses"""
+    rnd = random.Random(9)
+    for L in list(range(0, 301)) + [1000, 4095, 4096, 4097]:
+        msg = bytes(rnd.randrange(256) for _ in range(L))
+        for at in (0, 1, 2, 3) if L < 80 or L > 999 else (L % 4,):
+            assert D.sha256(msg, at) == hashlib.sha256(msg).digest(), (L, at)
+    for _ in range(50):
+        pk = bytes(rnd.randrange(256) for _ in range(64))
+        assert D.sha256_64(pk) == hashlib.sha256(pk).digest()
+
+
 def test_sha512_preimage_stream():
     rnd = random.Random(2)
     for L in list(range(0, 260)) + [1000, 4096]:

@@ -82,6 +82,11 @@ def lib():
             L.sigops_ed25519_verify_dev.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp]
             L.sigops_batch.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, sz, c_p, c_p]
             L.sigops_batch_dev.argtypes = [vp, vp, vp, vp, vp, vp, sz, vp, vp, vp]
+            L.sigops_message_new.argtypes = [vp, c_p, c_p, sz, c_p]
+            L.sigops_message_new_dev.argtypes = [vp, vp, vp, sz, vp, vp]
+            L.sigops_address_from_pubkeys_dev.argtypes = [vp, vp, vp, sz, vp, vp]
+            for name in ("sigops_secp256k1_recover_address", "sigops_secp256r1_recover_address"):
+                getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
             L.sigops_set_stage_timing.argtypes = [vp, ctypes.c_int]
             L.sigops_batch_stage_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
             L.sigops_gen_secp_dev.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp,
@@ -232,6 +237,41 @@ class Context:
         self._check(lib().sigops_batch_dev(self._h, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status,
                                            stream or None))
 
+    # ---- Fuel hashing (SHA-256) either side of the path ----
+    def message_new(self, data, off=None):
+        """Fuel Message::new on the device: SHA-256 of each message. ``data`` is a list of bytes
+        objects, or one uint8 array with ``off`` (n + 1 offsets). Returns an (n, 32) uint8 array."""
+        if off is None:
+            off = np.zeros(len(data) + 1, np.uint64)
+            np.cumsum([len(m) for m in data], out=off[1:])
+            data = np.frombuffer(b"".join(bytes(m) for m in data), np.uint8)
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        n = len(off) - 1
+        out = np.zeros((n, 32), np.uint8)
+        self._check(lib().sigops_message_new(self._h, _ptr(data) if data.size else None, _ptr(off), n, _ptr(out)))
+        return out
+
+    def message_new_dev(self, d_data, d_off, n, d_digests, stream=0):
+        self._check(lib().sigops_message_new_dev(self._h, d_data, d_off, n, d_digests, stream or None))
+
+    def address_from_pubkeys_dev(self, d_pubs, d_status, n, d_addr, stream=0):
+        self._check(lib().sigops_address_from_pubkeys_dev(self._h, d_pubs, d_status, n, d_addr, stream or None))
+
+    def recover_address(self, curve, sigs, digests):
+        """recovery then Fuel address SHA-256(X || Y) on the device: (n, 32) addresses (zeros for
+        failed items) and the status bytes"""
+        sigs = _u8(sigs, 64, "sigs")
+        digests = _u8(digests, 32, "digests")
+        n = sigs.shape[0]
+        if digests.shape[0] != n:
+            raise ValueError("signatures and digests differ in length")
+        out = np.zeros((n, 32), np.uint8)
+        st = np.zeros(n, np.uint8)
+        fn = lib().sigops_secp256k1_recover_address if curve == 0 else lib().sigops_secp256r1_recover_address
+        self._check(fn(self._h, _ptr(sigs), _ptr(digests), n, _ptr(out), _ptr(st)))
+        return out, st
+
     STAGES = ("bucketing", "ed25519_hash", "verify_kernel", "careful")
 
     def set_stage_timing(self, on):
@@ -351,4 +391,4 @@ def default_context():
     return _default_ctx
 
 
-from . import ed25519_eddsa, precompute, secp256k1_ecdsa, secp256r1_ecdsa  # noqa: E402,F401
+from . import ed25519_eddsa, fuel, precompute, secp256k1_ecdsa, secp256r1_ecdsa  # noqa: E402,F401

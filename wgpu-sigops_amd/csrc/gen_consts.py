@@ -366,6 +366,20 @@ def sha512_constants():
     return K, IV
 
 
+def sha256_constants():
+    """FIPS 180-4: first 32 bits of the fractional parts of the cube roots of the first 64 primes
+    (K) and of the square roots of the first 8 primes (IV)."""
+    primes = []
+    x = 2
+    while len(primes) < 64:
+        if all(x % q for q in primes if q * q <= x):
+            primes.append(x)
+        x += 1
+    K = [_iroot(q << 96, 3) & (2**32 - 1) for q in primes]
+    IV = [_iroot(q << 64, 2) & (2**32 - 1) for q in primes[:8]]
+    return K, IV
+
+
 def main():
     o = sys.stdout
     o.write("// Generated by gen_consts.py -- do not edit.\n#pragma once\n#include <cstdint>\n\n")
@@ -417,6 +431,9 @@ def main():
     K, IV = sha512_constants()
     o.write(arr("SHA512_K", K, "uint64_t"))
     o.write(arr("SHA512_IV", IV, "uint64_t"))
+    K2, IV2 = sha256_constants()
+    o.write(arr("SHA256_K", K2))
+    o.write(arr("SHA256_IV", IV2))
     o.write("\n// host-side generator tables: entries of 20 u32 (x[9], y[9], pad[2]) for secp,\n")
     o.write("// 28 u32 (ypx[9], ymx[9], t2d[9], pad) for ed25519; Montgomery form\n")
     ntab = 1 << (WG - 1)

@@ -54,6 +54,12 @@ struct BatchLaunch {
 
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
 
+// SHA-256: digests of messages (Fuel Message::new) and addresses of 64-byte public keys
+hipError_t launch_message_new(const void* msgs, const uint64_t* off, uint32_t n, void* out, uint32_t grid,
+                              hipStream_t s);
+hipError_t launch_fuel_address(const void* pubs, const void* status, uint32_t n, void* out, uint32_t grid,
+                               hipStream_t s);
+
 hipError_t launch_gen_secp(int curve, uint64_t seed, uint32_t n, uint32_t ppm, const void* gtab, void* sigs,
                            void* digs, void* pubs, void* corrupted, uint32_t grid, hipStream_t s);
 hipError_t launch_gen_ed25519(uint64_t seed, uint32_t n, uint32_t ppm, const void* btab, const uint64_t* msg_off,

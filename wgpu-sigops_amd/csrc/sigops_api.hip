@@ -109,12 +109,14 @@ int secp_recover_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d
   return SIGOPS_OK;
 }
 
+// out_pub: n x 64 recovered keys; or, with out_addr set instead, n x 32 addresses SHA-256(X || Y)
+// computed on the device after the recovery (only the addresses travel back)
 int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8_t* digs, size_t n,
-                      uint8_t* out_pub, uint8_t* status) {
+                      uint8_t* out_pub, uint8_t* status, uint8_t* out_addr = nullptr) {
   if (!c) return SIGOPS_E_ARG;
   c->err.clear();
   if (n == 0) return SIGOPS_OK;
-  if (!sigs || !digs || !out_pub) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (!sigs || !digs || !(out_pub || out_addr)) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   int rc;
   if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) || (rc = ensure(c, c->out, n * 64)) ||
@@ -131,7 +133,13 @@ int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8
     c->host_status.resize(n);
     st = c->host_status.data();
   }
-  HIPCHK(c, hipMemcpyAsync(out_pub, c->out.p, n * 64, hipMemcpyDeviceToHost, s));
+  if (out_addr) {
+    // the digests in aux are consumed by the recovery launched before on this stream
+    HIPCHK(c, sigops::launch_fuel_address(c->out.p, c->st.p, (uint32_t)n, c->aux.p, c->grid, s));
+    HIPCHK(c, hipMemcpyAsync(out_addr, c->aux.p, n * 32, hipMemcpyDeviceToHost, s));
+  } else {
+    HIPCHK(c, hipMemcpyAsync(out_pub, c->out.p, n * 64, hipMemcpyDeviceToHost, s));
+  }
   HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   for (size_t i = 0; i < n; i++)
@@ -380,6 +388,66 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
     it = c->graphs.emplace(key, ex).first;
   }
   HIPCHK(c, hipGraphLaunch(it->second, s));
+  return SIGOPS_OK;
+}
+
+int sigops_secp256k1_recover_address(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                     uint8_t* out_addr, uint8_t* status) {
+  if (!out_addr && n) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  return secp_recover_host(c, 0, sigs, digests, n, nullptr, status, out_addr);
+}
+
+int sigops_secp256r1_recover_address(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                     uint8_t* out_addr, uint8_t* status) {
+  if (!out_addr && n) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  return secp_recover_host(c, 1, sigs, digests, n, nullptr, status, out_addr);
+}
+
+int sigops_message_new_dev(sigops_ctx* c, const void* d_data, const uint64_t* d_off, size_t n, void* d_digests,
+                           void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  if (n == 0) return SIGOPS_OK;
+  if (n > 0xFFFFFFFFull) return fail(c, SIGOPS_E_ARG, "n too large");
+  if (!d_data || !d_off || !d_digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (((uintptr_t)d_data & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_data must be 4-byte aligned");
+  HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, c->grid, (hipStream_t)stream));
+  return SIGOPS_OK;
+}
+
+int sigops_message_new(sigops_ctx* c, const uint8_t* data, const uint64_t* off, size_t n, uint8_t* digests) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  if (n == 0) return SIGOPS_OK;
+  if (!off || !digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  const uint64_t base = off[0];
+  for (size_t i = 0; i < n; i++)
+    if (off[i + 1] < off[i] || off[i + 1] - off[i] > 0xFFFFFFFFull)
+      return fail(c, SIGOPS_E_ARG, "off must be non-decreasing with lengths < 2^32");
+  const uint64_t total = off[n] - base;
+  if (total && !data) return fail(c, SIGOPS_E_ARG, "NULL data");
+  int rc;
+  if ((rc = ensure(c, c->msg, total + 16)) || (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->aux, n * 32)))
+    return rc;
+  c->host_off.resize(n + 1);
+  for (size_t i = 0; i <= n; i++) c->host_off[i] = off[i] - base;
+  hipStream_t s = c->stream;
+  if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, data + base, total, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
+  HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, sigops::launch_message_new(c->msg.p, (const uint64_t*)c->off.p, (uint32_t)n, c->aux.p, c->grid, s));
+  HIPCHK(c, hipMemcpyAsync(digests, c->aux.p, n * 32, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return SIGOPS_OK;
+}
+
+int sigops_address_from_pubkeys_dev(sigops_ctx* c, const void* d_pubs, const void* d_status, size_t n, void* d_addr,
+                                    void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  if (n == 0) return SIGOPS_OK;
+  if (n > 0xFFFFFFFFull) return fail(c, SIGOPS_E_ARG, "n too large");
+  if (!d_pubs || !d_addr) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  HIPCHK(c, sigops::launch_fuel_address(d_pubs, d_status, (uint32_t)n, d_addr, c->grid, (hipStream_t)stream));
   return SIGOPS_OK;
 }
 
