@@ -36,6 +36,66 @@ MAC_PER_SIG = {"secp256k1": 595072, "secp256r1": 627456, "ed25519_per_block": 24
 LIMBS_MAC = 128  # MACs per canonical 256-bit modular multiplication
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, steps, dev):
+    """Fuel hashing either side of the path, each form timed with events on the launch stream and
+    checked against hashlib on a sample: Message::new over 2^20 64-byte messages (the reference's
+    benchmarks sign Message::new(hex of a 32-byte value), src/benchmarks/secp256k1_ecdsa.rs:162),
+    Message::new over the mixed batch's own layout (secp items hash empty messages, Ed25519 items
+    their log-uniform 0..4096-byte messages), and the addresses of the 2^20 recovered keys."""
+    import hashlib
+    import torch
+    hexchars = torch.tensor(list(b"0123456789abcdef"), dtype=torch.uint8, device=dev)
+    g = torch.Generator(device=dev).manual_seed(11)
+    hex_msgs = torch.zeros(n * 64 + 16, dtype=torch.uint8, device=dev)
+    hex_msgs[: n * 64] = hexchars[torch.randint(0, 16, (n * 64,), device=dev, generator=g)]
+    hex_off = torch.arange(0, n + 1, dtype=torch.int64, device=dev) * 64
+    forms = {"hex64": (hex_msgs, hex_off, hex_off.cpu().numpy()), "mixed": (mx.msgs, mx.msg_off, mx.msg_off_host)}
+    dig = torch.zeros((n, 32), dtype=torch.uint8, device=dev)
+    addr = torch.zeros((n, 32), dtype=torch.uint8, device=dev)
+    rng = np.random.default_rng(5)
+    res = {"messages": n}
+    ok = True
+    for name, (msgs, off, off_h) in forms.items():
+        ctx.message_new_dev(msgs.data_ptr(), off.data_ptr(), n, dig.data_ptr(), s_ptr)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            ctx.message_new_dev(msgs.data_ptr(), off.data_ptr(), n, dig.data_ptr(), s_ptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        h_dig, h_msgs = dig.cpu().numpy(), msgs.cpu().numpy()
+        for i in rng.choice(n, 2000, replace=False):
+            ok &= h_dig[i].tobytes() == hashlib.sha256(h_msgs[off_h[i]:off_h[i + 1]].tobytes()).digest()
+        nbytes = int(off_h[-1])
+        blocks = int(((np.diff(off_h) + 72) >> 6).sum())
+        res[f"message_new_{name}"] = {"ms": round(ms, 3), "messages_per_s": round(n / (ms / 1e3), 1),
+                                      "GBps": round(nbytes / (ms / 1e3) / 1e9, 2), "bytes": nbytes,
+                                      "sha256_blocks": blocks, "blocks_per_s": round(blocks / (ms / 1e3), 1)}
+        log(f"fuel Message::new ({name}): {ms:.3f} ms, {n / ms / 1e3:.1f} M msgs/s, "
+            f"{nbytes / (ms / 1e3) / 1e9:.1f} GB/s, {blocks / ms / 1e3:.1f} M blocks/s")
+    ctx.address_from_pubkeys_dev(out.data_ptr(), st.data_ptr(), n, addr.data_ptr(), s_ptr)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        ctx.address_from_pubkeys_dev(out.data_ptr(), st.data_ptr(), n, addr.data_ptr(), s_ptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    adr_ms = e0.elapsed_time(e1) / steps
+    h_addr, h_out, h_st = addr.cpu().numpy(), out.cpu().numpy(), st.cpu().numpy()
+    for i in rng.choice(n, 2000, replace=False):
+        want = hashlib.sha256(h_out[i].tobytes()).digest() if h_st[i] == 0 else bytes(32)
+        ok &= h_addr[i].tobytes() == want
+    res["address"] = {"ms": round(adr_ms, 3), "addresses_per_s": round(n / (adr_ms / 1e3), 1)}
+    res["sample_matches_hashlib"] = bool(ok)
+    log(f"fuel addresses: {adr_ms:.3f} ms, {n / adr_ms / 1e3:.1f} M/s; sample exact: {ok}")
+    return res
+
+
 def mixed_mac_per_sig(schemes, lens):
     """average canonical MACs per signature of this exact batch"""
     k1 = (schemes == 0).sum()
@@ -96,7 +156,6 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     n = args.n
-    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 
     if args.calib_copy_mib:
         # a plain copy kernel of known size, read by the FETCH_SIZE / WRITE_SIZE counter passes
@@ -153,6 +212,8 @@ def main():
     ctx.set_stage_timing(False)
     stage_ms = {k: sum(r[k] for r in stage_runs) / len(stage_runs) for k in stage_runs[0]}
     log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
+
+    fuel = fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, args.steps, dev)
 
     # correctness of the timed output by the round-trip property (all 2^20 items)
     out_h, st_h, cor = out.cpu().numpy(), st.cpu().numpy(), mx.corrupted.cpu().numpy()
@@ -359,6 +420,7 @@ def main():
                            "identical_to_device_path": incl_ok},
         "roundtrip_all_items_ok": roundtrip_ok,
         "served": served,
+        "fuel_hashing": fuel,
         "probe": {"mad_per_s": mad_per_s, "fma64_per_s": fma_per_s},
         "setup_ms": round(setup_ms, 1),
     }

@@ -36,7 +36,11 @@ extern "C" {
 #define SIGOPS_ST_BAD_R 1     /* r == 0 or r >= n */
 #define SIGOPS_ST_BAD_S 2     /* s == 0 or s >= n after clearing the recovery bit */
 #define SIGOPS_ST_NO_POINT 3  /* r is not the x-coordinate of a curve point */
-#define SIGOPS_ST_INFINITY 4  /* the recovered key is the point at infinity */
+#define SIGOPS_ST_INFINITY 4  /* the recovered key is the point at infinity (verification: u1 G + u2 Q is) */
+/* verification with a known public key only */
+#define SIGOPS_ST_HIGH_S 5    /* secp256k1: s > (n-1)/2 (libsecp256k1 verifies lower-S signatures only) */
+#define SIGOPS_ST_BAD_KEY 6   /* the public key is not a curve point with X, Y < p */
+#define SIGOPS_ST_MISMATCH 7  /* x(u1 G + u2 Q) mod n != r */
 
 /* per-item status for Ed25519 verification (valid[i] = status == 0) */
 #define SIGOPS_ED_OK 0
@@ -92,6 +96,18 @@ int sigops_secp256r1_recover(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t
 int sigops_ed25519_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
                           const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status);
 
+/* ECDSA verification with a known public key, fuel `Signature::verify(&public_key, &message)`
+   (the predicate next to recovery; fuel-crypto 0.49 verifies secp256k1 with libsecp256k1, which
+   accepts lower-S signatures only; secp256r1 follows p256 0.13 / ecdsa 0.16 `verify_prehash`).
+   sigs, digests as for recovery (the recid bit is ignored); pubkeys: n x 64 bytes X||Y big-endian;
+   valid: n bytes (1 valid, 0 invalid); status: n bytes (SIGOPS_ST_*) or NULL.
+   No counterpart among the reference's entry points: its tests verify by recovering and comparing
+   keys (src/tests/secp256k1_ecdsa.rs:27-30). */
+int sigops_secp256k1_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubkeys,
+                            size_t n, uint8_t* valid, uint8_t* status);
+int sigops_secp256r1_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubkeys,
+                            size_t n, uint8_t* valid, uint8_t* status);
+
 /* ---- device-resident variants: pointers are device memory, work is enqueued on `stream`
    (a hipStream_t, NULL for the default stream) and the call returns without synchronising.
    The context's scratch is shared, so calls on one context must not overlap. ---- */
@@ -99,6 +115,11 @@ int sigops_secp256k1_recover_dev(sigops_ctx* ctx, const void* d_sigs, const void
                                  void* d_out_pub, void* d_status, void* stream);
 int sigops_secp256r1_recover_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_digests, size_t n,
                                  void* d_out_pub, void* d_status, void* stream);
+/* d_status: n bytes SIGOPS_ST_* (0 = valid) */
+int sigops_secp256k1_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_digests, const void* d_pubkeys,
+                                size_t n, void* d_status, void* stream);
+int sigops_secp256r1_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_digests, const void* d_pubkeys,
+                                size_t n, void* d_status, void* stream);
 /* d_msgs must stay readable for 16 bytes past its last message byte */
 int sigops_ed25519_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
                               const uint64_t* d_msg_off, size_t n, void* d_status, void* stream);
@@ -125,7 +146,8 @@ int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs,
    Addresses (the step after it): address i = SHA-256(X || Y) of a 64-byte public key, 32 zero
    bytes where the recovery failed (status != 0). No counterpart in the reference's API. */
 int sigops_message_new(sigops_ctx* ctx, const uint8_t* data, const uint64_t* off, size_t n, uint8_t* digests);
-/* device pointers, d_data 4-byte aligned and readable 16 bytes past its last byte */
+/* device pointers, d_data 4-byte aligned and readable 16 bytes past its last byte; n < 2^31.
+   The context's hashing work counter is shared: calls on one context must not overlap on the device. */
 int sigops_message_new_dev(sigops_ctx* ctx, const void* d_data, const uint64_t* d_off, size_t n, void* d_digests,
                            void* stream);
 /* d_status may be NULL (every key hashed) */

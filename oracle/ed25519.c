@@ -209,13 +209,33 @@ static void sha512_3(uint8_t out[64], const uint8_t* a, size_t la, const uint8_t
   if (buf != stackbuf) free(buf);
 }
 
-int or_ed25519_verify(const uint8_t sig[64], const uint8_t pk[32], const uint8_t* msg, size_t len) {
+/* order divides 8: [8]P is the identity (X = 0, Y = Z) */
+static int small_order(const ept* P) {
+  ept Q = *P;
+  for (int i = 0; i < 3; i++) ed_add(&Q, &Q, &Q);
+  return u256_is_zero(&Q.X) && mod_eq(&Q.Y, &Q.Z);
+}
+
+int or_ed25519_is_small_order(const uint8_t enc[32]) {
+  init_ed();
+  ept P;
+  return decompress(&P, enc) && small_order(&P);
+}
+
+/* verify (strict = 0) and dalek 2.1 verify_strict (strict = 1), which also requires that R
+   decompresses and that neither R nor A has small order; both compare compressed R bytes */
+static int verify_impl(const uint8_t sig[64], const uint8_t pk[32], const uint8_t* msg, size_t len, int strict) {
   init_ed();
   u256 s;
   u256_from_le(&s, sig + 32);
   if (u256_cmp(&s, &Fl->m) >= 0) return 0;
   ept A;
   if (!decompress(&A, pk)) return 0;
+  if (strict) {
+    ept R;
+    if (!decompress(&R, sig)) return 0;
+    if (small_order(&R) || small_order(&A)) return 0;
+  }
   uint8_t h[64];
   sha512_3(h, sig, 32, pk, 32, msg, len);
   u256 k;
@@ -230,6 +250,14 @@ int or_ed25519_verify(const uint8_t sig[64], const uint8_t pk[32], const uint8_t
   uint8_t enc[32];
   compress(enc, &P);
   return memcmp(enc, sig, 32) == 0;
+}
+
+int or_ed25519_verify(const uint8_t sig[64], const uint8_t pk[32], const uint8_t* msg, size_t len) {
+  return verify_impl(sig, pk, msg, len, 0);
+}
+
+int or_ed25519_verify_strict(const uint8_t sig[64], const uint8_t pk[32], const uint8_t* msg, size_t len) {
+  return verify_impl(sig, pk, msg, len, 1);
 }
 
 static void expand_seed(const uint8_t seed[32], u256* a, uint8_t prefix[32]) {

@@ -95,6 +95,28 @@ void or_ed25519_verify_batch(const uint8_t* sigs, const uint8_t* pks, const uint
   parallel_for(n, nthreads, ver_item, &c);
 }
 
+static void ver_strict_item(void* c, size_t i) {
+  ver_ctx* v = (ver_ctx*)c;
+  v->valid[i] = (uint8_t)or_ed25519_verify_strict(v->sigs + 64 * i, v->pks + 32 * i, v->msgs + v->off[i],
+                                                  (size_t)(v->off[i + 1] - v->off[i]));
+}
+void or_ed25519_verify_strict_batch(const uint8_t* sigs, const uint8_t* pks, const uint8_t* msgs,
+                                    const uint64_t* msg_off, size_t n, uint8_t* valid, int nthreads) {
+  ver_ctx c = {sigs, pks, msgs, msg_off, valid};
+  parallel_for(n, nthreads, ver_strict_item, &c);
+}
+
+typedef struct { int curve; const uint8_t *sigs, *digests, *pubs; uint8_t* status; } sver_ctx;
+static void sver_item(void* c, size_t i) {
+  sver_ctx* v = (sver_ctx*)c;
+  v->status[i] = (uint8_t)or_secp_verify(v->curve, v->sigs + 64 * i, v->digests + 32 * i, v->pubs + 64 * i);
+}
+void or_secp_verify_batch(int curve, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubs, size_t n,
+                          uint8_t* status, int nthreads) {
+  sver_ctx c = {curve, sigs, digests, pubs, status};
+  parallel_for(n, nthreads, sver_item, &c);
+}
+
 /* ---- generation ---- */
 enum { TAG_SK = 1, TAG_DIGEST = 2, TAG_NONCE = 3, TAG_CORRUPT = 4, TAG_SEED = 5, TAG_MSG = 6 };
 

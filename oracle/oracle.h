@@ -36,7 +36,10 @@ enum {
   OR_ST_BAD_R = 1,       /* r == 0 or r >= n */
   OR_ST_BAD_S = 2,       /* s == 0 or s >= n (after clearing the recid bit) */
   OR_ST_NO_POINT = 3,    /* x = r is not the x-coordinate of a curve point */
-  OR_ST_INFINITY = 4     /* recovered key is the point at infinity */
+  OR_ST_INFINITY = 4,    /* recovered key (verification: u1*G + u2*Q) is the point at infinity */
+  OR_ST_HIGH_S = 5,      /* verification, secp256k1: s > (n-1)/2 */
+  OR_ST_BAD_KEY = 6,     /* verification: the public key is not a curve point with X, Y < p */
+  OR_ST_MISMATCH = 7     /* verification: x(u1*G + u2*Q) mod n != r */
 };
 
 /* ---- hashes ---- */
@@ -52,7 +55,15 @@ int or_secp_pubkey(int curve, const uint8_t sk[32], uint8_t out_pub[64]);
 int or_secp_sign(int curve, const uint8_t sk[32], const uint8_t digest[32], const uint8_t nonce[32],
                  uint8_t out_sig[64]);
 
+/* ECDSA verification with a known public key X||Y (fuel Signature::verify); returns OR_ST_* */
+int or_secp_verify(int curve, const uint8_t sig[64], const uint8_t digest[32], const uint8_t pub[64]);
+
 int or_ed25519_verify(const uint8_t sig[64], const uint8_t pk[32], const uint8_t* msg, size_t len);
+/* dalek VerifyingKey::verify_strict: verify plus R must decompress and neither R nor A may have
+   small order (returns 1 if valid) */
+int or_ed25519_verify_strict(const uint8_t sig[64], const uint8_t pk[32], const uint8_t* msg, size_t len);
+/* 1 if the encoding decompresses (dalek rules) to a point of order dividing 8 */
+int or_ed25519_is_small_order(const uint8_t enc[32]);
 void or_ed25519_pubkey(const uint8_t seed[32], uint8_t out_pk[32]);
 void or_ed25519_sign(const uint8_t seed[32], const uint8_t* msg, size_t len, uint8_t out_sig[64]);
 
@@ -78,6 +89,10 @@ void or_secp_recover_batch(int curve, const uint8_t* sigs, const uint8_t* digest
                            uint8_t* out_pub, uint8_t* status, int nthreads);
 void or_ed25519_verify_batch(const uint8_t* sigs, const uint8_t* pks, const uint8_t* msgs,
                              const uint64_t* msg_off, size_t n, uint8_t* valid, int nthreads);
+void or_ed25519_verify_strict_batch(const uint8_t* sigs, const uint8_t* pks, const uint8_t* msgs,
+                                    const uint64_t* msg_off, size_t n, uint8_t* valid, int nthreads);
+void or_secp_verify_batch(int curve, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubs, size_t n,
+                          uint8_t* status, int nthreads);
 
 /* ---- seeded workload generation (deterministic, independent of thread count) ----
  * Item i draws from a per-item stream derived from (seed, i).

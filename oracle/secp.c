@@ -271,6 +271,53 @@ int or_secp_recover(int curve, const uint8_t sig[64], const uint8_t digest[32], 
   return OR_ST_OK;
 }
 
+/*
+ * ECDSA verification with a known public key, fuel `Signature::verify(&pk, &message)`
+ * (the Fuel predicate next to recovery). secp256k1 follows fuel-crypto 0.49 over libsecp256k1
+ * (secp256k1_ecdsa_signature_parse_compact, secp256k1_ec_pubkey_parse of 0x04 || X || Y,
+ * secp256k1_ecdsa_verify); secp256r1 follows p256 0.13 / ecdsa 0.16 VerifyingKey::verify_prehash:
+ *   r, s in [1, n-1] after clearing the recid bit of s (which verification ignores);
+ *   secp256k1 only: s <= (n-1)/2 (libsecp256k1 accepts lower-S signatures only);
+ *   Q = X || Y with X, Y < p on the curve;
+ *   z = digest mod n, u1 = z/s, u2 = r/s, P = u1*G + u2*Q != infinity, accept iff x(P) mod n == r.
+ */
+int or_secp_verify(int curve, const uint8_t sig[64], const uint8_t digest[32], const uint8_t pub[64]) {
+  const curve_t* C = or_curve(curve);
+  const modctx* Fn = C->fn;
+  uint8_t sb[32];
+  memcpy(sb, sig + 32, 32);
+  sb[0] &= 0x7f;
+  u256 r, s, z, zr;
+  u256_from_be(&r, sig);
+  u256_from_be(&s, sb);
+  u256_from_be(&z, digest);
+  if (u256_is_zero(&r) || u256_cmp(&r, &Fn->m) >= 0) return OR_ST_BAD_R;
+  if (u256_is_zero(&s) || u256_cmp(&s, &Fn->m) >= 0) return OR_ST_BAD_S;
+  if (curve == OR_CURVE_K1) {
+    u256 half = Fn->m;
+    for (int i = 0; i < 4; i++) half.v[i] = (half.v[i] >> 1) | (i < 3 ? half.v[i + 1] << 63 : 0);
+    if (u256_cmp(&s, &half) > 0) return OR_ST_HIGH_S;
+  }
+  jpoint Q;
+  if (!load_affine(C, &Q, pub)) return OR_ST_BAD_KEY;
+  mod_reduce256(Fn, &zr, &z);
+  u256 sm, sinv, zm, rm, u1m, u2m, u1, u2;
+  mod_to(Fn, &sm, &s);
+  mod_inv(Fn, &sinv, &sm);
+  mod_to(Fn, &zm, &zr);
+  mod_to(Fn, &rm, &r);
+  mod_mul(Fn, &u1m, &zm, &sinv);
+  mod_mul(Fn, &u2m, &rm, &sinv);
+  mod_from(Fn, &u1, &u1m);
+  mod_from(Fn, &u2, &u2m);
+  jpoint P;
+  jp_mul2(C, &P, &Q, &u1, &u2);
+  u256 x, y;
+  if (!jp_to_affine(C, &x, &y, &P)) return OR_ST_INFINITY;
+  if (u256_cmp(&x, &Fn->m) >= 0) u256_sub(&x, &x, &Fn->m); /* x < p < 2n */
+  return u256_cmp(&x, &r) == 0 ? OR_ST_OK : OR_ST_MISMATCH;
+}
+
 int or_secp_pubkey(int curve, const uint8_t sk[32], uint8_t out_pub[64]) {
   const curve_t* C = or_curve(curve);
   u256 d;

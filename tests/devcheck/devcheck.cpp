@@ -138,14 +138,31 @@ void dc_mont_raw(int op, int mod, const u32* a, const u32* b, u32* out) {
 }
 
 // SHA-256 of len bytes placed at byte offset `at` of a word buffer (exercises unaligned starts)
-void dc_sha256(const unsigned char* msg, unsigned len, unsigned at, unsigned char* out) {
+// memory view that records reads outside the words [lo, hi]
+struct MemChecked {
+  const u32* w;
+  u64 lo, hi;
+  mutable bool outside;
+  u32 word(u64 i) const {
+    if (i < lo || i > hi) {
+      outside = true;
+      return 0;
+    }
+    return w[i];
+  }
+};
+
+// returns -1 if the hash read a word other than those holding message bytes (for an empty message:
+// the word at its offset)
+int dc_sha256(const unsigned char* msg, unsigned len, unsigned at, unsigned char* out) {
   u32* buf = new u32[(at + len + 16) / 4 + 4]();
   memcpy((unsigned char*)buf + at, msg, len);
-  MemHost mem{buf};
+  MemChecked mem{buf, at / 4u, (at + len - (len != 0)) / 4u, false};
   u32 d[8];
   sha256_msg(mem, at, len, d);
   memcpy(out, d, 32);
   delete[] buf;
+  return mem.outside ? -1 : 0;
 }
 
 void dc_sha256_64(const unsigned char* in, unsigned char* out) {

@@ -83,7 +83,9 @@ def glv_split(k):
 
 def sha256(msg, at=0):
     out = ctypes.create_string_buffer(32)
-    lib().dc_sha256(bytes(msg), len(msg), at, out)
+    rc = lib().dc_sha256(bytes(msg), len(msg), at, out)
+    if rc != 0:
+        raise AssertionError(f"SHA-256 of {len(msg)} bytes at offset {at} read outside the message's words")
     return out.raw
 
 

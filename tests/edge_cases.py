@@ -89,6 +89,119 @@ def secp_expected(curve, cases):
     return [pyref.secp_recover(curve, sig, dig) for (_, sig, dig) in cases]
 
 
+def secp_verify_cases(curve, seed=8):
+    """list of (name, sig64, digest32, pub64) for verification with a known key; expected via
+    pyref.secp_verify. The key is solved from a chosen signature, Q = r^-1 (s P - z G), which gives
+    full control over r, s, z and the point P the verifier reaches."""
+    c = pyref.CURVES[curve]
+    p, n, G = c["p"], c["n"], c["G"]
+    rnd = random.Random(seed + 1000 * curve)
+    half = (n - 1) // 2
+    out = []
+
+    def enc_pub(Q):
+        return Q[0].to_bytes(32, "big") + Q[1].to_bytes(32, "big")
+
+    def neg(P):
+        return (P[0], (p - P[1]) % p)
+
+    def solve_key(P, r, s, z):
+        """Q with (z/s) G + (r/s) Q = P"""
+        T = pyref.w_add(c, pyref.w_mul(c, P, s), neg(pyref.w_mul(c, G, z % n)) if z % n else None)
+        return pyref.w_mul(c, T, pow(r, -1, n)) if T is not None else None
+
+    def sig(r, s, odd=0):
+        b = bytearray(r.to_bytes(32, "big") + s.to_bytes(32, "big"))
+        b[32] |= odd << 7
+        return bytes(b)
+
+    # an honest low-S signature
+    sk = rnd.randrange(1, n)
+    Q = pyref.w_mul(c, G, sk)
+    z = rnd.randrange(0, 2**256)
+    k = rnd.randrange(1, n)
+    R = pyref.w_mul(c, G, k)
+    r = R[0] % n
+    s = pow(k, -1, n) * (z % n + r * sk) % n
+    if s > half:
+        s = n - s
+    d = z.to_bytes(32, "big")
+    pub = enc_pub(Q)
+    out.append(("valid", sig(r, s), d, pub))
+    out.append(("valid_recid_bit_ignored", sig(r, s, 1), d, pub))
+    out.append(("wrong_digest", sig(r, s), ((z + 1) % 2**256).to_bytes(32, "big"), pub))
+    out.append(("wrong_key", sig(r, s), d, enc_pub(pyref.w_mul(c, G, sk + 1))))
+    out.append(("key_is_G", sig(r, s), d, enc_pub(G)))
+    for name, rr in [("r_zero", 0), ("r_eq_n", n), ("r_max", 2**256 - 1)]:
+        out.append((name, sig(rr, s), d, pub))
+    # fuel's format keeps s below 2^255 (bit 255 is the recid) and n > 2^255, so s >= n cannot occur
+    out.append(("s_zero", sig(r, 0), d, pub))
+    # digest >= n is reduced
+    for name, zz in [("z_plus_n", z % n + n), ("z_max", 2**256 - 1), ("z_zero", 0)]:
+        if zz < 2**256:
+            P = pyref.w_mul(c, G, rnd.randrange(1, n))
+            rr = P[0] % n
+            ss = rnd.randrange(1, half)
+            Qz = solve_key(P, rr, ss, zz)
+            if Qz is not None:
+                out.append((name, sig(rr, ss), zz.to_bytes(32, "big"), enc_pub(Qz)))
+    # low-S boundary, with otherwise valid equations: s = (n-1)/2, (n+1)/2 and the largest s the
+    # format holds (secp256k1 rejects the last two, P-256 accepts them)
+    for name, ss in [("s_half", half), ("s_half_plus_1", half + 1), ("s_max", 2**255 - 1)]:
+        P = pyref.w_mul(c, G, rnd.randrange(1, n))
+        rr = P[0] % n
+        zz = rnd.randrange(0, n)
+        out.append((name, sig(rr, ss), zz.to_bytes(32, "big"), enc_pub(solve_key(P, rr, ss, zz))))
+    # bad keys
+    bad_y = bytearray(pub)
+    bad_y[63] ^= 1
+    out.append(("key_not_on_curve", sig(r, s), d, bytes(bad_y)))
+    out.append(("key_zero", sig(r, s), d, bytes(64)))
+    out.append(("key_x_ge_p", sig(r, s), d, (Q[0] + p).to_bytes(32, "big") + pub[32:] if Q[0] + p < 2**256
+                else b"\xff" * 32 + pub[32:]))
+    out.append(("key_y_plus_p", sig(r, s), d, pub[:32] + (Q[1] + p).to_bytes(32, "big") if Q[1] + p < 2**256
+                else pub[:32] + b"\xff" * 32))
+    # the verifier's point at infinity: Q = -(z/r) G
+    rr, ss, zz = rnd.randrange(1, n), rnd.randrange(1, half), rnd.randrange(1, n)
+    Qi = pyref.w_mul(c, G, (-zz * pow(rr, -1, n)) % n)
+    out.append(("infinity", sig(rr, ss), zz.to_bytes(32, "big"), enc_pub(Qi)))
+    # x(P) in [n, p): accepted through x mod n == r (the r + n comparison)
+    x = n + 1
+    while x < p:
+        rhs = (x**3 + c["a"] * x + c["b"]) % p
+        y = pow(rhs, (p + 1) // 4, p)
+        if y * y % p == rhs:
+            break
+        x += 1
+    if x < p:
+        P = (x, y)
+        rr = x - n
+        ss, zz = rnd.randrange(1, half), rnd.randrange(0, n)
+        out.append(("x_ge_n", sig(rr, ss), zz.to_bytes(32, "big"), enc_pub(solve_key(P, rr, ss, zz))))
+        out.append(("x_ge_n_r_is_x", sig(x, ss), zz.to_bytes(32, "big"), enc_pub(solve_key(P, rr, ss, zz))))
+    # exceptional additions in windowed ladders: Q = m G with chosen (u1, u2); P = (u1 + u2 m) G
+    for m in [1, n - 1, 2, 3, 15, 16, 17, 255, 2**128 + 1]:
+        for (u1, u2) in [(1, 1), (1, n - 1), (n - 1, 1), (2, 3), (0, 1), (5, n - 5), (2**128, 2**128),
+                         (17, 1), (1, 17), ((-7 * m) % n, 7)]:
+            Pm = pyref.w_mul(c, G, (u1 + u2 * m) % n) if (u1 + u2 * m) % n else None
+            if Pm is None:
+                continue
+            rr = Pm[0] % n
+            if rr == 0:
+                continue
+            ss = rr * pow(u2, -1, n) % n
+            zz = u1 * ss % n
+            if ss > half:  # (r, n - s) negates u1, u2 and P (same x) and fits below 2^255
+                ss = n - ss
+            out.append((f"crafted_m{m % 1000}_u{u1 % 1000}_{u2 % 1000}", sig(rr, ss), zz.to_bytes(32, "big"),
+                        enc_pub(pyref.w_mul(c, G, m))))
+    return out
+
+
+def secp_verify_expected(curve, cases):
+    return [pyref.secp_verify(curve, sg, dg, pb) for (_, sg, dg, pb) in cases]
+
+
 # ---------------- ed25519 ----------------
 def _torsion8():
     """a point of exact order 8 (in the torsion subgroup), found as l * P for a curve point P"""
@@ -191,3 +304,47 @@ def ed_cases(seed=11):
 
 def ed_expected(cases):
     return [pyref.ed25519_verify(sig, pk, m) for (_, sig, pk, m) in cases]
+
+
+def ed_strict_cases(seed=12):
+    """ed_cases plus cases aimed at verify_strict: R of small order (with a small-order key and
+    s = 0, so that the cofactorless equation holds and plain verify accepts), and R encodings that
+    do not decompress or are non-canonical"""
+    rnd = random.Random(seed)
+    out = list(ed_cases())
+    EL, EP = pyref.EL, pyref.EP
+    tors = pyref.ed25519_torsion()
+    T8 = _torsion8()
+    A = pyref.e_compress(T8)
+    hits = 0
+    for i in range(400):
+        Rj = tors[i % 8]
+        Rb = pyref.e_compress(Rj)
+        m = bytes([i & 255, i >> 8]) + b"strict"
+        k = int.from_bytes(hashlib.sha512(Rb + A + m).digest(), "little") % EL
+        negkA = pyref.e_mul(((EP - T8[0]) % EP, T8[1]), k)
+        if pyref.e_compress(negkA) == Rb:  # [0]B - [k]A == R: verify accepts, verify_strict must not
+            out.append((f"R_small_order_{i}", Rb + bytes(32), A, m))
+            hits += 1
+            if hits == 6:
+                break
+    seedk = bytes(rnd.randrange(256) for _ in range(32))
+    msg = b"verify_strict"
+    sig = pyref.ed25519_sign(seedk, msg)
+    h = hashlib.sha512(seedk).digest()
+    a = (int.from_bytes(h[:32], "little") & ((1 << 254) - 8)) | (1 << 254)
+    pk = pyref.e_compress(pyref.e_mul(pyref.EB, a))
+    out.append(("strict_valid", sig, pk, msg))
+    y = 2
+    while pyref.recover_x(y, 0) is not None:
+        y += 1
+    out.append(("R_undecodable", y.to_bytes(32, "little") + sig[32:], pk, msg))
+    for yy in range(0, 19):
+        b = (yy + EP).to_bytes(32, "little")
+        if pyref.e_decompress(b) is not None:
+            out.append((f"R_noncanon_y{yy}", b + sig[32:], pk, msg))
+    return out
+
+
+def ed_strict_expected(cases):
+    return [pyref.ed25519_verify_strict(sig, pk, m) for (_, sig, pk, m) in cases]

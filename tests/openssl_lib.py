@@ -42,6 +42,9 @@ def crypto():
             ("ECDSA_do_sign", vp, [ctypes.c_char_p, ctypes.c_int, vp]),
             ("ECDSA_SIG_get0", None, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
             ("ECDSA_SIG_free", None, [vp]),
+            ("ECDSA_SIG_new", vp, []),
+            ("ECDSA_SIG_set0", ctypes.c_int, [vp, vp, vp]),
+            ("ECDSA_do_verify", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, vp, vp]),
             ("EVP_PKEY_new_raw_private_key", vp, [ctypes.c_int, vp, ctypes.c_char_p, ctypes.c_size_t]),
             ("EVP_PKEY_new_raw_public_key", vp, [ctypes.c_int, vp, ctypes.c_char_p, ctypes.c_size_t]),
             ("EVP_PKEY_get_raw_public_key", ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]),
@@ -128,6 +131,27 @@ def ecdsa_sign(curve: int, sk: int, digest: bytes):
     C.BN_free(k)
     C.EC_KEY_free(key)
     return int.from_bytes(rb.raw, "big"), int.from_bytes(sb.raw, "big")
+
+
+def ecdsa_verify(curve: int, pub: bytes, digest: bytes, r: int, s: int):
+    """OpenSSL ECDSA_do_verify of (r, s) on digest under pub = X||Y: True / False, or None when
+    OpenSSL refuses the key itself (not a curve point)"""
+    C = crypto()
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    pt = C.EC_POINT_new(grp)
+    try:
+        if C.EC_POINT_oct2point(grp, pt, b"\x04" + pub, 65, None) != 1 or C.EC_KEY_set_public_key(key, pt) != 1:
+            return None
+        sig = C.ECDSA_SIG_new()
+        rb, sb = crypto().BN_bin2bn(r.to_bytes(32, "big"), 32, None), crypto().BN_bin2bn(s.to_bytes(32, "big"), 32, None)
+        assert C.ECDSA_SIG_set0(sig, rb, sb) == 1  # the signature object now owns rb, sb
+        rc = C.ECDSA_do_verify(digest, len(digest), sig, key)
+        C.ECDSA_SIG_free(sig)
+        return rc == 1
+    finally:
+        C.EC_POINT_free(pt)
+        C.EC_KEY_free(key)
 
 
 def ed25519_pubkey(seed: bytes) -> bytes:

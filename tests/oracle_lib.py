@@ -45,6 +45,11 @@ def lib():
         L.or_ed25519_decompress.argtypes = [c_p, c_p]
         L.or_secp_recover_batch.argtypes = [ctypes.c_int, vp, vp, ctypes.c_size_t, vp, vp, ctypes.c_int]
         L.or_ed25519_verify_batch.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_int]
+        L.or_ed25519_verify_strict_batch.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_int]
+        L.or_secp_verify.argtypes = [ctypes.c_int, c_p, c_p, c_p]
+        L.or_secp_verify_batch.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_int]
+        L.or_ed25519_verify_strict.argtypes = [c_p, c_p, c_p, ctypes.c_size_t]
+        L.or_ed25519_is_small_order.argtypes = [c_p]
         L.or_gen_secp.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint32,
                                   vp, vp, vp, vp, ctypes.c_int]
         L.or_gen_ed25519.argtypes = [ctypes.c_uint64, ctypes.c_size_t, vp, ctypes.c_uint32,
@@ -90,6 +95,19 @@ def secp_sign(curve: int, sk: bytes, digest: bytes, nonce: bytes):
 
 def ed25519_verify(sig: bytes, pk: bytes, msg: bytes) -> bool:
     return bool(lib().or_ed25519_verify(sig, pk, msg, len(msg)))
+
+
+def ed25519_verify_strict(sig: bytes, pk: bytes, msg: bytes) -> bool:
+    return bool(lib().or_ed25519_verify_strict(sig, pk, msg, len(msg)))
+
+
+def ed25519_is_small_order(enc: bytes) -> bool:
+    return bool(lib().or_ed25519_is_small_order(enc))
+
+
+def secp_verify(curve: int, sig: bytes, digest: bytes, pub: bytes) -> int:
+    """status of fuel Signature::verify(pub, digest): 0 valid, else a SIGOPS_ST_* code"""
+    return lib().or_secp_verify(curve, sig, digest, pub)
 
 
 def ed25519_pubkey(seed: bytes) -> bytes:
@@ -151,6 +169,24 @@ def verify_batch(sigs, pks, msgs, msg_off, nthreads=0):
     lib().or_ed25519_verify_batch(_ptr(np.ascontiguousarray(sigs)), _ptr(np.ascontiguousarray(pks)), _ptr(msgs),
                                   _ptr(np.ascontiguousarray(msg_off, dtype=np.uint64)), n, _ptr(valid), nthreads)
     return valid
+
+
+def verify_strict_batch(sigs, pks, msgs, msg_off, nthreads=0):
+    n = sigs.shape[0]
+    valid = np.zeros(n, np.uint8)
+    msgs = np.ascontiguousarray(msgs) if msgs.size else np.zeros(1, np.uint8)
+    lib().or_ed25519_verify_strict_batch(_ptr(np.ascontiguousarray(sigs)), _ptr(np.ascontiguousarray(pks)),
+                                         _ptr(msgs), _ptr(np.ascontiguousarray(msg_off, dtype=np.uint64)), n,
+                                         _ptr(valid), nthreads)
+    return valid
+
+
+def secp_verify_batch(curve, sigs, digests, pubs, nthreads=0):
+    n = sigs.shape[0]
+    st = np.zeros(n, np.uint8)
+    lib().or_secp_verify_batch(curve, _ptr(np.ascontiguousarray(sigs)), _ptr(np.ascontiguousarray(digests)),
+                               _ptr(np.ascontiguousarray(pubs)), n, _ptr(st), nthreads)
+    return st
 
 
 def gen_secp(curve, seed, n, corrupt_ppm=0, nthreads=0):

@@ -81,6 +81,32 @@ def secp_recover(curve, sig: bytes, digest: bytes):
     return 0, Q[0].to_bytes(32, "big") + Q[1].to_bytes(32, "big")
 
 
+def secp_verify(curve, sig: bytes, digest: bytes, pub: bytes) -> int:
+    """fuel Signature::verify(pub, digest): libsecp256k1 rules for secp256k1 (lower-S only),
+    p256 / ecdsa 0.16 rules for secp256r1; returns 0 or the SIGOPS_ST_* rejection code."""
+    c = CURVES[curve]
+    p, n = c["p"], c["n"]
+    r = int.from_bytes(sig[:32], "big")
+    sb = bytearray(sig[32:])
+    sb[0] &= 0x7F
+    s = int.from_bytes(bytes(sb), "big")
+    if not 0 < r < n:
+        return 1
+    if not 0 < s < n:
+        return 2
+    if curve == 0 and s > (n - 1) // 2:
+        return 5
+    x, y = int.from_bytes(pub[:32], "big"), int.from_bytes(pub[32:], "big")
+    if x >= p or y >= p or (y * y - (x**3 + c["a"] * x + c["b"])) % p != 0:
+        return 6
+    w = pow(s, -1, n)
+    z = int.from_bytes(digest, "big") % n
+    P = w_add(c, w_mul(c, c["G"], z * w % n), w_mul(c, (x, y), r * w % n))
+    if P is None:
+        return 4
+    return 0 if P[0] % n == r else 7
+
+
 # ---- Ed25519, RFC 8032 section 6 style (affine) ----
 EP = 2**255 - 19
 EL = 2**252 + 27742317777372353535851937790883648493
@@ -151,6 +177,35 @@ def ed25519_verify(sig: bytes, pk: bytes, msg: bytes) -> bool:
     negA = ((EP - A[0]) % EP, A[1])
     P = e_add(e_mul(EB, s), e_mul(negA, k))
     return e_compress(P) == sig[:32]
+
+
+def e_small_order(P) -> bool:
+    """order divides 8: three doublings reach the identity (0, 1)"""
+    for _ in range(3):
+        P = e_add(P, P)
+    return P == (0, 1)
+
+
+def ed25519_verify_strict(sig: bytes, pk: bytes, msg: bytes) -> bool:
+    """dalek 2.1 VerifyingKey::verify_strict: verify, plus R must decompress and neither R nor A
+    may have small order; the final comparison is on the compressed bytes of R"""
+    R = e_decompress(sig[:32])
+    A = e_decompress(pk)
+    if R is None or A is None or e_small_order(R) or e_small_order(A):
+        return False
+    return ed25519_verify(sig, pk, msg)
+
+
+def ed25519_torsion():
+    """the eight points of order dividing 8, as [l]P for a point P with a torsion component"""
+    pts = set()
+    y = 2
+    while len(pts) < 8:
+        x = recover_x(y, 0)
+        if x is not None:
+            pts.add(e_mul((x, y), EL))
+        y += 1
+    return sorted(pts)
 
 
 def ed25519_sign(seed: bytes, msg: bytes) -> bytes:

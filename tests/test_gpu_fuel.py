@@ -51,6 +51,29 @@ def test_message_new_dev_unaligned_offsets(ctx):
         assert got[i].tobytes() == hashlib.sha256(data[off[i]:off[i + 1]].tobytes()).digest(), i
 
 
+def test_message_new_queue_many_messages_twice(ctx):
+    """far more messages than resident lanes, a heavy-tailed length mix (empty, 64-byte and
+    log-uniform up to 8 KiB), then a second, smaller call: every lane takes many messages from the
+    work counter, which each call clears"""
+    rng = np.random.default_rng(24)
+    n = 300000
+    kind = rng.random(n)
+    lens = np.where(kind < 0.6, 0, np.where(kind < 0.85, 64, np.exp(rng.uniform(0, np.log(8192), n)).astype(np.int64)))
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    data = rng.integers(0, 256, int(off[-1]) + 16, dtype=np.uint8)
+    d_data = torch.from_numpy(data).cuda()
+    d_off = torch.from_numpy(off).cuda()
+    want = [hashlib.sha256(data[off[i]:off[i + 1]].tobytes()).digest() for i in range(n)]
+    for m in (n, 12345):
+        _say(f"message_new_dev queue n={m}")
+        d_out = torch.zeros((m, 32), dtype=torch.uint8, device="cuda")
+        ctx.message_new_dev(d_data.data_ptr(), d_off.data_ptr(), m, d_out.data_ptr())
+        torch.cuda.synchronize()
+        got = d_out.cpu().numpy()
+        bad = [i for i in range(m) if got[i].tobytes() != want[i]]
+        assert not bad, (m, bad[:5])
+
+
 @pytest.mark.parametrize("curve", [0, 1])
 def test_recover_address_matches_hash_of_recovered_key(ctx, curve):
     n = 3000

@@ -624,6 +624,18 @@ DEV u32 words_sub(const u32 a[8], const u32 b[8], u32 r[8]) {
   return br;
 }
 
+// r = a + b, returns the carry
+DEV u32 words_add(const u32 a[8], const u32 b[8], u32 r[8]) {
+  u32 c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u64 x = (u64)a[i] + b[i] + c;
+    r[i] = (u32)x;
+    c = (u32)(x >> 32);
+  }
+  return c;
+}
+
 // ---- exponentiation by a fixed exponent along an addition chain ----
 // CH (generated into consts.hpp) lists steps t[DST] = t[SRC]^(2^NSQ) * t[MUL] (MUL < 0: no
 // multiplication) with t[0] = x on entry. The steps are expanded at compile time, so every slot

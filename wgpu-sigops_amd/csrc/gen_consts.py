@@ -380,6 +380,20 @@ def sha256_constants():
     return K, IV
 
 
+def sha256_pad_kw(bitlen):
+    """K[t] + W[t] (mod 2^32) for the padding-only block of a message whose length is a multiple
+    of 64 bytes (0x80, zeros, 64-bit bit length): its message schedule is a constant."""
+    K, _ = sha256_constants()
+    M32 = 2**32 - 1
+    rotr = lambda x, n: ((x >> n) | (x << (32 - n))) & M32
+    W = [0x80000000] + [0] * 13 + [bitlen >> 32, bitlen & M32]
+    for t in range(16, 64):
+        s0 = rotr(W[t - 15], 7) ^ rotr(W[t - 15], 18) ^ (W[t - 15] >> 3)
+        s1 = rotr(W[t - 2], 17) ^ rotr(W[t - 2], 19) ^ (W[t - 2] >> 10)
+        W.append((W[t - 16] + s0 + W[t - 7] + s1) & M32)
+    return [(k + w) & M32 for k, w in zip(K, W)]
+
+
 def main():
     o = sys.stdout
     o.write("// Generated by gen_consts.py -- do not edit.\n#pragma once\n#include <cstdint>\n\n")
@@ -399,6 +413,14 @@ def main():
     o.write(arr("R1_N32", [N_R1 >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
     o.write(arr("ED_L32", [L_ED >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
     o.write(arr("ED_P32", [P_ED >> (32 * i) & 0xFFFFFFFF for i in range(8)]))
+    # verification with a known key: p (key range), p - n (x(P) in [n, p) compares as x - n) and
+    # (n - 1) / 2 (secp256k1 accepts lower-S signatures only)
+    w8 = lambda v: [v >> (32 * i) & 0xFFFFFFFF for i in range(8)]
+    o.write(arr("K1_P32", w8(P_K1)))
+    o.write(arr("R1_P32", w8(P_R1)))
+    o.write(arr("K1_PMN32", w8(P_K1 - N_K1)))
+    o.write(arr("R1_PMN32", w8(P_R1 - N_R1)))
+    o.write(arr("K1_NHALF32", w8((N_K1 - 1) // 2)))
     # 2^512 mod l and 2^256 mod l in Montgomery-of-l form for the wide hash reduction:
     #   h = hi*2^256 + lo ->  mont(hi, 2^(2*261+256-261) ...) : we need x*2^256 mod l = mont_mul(x, 2^(256+261) mod l)
     o.write(arr("L_C256", limbs(2**(256 + RBITS) % L_ED)))
@@ -434,6 +456,7 @@ def main():
     K2, IV2 = sha256_constants()
     o.write(arr("SHA256_K", K2))
     o.write(arr("SHA256_IV", IV2))
+    o.write(arr("SHA256_PAD64_KW", sha256_pad_kw(512)))  # second block of every 64-byte message
     o.write("\n// host-side generator tables: entries of 20 u32 (x[9], y[9], pad[2]) for secp,\n")
     o.write("// 28 u32 (ypx[9], ymx[9], t2d[9], pad) for ed25519; Montgomery form\n")
     ntab = 1 << (WG - 1)

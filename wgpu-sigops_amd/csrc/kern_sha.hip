@@ -5,17 +5,45 @@
 
 namespace sigops {
 
-// digest i = SHA-256(msgs[off[i] .. off[i+1])); msgs 4-byte aligned with slack after the last byte
+// digest i = SHA-256(msgs[off[i] .. off[i+1])); msgs 4-byte aligned with slack after the last byte.
+// Messages differ in length, so no lane owns one message for the whole launch: each loop trip
+// compresses one block per lane, and a lane whose message is done takes the next index from the
+// queue counter (the lanes of a wave that need one in the same trip take consecutive indices with
+// one atomic). A wave stays busy until the queue runs dry instead of idling behind its longest
+// message. queue[0] must be 0 at launch; every lane leaves once an index >= n comes out.
 static __global__ __launch_bounds__(256) void k_message_new(const u32* __restrict__ msgs,
                                                            const u64* __restrict__ off, u32 n,
-                                                           uint4* __restrict__ out) {
+                                                           uint4* __restrict__ out, u32* __restrict__ queue) {
   const MemDev mem{msgs};
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const u64 o = off[i];
-    u32 d[8];
-    sha256_msg(mem, o, (u32)(off[i + 1] - o), d);
-    out[2 * (u64)i] = make_uint4(d[0], d[1], d[2], d[3]);
-    out[2 * (u64)i + 1] = make_uint4(d[4], d[5], d[6], d[7]);
+  u32 i = 0, b = 0, nblk = 0, len = 0;
+  u64 o = 0;
+  u32 h[8];
+  for (;;) {
+    const bool need = b == nblk;
+    const u64 want = __ballot(need);
+    if (want != 0) {
+      const u32 leader = (u32)__ffsll((unsigned long long)want) - 1;
+      u32 first = 0;
+      if (__lane_id() == leader) first = atomicAdd(queue, (u32)__popcll(want));
+      first = __builtin_amdgcn_readlane(first, leader);
+      if (need) {
+        i = first + __builtin_amdgcn_mbcnt_hi((u32)(want >> 32), __builtin_amdgcn_mbcnt_lo((u32)want, 0u));
+        if (i >= n) break;
+        o = off[i];
+        len = (u32)(off[i + 1] - o);
+        nblk = sha256_nblocks(len);
+        b = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) h[k] = SHA256_IV[k];
+      }
+    }
+    u32 w[16];
+    sha256_block_words(mem, o, len, b, nblk, w);
+    sha256_compress(h, w);
+    if (++b == nblk) {
+      out[2 * (u64)i] = make_uint4(bswap32(h[0]), bswap32(h[1]), bswap32(h[2]), bswap32(h[3]));
+      out[2 * (u64)i + 1] = make_uint4(bswap32(h[4]), bswap32(h[5]), bswap32(h[6]), bswap32(h[7]));
+    }
   }
 }
 
@@ -38,9 +66,11 @@ static u32 grid_for(u32 n, u32 grid) {
   return want < grid ? want : grid;
 }
 
-hipError_t launch_message_new(const void* msgs, const u64* off, u32 n, void* out, u32 grid, hipStream_t s) {
+hipError_t launch_message_new(const void* msgs, const u64* off, u32 n, void* out, u32* queue, u32 grid,
+                              hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_message_new<<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out);
+  k_fill_u32<<<1, 64, 0, s>>>(queue, 0u, 1u);
+  k_message_new<<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out, queue);
   return hipGetLastError();
 }
 

@@ -19,6 +19,14 @@ hipError_t launch_secp_recover_CurveR1(const void* sigs, const void* digs, void*
                                       const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
                                       uint32_t* pend_count, uint32_t grid, hipStream_t s);
 
+// verification with a known public key (64 bytes X || Y per item); status only
+hipError_t launch_secp_verify_CurveK1(const void* sigs, const void* digs, const void* pubs, void* status, uint32_t n,
+                                     const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
+                                     uint32_t* pend_count, uint32_t grid, hipStream_t s);
+hipError_t launch_secp_verify_CurveR1(const void* sigs, const void* digs, const void* pubs, void* status, uint32_t n,
+                                     const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
+                                     uint32_t* pend_count, uint32_t grid, hipStream_t s);
+
 hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* msgs, const uint64_t* msg_off,
                                  void* status, uint32_t n, const void* btab, void* ctab, uint32_t slots, void* kbuf,
                                  uint32_t grid, hipStream_t s);
@@ -55,8 +63,9 @@ struct BatchLaunch {
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
 
 // SHA-256: digests of messages (Fuel Message::new) and addresses of 64-byte public keys
-hipError_t launch_message_new(const void* msgs, const uint64_t* off, uint32_t n, void* out, uint32_t grid,
-                              hipStream_t s);
+// queue: one device word the message kernel uses as its work counter (cleared by the launch)
+hipError_t launch_message_new(const void* msgs, const uint64_t* off, uint32_t n, void* out, uint32_t* queue,
+                              uint32_t grid, hipStream_t s);
 hipError_t launch_fuel_address(const void* pubs, const void* status, uint32_t n, void* out, uint32_t grid,
                                hipStream_t s);
 

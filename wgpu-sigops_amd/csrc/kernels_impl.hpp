@@ -203,6 +203,69 @@ __global__ __launch_bounds__(64) void k_secp_careful(const uint4* __restrict__ s
   }
 }
 
+// ECDSA verification with a known key (fuel Signature::verify): the recovery kernel with the key
+// in place of R and the projective x comparison in place of the affine output. Lanes that meet an
+// exceptional addition are finished by k_secp_verify_careful.
+template <class C>
+__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_verify(
+    const uint4* __restrict__ sigs, const uint4* __restrict__ digs, const uint4* __restrict__ pubs,
+    u8* __restrict__ status, u32 n, const uint4* __restrict__ gtab, uint4* __restrict__ rtab, u32 slots,
+    u32* __restrict__ pend_list, u32* __restrict__ pend_count) {
+  __shared__ uint4 sg[GTAB_ENTRIES * 5];
+  for (int t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) sg[t] = gtab[t];
+  __syncthreads();
+  const GTabLds gt{sg};
+  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+  JTabDev tab;
+  tab.base = rtab;
+  tab.slots = slots;
+  tab.slot = slot;
+  const u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = slot; i < n; i += stride) {
+    u32 sr[16], dr[8], pr[16];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(digs + 2 * (u64)i, dr, 2);
+    load_words(pubs + 4 * (u64)i, pr, 4);
+    secp_pro pro;
+    secp_prologue_verify<C>(sr, dr, pr, pro);
+    u32 st = pro.status;
+    if (st == ST_OK) {
+      bool exc = false;
+      const jac P = secp_ladder<C>(pro, tab, gt, exc);
+      if (exc) {
+        st = ST_PENDING;
+        pend_list[atomicAdd(pend_count, 1u)] = i;
+      } else {
+        load_words(sigs + 4 * (u64)i, sr, 4);
+        st = secp_verify_x<C>(P, sr);
+      }
+    }
+    status[i] = (u8)st;
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(64) void k_secp_verify_careful(const uint4* __restrict__ sigs,
+                                                            const uint4* __restrict__ digs,
+                                                            const uint4* __restrict__ pubs, u8* __restrict__ status,
+                                                            const uint4* __restrict__ gtab,
+                                                            const u32* __restrict__ pend_list,
+                                                            const u32* __restrict__ pend_count) {
+  const u32 cnt = *pend_count;
+  const GTabGlobal gt{gtab};
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const u32 i = pend_list[j];
+    u32 sr[16], dr[8], pr[16];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(digs + 2 * (u64)i, dr, 2);
+    load_words(pubs + 4 * (u64)i, pr, 4);
+    secp_pro pro;
+    secp_prologue_verify<C>(sr, dr, pr, pro);
+    const jacf P = secp_ladder_careful<C>(pro, gt);
+    status[i] = (u8)(P.inf ? ST_INFINITY : secp_verify_x<C>(P.P, sr));
+  }
+}
+
 // k = SHA-512(R || A || M) mod l per item; a pass of its own so that the verifier's register
 // budget carries no hash state (the reference also hashes in a separate stage,
 // src/wgsl/main/ed25519_eddsa_main_1.wgsl). `order` (may be NULL) lists the items to hash.

@@ -26,7 +26,10 @@ enum : u32 {
   ST_BAD_R = 1,
   ST_BAD_S = 2,
   ST_NO_POINT = 3,
-  ST_INFINITY = 4,
+  ST_INFINITY = 4,    // recovery: Q = infinity; verification: u1 G + u2 Q = infinity
+  ST_HIGH_S = 5,      // verification, secp256k1: s > (n-1)/2
+  ST_BAD_KEY = 6,     // verification: the key is not a curve point with X, Y < p
+  ST_MISMATCH = 7,    // verification: x(u1 G + u2 Q) mod n != r
   ST_PENDING = 0xFF,  // exceptional addition met: the careful path finishes the item
 };
 
@@ -34,9 +37,13 @@ struct CurveK1 {
   typedef ModPk1 Fp;
   typedef ModNk1 Fn;
   static constexpr bool A_IS_M3 = false;
-  static constexpr bool GLV = true;  // phi(x, y) = (beta x, y) = [lambda](x, y)
+  static constexpr bool GLV = true;    // phi(x, y) = (beta x, y) = [lambda](x, y)
+  static constexpr bool LOW_S = true;  // verification accepts s <= (n-1)/2 only (libsecp256k1)
   DEV static fe b() { return fe_from_const(K1_B); }
   DEV static const u32 (&n32())[8] { return K1_N32; }
+  DEV static const u32 (&p32())[8] { return K1_P32; }
+  DEV static const u32 (&pmn32())[8] { return K1_PMN32; }
+  DEV static const u32 (&nhalf32())[8] { return K1_NHALF32; }
   typedef CH_K1_SQRT SqrtChain;  // x^((p+1)/4)
 };
 struct CurveR1 {
@@ -44,8 +51,12 @@ struct CurveR1 {
   typedef ModNr1 Fn;
   static constexpr bool A_IS_M3 = true;
   static constexpr bool GLV = false;
+  static constexpr bool LOW_S = false;  // p256 / ecdsa 0.16 verification accepts either s
   DEV static fe b() { return fe_from_const(R1_B); }
   DEV static const u32 (&n32())[8] { return R1_N32; }
+  DEV static const u32 (&p32())[8] { return R1_P32; }
+  DEV static const u32 (&pmn32())[8] { return R1_PMN32; }
+  DEV static const u32 (&nhalf32())[8] { return R1_N32; }  // unused (LOW_S is false)
   typedef CH_R1_SQRT SqrtChain;
 };
 
@@ -297,6 +308,50 @@ struct secp_pro {
   u32 u1[8], u2[8];  // plain u1, u2 (for the careful path)
 };
 
+// Q = u1 G + u2 P ladder set-up shared by recovery (P = R) and verification (P = the key):
+// plain u1, u2 for the careful path, and the digit carriers of the fast ladder (GLV halves for
+// secp256k1, odd representatives otherwise). u1m, u2m in Montgomery form of n; P affine (x, y).
+template <class C>
+DEV void secp_scalars(secp_pro& o, const fe& u1m, const fe& u2m, const fe& x, const fe& y) {
+  typedef typename C::Fp Fp;
+  typedef typename C::Fn Fn;
+  const u32(&n)[8] = C::n32();
+  fe_to_words(from_mont<Fn>(u1m), o.u1);
+  fe_to_words(from_mont<Fn>(u2m), o.u2);
+  o.rx = x;
+  if constexpr (C::GLV) {
+    u32 h1[5], h2[5];
+    glv_split(o.u1, h1, h2, o.sg1, o.sg2);
+    glv_carrier<GLV_BG>(h1, o.cg1);
+    glv_carrier<GLV_BG>(h2, o.cg2);
+    glv_split(o.u2, h1, h2, o.sr1, o.sr2);
+    glv_carrier<GLV_BR>(h1, o.cr1);
+    glv_carrier<GLV_BR>(h2, o.cr2);
+    o.ry = y;
+    o.ry_neg = false;
+    o.g_neg = false;
+    return;
+  }
+  // odd representatives
+  o.g_neg = (o.u1[0] & 1u) == 0;
+  const bool r_neg = (o.u2[0] & 1u) == 0;
+  u32 k1[8], k2[8];
+  words_sub(n, o.u1, k1);
+  words_sub(n, o.u2, k2);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    k1[i] = o.g_neg ? k1[i] : o.u1[i];
+    k2[i] = r_neg ? k2[i] : o.u2[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    o.c1[i] = (k1[i] >> 1) | (i < 7 ? k1[i + 1] << 31 : 0x80000000u);
+    o.c2[i] = (k2[i] >> 1) | (i < 7 ? k2[i + 1] << 31 : 0x80000000u);
+  }
+  o.ry = r_neg ? fneg<Fp>(y) : y;
+  o.ry_neg = r_neg;
+}
+
 template <class C>
 DEV void secp_prologue(const u32 sig_raw[16], const u32 dig_raw[8], secp_pro& o) {
   typedef typename C::Fp Fp;
@@ -333,40 +388,68 @@ DEV void secp_prologue(const u32 sig_raw[16], const u32 dig_raw[8], secp_pro& o)
   const fe rinv = finv<Fn>(rn);
   const fe u1m = fneg<Fn>(mont_mul<Fn>(to_mont<Fn>(fe_from_words(z)), rinv));
   const fe u2m = mont_mul<Fn>(to_mont<Fn>(fe_from_words(s)), rinv);
-  fe_to_words(from_mont<Fn>(u1m), o.u1);
-  fe_to_words(from_mont<Fn>(u2m), o.u2);
-  o.rx = x;
-  if constexpr (C::GLV) {
-    u32 h1[5], h2[5];
-    glv_split(o.u1, h1, h2, o.sg1, o.sg2);
-    glv_carrier<GLV_BG>(h1, o.cg1);
-    glv_carrier<GLV_BG>(h2, o.cg2);
-    glv_split(o.u2, h1, h2, o.sr1, o.sr2);
-    glv_carrier<GLV_BR>(h1, o.cr1);
-    glv_carrier<GLV_BR>(h2, o.cr2);
-    o.ry = y;
-    o.ry_neg = false;
-    o.g_neg = false;
-    return;
-  }
-  // odd representatives
-  o.g_neg = (o.u1[0] & 1u) == 0;
-  const bool r_neg = (o.u2[0] & 1u) == 0;
-  u32 k1[8], k2[8];
-  words_sub(n, o.u1, k1);
-  words_sub(n, o.u2, k2);
+  secp_scalars<C>(o, u1m, u2m, x, y);
+}
+
+// ---- verification with a known public key (fuel Signature::verify) ----
+// Rules of the reference's CPU stack (fuel-crypto 0.49 over libsecp256k1 for secp256k1, p256 /
+// ecdsa 0.16 verify_prehash for secp256r1): r, s in [1, n-1] after clearing the recid bit (which
+// verification ignores); secp256k1 also s <= (n-1)/2; the key X || Y with X, Y < p on the curve;
+// z = digest mod n, u1 = z/s, u2 = r/s, P = u1 G + u2 Q != infinity, accept iff x(P) mod n == r.
+// pub_raw: the 64 key bytes as 16 little-endian u32 loads. The same ladders then run with the key
+// in place of R.
+template <class C>
+DEV void secp_prologue_verify(const u32 sig_raw[16], const u32 dig_raw[8], const u32 pub_raw[16], secp_pro& o) {
+  typedef typename C::Fp Fp;
+  typedef typename C::Fn Fn;
+  const u32(&n)[8] = C::n32();
+  u32 r[8], s[8], z[8], sraw[8], xw[8], yw[8];
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    k1[i] = o.g_neg ? k1[i] : o.u1[i];
-    k2[i] = r_neg ? k2[i] : o.u2[i];
-  }
+  for (int i = 0; i < 8; i++) sraw[i] = sig_raw[8 + i];
+  sraw[0] &= ~0x80u;
+  words_from_be_bytes(sig_raw, r);
+  words_from_be_bytes(sraw, s);
+  words_from_be_bytes(dig_raw, z);
+  words_from_be_bytes(pub_raw, xw);
+  words_from_be_bytes(pub_raw + 8, yw);
+  o.status = ST_OK;
+  if (words_is_zero(r) || !words_lt(r, n)) o.status = ST_BAD_R;
+  else if (words_is_zero(s) || !words_lt(s, n)) o.status = ST_BAD_S;
+  else if (C::LOW_S && words_lt(C::nhalf32(), s)) o.status = ST_HIGH_S;
+  else if (!words_lt(xw, C::p32()) || !words_lt(yw, C::p32())) o.status = ST_BAD_KEY;
+  if (!words_lt(z, n)) {
+    u32 t[8];
+    words_sub(z, n, t);
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    o.c1[i] = (k1[i] >> 1) | (i < 7 ? k1[i + 1] << 31 : 0x80000000u);
-    o.c2[i] = (k2[i] >> 1) | (i < 7 ? k2[i + 1] << 31 : 0x80000000u);
+    for (int i = 0; i < 8; i++) z[i] = t[i];
   }
-  o.ry = r_neg ? fneg<Fp>(y) : y;
-  o.ry_neg = r_neg;
+  const fe x = to_mont<Fp>(fe_from_words(xw));
+  const fe y = to_mont<Fp>(fe_from_words(yw));
+  fe rhs = mont_mul<Fp>(mont_sqr<Fp>(x), x);
+  if (C::A_IS_M3) rhs = fsub<Fp>(rhs, fadd<Fp>(fdbl<Fp>(x), x));
+  rhs = fadd<Fp>(rhs, C::b());
+  if (o.status == ST_OK && !feq<Fp>(mont_sqr<Fp>(y), rhs)) o.status = ST_BAD_KEY;
+  // u1 = z/s, u2 = r/s (mod n)
+  const fe sinv = finv<Fn>(to_mont<Fn>(fe_from_words(s)));
+  const fe u1m = mont_mul<Fn>(to_mont<Fn>(fe_from_words(z)), sinv);
+  const fe u2m = mont_mul<Fn>(to_mont<Fn>(fe_from_words(r)), sinv);
+  secp_scalars<C>(o, u1m, u2m, x, y);
+}
+
+// the verifier's final check on P = (X : Y : Z) (Jacobian) without an inversion: x(P) < p < 2n,
+// so x(P) mod n == r means X == r Z^2, or X == (r + n) Z^2 when r + n < p. sig_raw: the signature
+// as loaded (r is decoded again here rather than kept live through the ladder).
+template <class C>
+DEV u32 secp_verify_x(const jac& P, const u32 sig_raw[16]) {
+  typedef typename C::Fp F;
+  u32 r[8], rn[8];
+  words_from_be_bytes(sig_raw, r);
+  words_add(r, C::n32(), rn);  // meaningful only when r < p - n
+  const fe z2 = mont_sqr<F>(P.Z);
+  const bool eq1 = feq<F>(P.X, mont_mul<F>(to_mont<F>(fe_from_words(r)), z2));
+  const bool eq2 = feq<F>(P.X, mont_mul<F>(to_mont<F>(fe_from_words(rn)), z2));
+  if (fis_zero<F>(P.Z)) return ST_INFINITY;
+  return (eq1 || (eq2 && words_lt(r, C::pmn32()))) ? ST_OK : ST_MISMATCH;
 }
 
 DEV u32 nibble(const u32 c[8], int i) { return (c[i >> 3] >> ((i & 7) * 4)) & 15u; }

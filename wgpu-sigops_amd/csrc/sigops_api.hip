@@ -47,6 +47,7 @@ struct sigops_ctx {
   uint32_t* d_work = nullptr;
   uint32_t* d_order = nullptr;
   uint32_t* d_n = nullptr;
+  uint32_t* d_hq = nullptr;  // work counter of the message-hashing kernel
   std::map<std::tuple<const void*, const void*, const void*, const void*, const void*, void*, void*>,
            hipGraphExec_t>
       graphs;
@@ -147,6 +148,52 @@ int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8
   return SIGOPS_OK;
 }
 
+int secp_verify_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_digs, const void* d_pubs, size_t n,
+                    void* d_status, hipStream_t s) {
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (n == 0) return SIGOPS_OK;
+  if (!d_sigs || !d_digs || !d_pubs || !d_status) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (curve == 0)
+    HIPCHK(c, sigops::launch_secp_verify_CurveK1(d_sigs, d_digs, d_pubs, d_status, (uint32_t)n, c->d_gtab_k1,
+                                                c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+  else
+    HIPCHK(c, sigops::launch_secp_verify_CurveR1(d_sigs, d_digs, d_pubs, d_status, (uint32_t)n, c->d_gtab_r1,
+                                                c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+  return SIGOPS_OK;
+}
+
+int secp_verify_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8_t* digs, const uint8_t* pubs,
+                     size_t n, uint8_t* valid, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  if (n == 0) return SIGOPS_OK;
+  if (!sigs || !digs || !pubs || !valid) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  int rc;
+  if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) || (rc = ensure(c, c->out, n * 64)) ||
+      (rc = ensure(c, c->st, n)))
+    return rc;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->aux.p, digs, n * 32, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->out.p, pubs, n * 64, hipMemcpyHostToDevice, s));
+  k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = secp_verify_dev(c, curve, c->in.p, c->aux.p, c->out.p, n, c->st.p, s))) return rc;
+  uint8_t* st = status;
+  if (!st) {
+    c->host_status.resize(n);
+    st = c->host_status.data();
+  }
+  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; i++) {
+    if (st[i] > SIGOPS_ST_MISMATCH) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
+    valid[i] = st[i] == SIGOPS_ST_OK;
+  }
+  return SIGOPS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -210,6 +257,7 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
       hipSuccess)
     return bad(e, "hipMalloc batch order");
   if ((e = hipMalloc((void**)&c->d_n, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc n");
+  if ((e = hipMalloc((void**)&c->d_hq, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc hash queue");
   if ((e = hipMalloc(&c->d_kbuf, (size_t)c->max_batch * 32)) != hipSuccess) return bad(e, "hipMalloc challenges");
   return c;
 }
@@ -223,7 +271,7 @@ void sigops_destroy(sigops_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
                   c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
-                  (void*)c->d_order, (void*)c->d_n, c->d_kbuf})
+                  (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, c->d_kbuf})
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -251,6 +299,28 @@ int sigops_secp256r1_recover_dev(sigops_ctx* c, const void* d_sigs, const void* 
                                  void* d_out_pub, void* d_status, void* stream) {
   if (!c) return SIGOPS_E_ARG;
   return secp_recover_dev(c, 1, d_sigs, d_digests, n, d_out_pub, d_status, (hipStream_t)stream);
+}
+
+int sigops_secp256k1_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubkeys,
+                            size_t n, uint8_t* valid, uint8_t* status) {
+  return secp_verify_host(c, 0, sigs, digests, pubkeys, n, valid, status);
+}
+
+int sigops_secp256r1_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubkeys,
+                            size_t n, uint8_t* valid, uint8_t* status) {
+  return secp_verify_host(c, 1, sigs, digests, pubkeys, n, valid, status);
+}
+
+int sigops_secp256k1_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_digests, const void* d_pubkeys,
+                                size_t n, void* d_status, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  return secp_verify_dev(c, 0, d_sigs, d_digests, d_pubkeys, n, d_status, (hipStream_t)stream);
+}
+
+int sigops_secp256r1_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_digests, const void* d_pubkeys,
+                                size_t n, void* d_status, void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  return secp_verify_dev(c, 1, d_sigs, d_digests, d_pubkeys, n, d_status, (hipStream_t)stream);
 }
 
 int sigops_ed25519_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
@@ -407,10 +477,11 @@ int sigops_message_new_dev(sigops_ctx* c, const void* d_data, const uint64_t* d_
                            void* stream) {
   if (!c) return SIGOPS_E_ARG;
   if (n == 0) return SIGOPS_OK;
-  if (n > 0xFFFFFFFFull) return fail(c, SIGOPS_E_ARG, "n too large");
+  if (n > 0x7FFFFFFFull) return fail(c, SIGOPS_E_ARG, "n too large");
   if (!d_data || !d_off || !d_digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (((uintptr_t)d_data & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_data must be 4-byte aligned");
-  HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, c->grid, (hipStream_t)stream));
+  HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, c->d_hq, 2 * c->grid,
+                                       (hipStream_t)stream));
   return SIGOPS_OK;
 }
 
@@ -435,7 +506,8 @@ int sigops_message_new(sigops_ctx* c, const uint8_t* data, const uint64_t* off, 
   if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, data + base, total, hipMemcpyHostToDevice, s));
   k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
   HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(c, sigops::launch_message_new(c->msg.p, (const uint64_t*)c->off.p, (uint32_t)n, c->aux.p, c->grid, s));
+  HIPCHK(c, sigops::launch_message_new(c->msg.p, (const uint64_t*)c->off.p, (uint32_t)n, c->aux.p, c->d_hq,
+                                       2 * c->grid, s));
   HIPCHK(c, hipMemcpyAsync(digests, c->aux.p, n * 32, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   return SIGOPS_OK;
