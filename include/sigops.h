@@ -47,6 +47,9 @@ extern "C" {
 #define SIGOPS_ED_BAD_S 1     /* s >= l (non-canonical scalar) */
 #define SIGOPS_ED_BAD_KEY 2   /* public key does not decompress */
 #define SIGOPS_ED_MISMATCH 3  /* [s]B - [k]A does not compress to R */
+/* verify_strict only */
+#define SIGOPS_ED_BAD_R 4       /* R does not decompress */
+#define SIGOPS_ED_SMALL_ORDER 5 /* R or A has small order ([8]P is the identity) */
 
 /* schemes of the mixed batch */
 #define SIGOPS_SCHEME_SECP256K1 0
@@ -96,6 +99,14 @@ int sigops_secp256r1_recover(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t
 int sigops_ed25519_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
                           const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status);
 
+/* Ed25519 verification with ed25519-dalek 2.1 `VerifyingKey::verify_strict` rules: `verify`, plus
+   R must decompress and neither R nor A may have small order (the adjacent Fuel predicate the
+   reference names, src/curve_algos/ed25519_eddsa.rs:197-198). Same layout as sigops_ed25519_verify;
+   status adds SIGOPS_ED_BAD_R and SIGOPS_ED_SMALL_ORDER. No counterpart among the reference's
+   entry points. */
+int sigops_ed25519_verify_strict(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                                 const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status);
+
 /* ECDSA verification with a known public key, fuel `Signature::verify(&public_key, &message)`
    (the predicate next to recovery; fuel-crypto 0.49 verifies secp256k1 with libsecp256k1, which
    accepts lower-S signatures only; secp256r1 follows p256 0.13 / ecdsa 0.16 `verify_prehash`).
@@ -123,6 +134,8 @@ int sigops_secp256r1_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void*
 /* d_msgs must stay readable for 16 bytes past its last message byte */
 int sigops_ed25519_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
                               const uint64_t* d_msg_off, size_t n, void* d_status, void* stream);
+int sigops_ed25519_verify_strict_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
+                                     const uint64_t* d_msg_off, size_t n, void* d_status, void* stream);
 
 /* ---- mixed-scheme batch: one call for secp256k1 / secp256r1 recoveries and Ed25519 verifications
    in any order. No single counterpart in the reference (one entry point per scheme, 32-byte

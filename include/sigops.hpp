@@ -7,6 +7,8 @@
 //   ed25519_eddsa::ecverify(signatures, messages, verifying_keys, table_limbs, log_limb_size)
 //   ed25519_eddsa::ecverify_single(signatures, messages, verifying_keys, log_limb_size)
 //                                                                                src/ed25519_eddsa.rs:67-73, :259-264
+// and, next to the path: secp256k1_ecdsa::verify / secp256r1_ecdsa::verify (known public key) and
+// ed25519_eddsa::ecverify_strict (dalek verify_strict)
 // Rust's Result<_, ShaderFailureError> (src/lib.rs:12-14) becomes a thrown sigops::ShaderFailureError;
 // the reference's panics on bad arguments (length mismatch, src/secp256k1_ecdsa.rs:21) become
 // std::invalid_argument. The table arguments are checked for shape only: the library keeps its own
@@ -84,9 +86,35 @@ std::vector<std::vector<uint8_t>> recover(int curve, const std::vector<Sig>& sig
   for (size_t i = 0; i < n; i++) res[i].assign(out.begin() + i * 64, out.begin() + i * 64 + 64);
   return res;
 }
+// fuel Signature::verify(&public_key, &message) per item (public keys: 64 bytes X || Y)
+template <class Sig>
+std::vector<bool> verify(int curve, const std::vector<Sig>& signatures, const std::vector<Message>& messages,
+                         const std::vector<Bytes64>& public_keys, Context& ctx) {
+  if (signatures.size() != messages.size() || signatures.size() != public_keys.size())
+    throw std::invalid_argument("signatures, messages and public keys differ in length");
+  const size_t n = signatures.size();
+  if (n > (size_t(1) << 30)) throw std::invalid_argument("at most 2^30 signatures");
+  if (n == 0) return {};
+  std::vector<uint8_t> sigs(n * 64), digs(n * 32), pubs(n * 64), valid(n);
+  for (size_t i = 0; i < n; i++) {
+    memcpy(&sigs[i * 64], signatures[i].data(), 64);
+    memcpy(&digs[i * 32], messages[i].data(), 32);
+    memcpy(&pubs[i * 64], public_keys[i].data(), 64);
+  }
+  ctx.check(curve == 0 ? sigops_secp256k1_verify(ctx.get(), sigs.data(), digs.data(), pubs.data(), n, valid.data(), nullptr)
+                       : sigops_secp256r1_verify(ctx.get(), sigs.data(), digs.data(), pubs.data(), n, valid.data(), nullptr));
+  std::vector<bool> res(n);
+  for (size_t i = 0; i < n; i++) res[i] = valid[i] == 1;
+  return res;
+}
 }  // namespace detail
 
 namespace secp256k1_ecdsa {
+// verification with a known key (libsecp256k1 rules: lower-S only); no counterpart in the reference's API
+inline std::vector<bool> verify(const std::vector<Signature>& signatures, const std::vector<Message>& messages,
+                                const std::vector<Bytes64>& public_keys, Context& ctx = default_context()) {
+  return detail::verify(0, signatures, messages, public_keys, ctx);
+}
 inline std::vector<std::vector<uint8_t>> ecrecover(const std::vector<Signature>& signatures,
                                                    const std::vector<Message>& messages,
                                                    const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
@@ -109,6 +137,11 @@ inline std::vector<std::vector<uint8_t>> ecrecover_single(const std::vector<Sign
 }  // namespace secp256k1_ecdsa
 
 namespace secp256r1_ecdsa {
+// verification with a known key (p256 / ecdsa 0.16 rules); no counterpart in the reference's API
+inline std::vector<bool> verify(const std::vector<Bytes64>& signatures, const std::vector<Message>& messages,
+                                const std::vector<Bytes64>& public_keys, Context& ctx = default_context()) {
+  return detail::verify(1, signatures, messages, public_keys, ctx);
+}
 inline std::vector<std::vector<uint8_t>> ecrecover(const std::vector<Bytes64>& signatures,
                                                    const std::vector<Message>& messages,
                                                    const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
@@ -168,12 +201,10 @@ inline std::vector<bool> ecverify_single(const std::vector<Ed25519Signature>& si
   detail::check_log_limb_size(log_limb_size);
   return detail_ed::verify(signatures, messages, verifying_keys, ctx);
 }
-// variable-length extension: message i is messages[i] (any length), hashed as given; the
-// reference's Message is fixed at 32 bytes (src/ed25519_eddsa.rs:39)
-inline std::vector<bool> ecverify_bytes(const std::vector<Ed25519Signature>& signatures,
-                                        const std::vector<std::vector<uint8_t>>& messages,
-                                        const std::vector<VerifyingKey>& verifying_keys,
-                                        Context& ctx = default_context()) {
+namespace detail_ed {
+inline std::vector<bool> verify_bytes(const std::vector<Ed25519Signature>& signatures,
+                                      const std::vector<std::vector<uint8_t>>& messages,
+                                      const std::vector<VerifyingKey>& verifying_keys, Context& ctx, bool strict) {
   if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
   const size_t n = signatures.size();
   if (n == 0) return {};
@@ -187,11 +218,28 @@ inline std::vector<bool> ecverify_bytes(const std::vector<Ed25519Signature>& sig
     msgs.insert(msgs.end(), messages[i].begin(), messages[i].end());
   }
   off[n] = msgs.size();
-  ctx.check(sigops_ed25519_verify(ctx.get(), sigs.data(), pks.data(), msgs.empty() ? nullptr : msgs.data(), off.data(),
-                                  n, valid.data(), nullptr));
+  ctx.check((strict ? sigops_ed25519_verify_strict : sigops_ed25519_verify)(
+      ctx.get(), sigs.data(), pks.data(), msgs.empty() ? nullptr : msgs.data(), off.data(), n, valid.data(), nullptr));
   std::vector<bool> res(n);
   for (size_t i = 0; i < n; i++) res[i] = valid[i] == 1;
   return res;
+}
+}  // namespace detail_ed
+
+// variable-length extension: message i is messages[i] (any length), hashed as given; the
+// reference's Message is fixed at 32 bytes (src/ed25519_eddsa.rs:39)
+inline std::vector<bool> ecverify_bytes(const std::vector<Ed25519Signature>& signatures,
+                                        const std::vector<std::vector<uint8_t>>& messages,
+                                        const std::vector<VerifyingKey>& verifying_keys,
+                                        Context& ctx = default_context()) {
+  return detail_ed::verify_bytes(signatures, messages, verifying_keys, ctx, false);
+}
+// ed25519-dalek verify_strict rules (src/curve_algos/ed25519_eddsa.rs:197-198 names it)
+inline std::vector<bool> ecverify_strict(const std::vector<Ed25519Signature>& signatures,
+                                         const std::vector<std::vector<uint8_t>>& messages,
+                                         const std::vector<VerifyingKey>& verifying_keys,
+                                         Context& ctx = default_context()) {
+  return detail_ed::verify_bytes(signatures, messages, verifying_keys, ctx, true);
 }
 }  // namespace ed25519_eddsa
 

@@ -2,6 +2,8 @@
 // Reads vectors written by tests/test_gpu_mirror.py, one per line:
 //   k1|r1 <sig hex> <digest hex> <expected pub hex>
 //   ed <sig hex> <pubkey hex> <message hex (32 bytes)> <0|1>
+//   k1v|r1v <sig hex> <digest hex> <pub hex> <0|1>      (verification with a known key)
+//   eds <sig hex> <pubkey hex> <message hex or -> <0|1> (verify_strict, any message length)
 // and checks every result; also checks the error behaviour (length mismatch, bad limb size).
 #include <cstdio>
 #include <fstream>
@@ -33,6 +35,14 @@ int main(int argc, char** argv) {
   std::vector<sigops::Ed25519Signature> eds;
   std::vector<sigops::VerifyingKey> edk;
   std::vector<bool> ede;
+  std::vector<sigops::Signature> kvs[2];
+  std::vector<sigops::Message> kvd[2];
+  std::vector<sigops::Bytes64> kvp[2];
+  std::vector<bool> kve[2];
+  std::vector<sigops::Ed25519Signature> sts;
+  std::vector<sigops::VerifyingKey> stk;
+  std::vector<std::vector<uint8_t>> stm;
+  std::vector<bool> ste;
   std::string line;
   while (std::getline(in, line)) {
     std::istringstream ss(line);
@@ -41,6 +51,16 @@ int main(int argc, char** argv) {
     if (kind == "k1") { k1s.push_back(hex<64>(a)); k1d.push_back(hex<32>(b)); k1e.push_back(hexv(c)); }
     if (kind == "r1") { r1s.push_back(hex<64>(a)); r1d.push_back(hex<32>(b)); r1e.push_back(hexv(c)); }
     if (kind == "ed") { ss >> d; eds.push_back(hex<64>(a)); edk.push_back(hex<32>(b)); edm.push_back(hex<32>(c)); ede.push_back(d == "1"); }
+    if (kind == "k1v" || kind == "r1v") {
+      const int cv = kind == "r1v";
+      ss >> d;
+      kvs[cv].push_back(hex<64>(a)); kvd[cv].push_back(hex<32>(b)); kvp[cv].push_back(hex<64>(c)); kve[cv].push_back(d == "1");
+    }
+    if (kind == "eds") {
+      ss >> d;
+      sts.push_back(hex<64>(a)); stk.push_back(hex<32>(b)); stm.push_back(c == "-" ? std::vector<uint8_t>() : hexv(c));
+      ste.push_back(d == "1");
+    }
   }
   // the tables as the reference builds them: 16 points x {x,y} (x {x,y,t}) x 20 limbs at 13 bits
   std::vector<uint32_t> t2(16 * 2 * 20), t3(16 * 3 * 20);
@@ -59,13 +79,22 @@ int main(int argc, char** argv) {
   std::vector<std::vector<uint8_t>> edv;
   for (auto& m : edm) edv.emplace_back(m.begin(), m.end());
   bad += sigops::ed25519_eddsa::ecverify_bytes(eds, edv, edk) != v;
+  // verification with known keys and verify_strict
+  size_t nv = 0;
+  bad += sigops::secp256k1_ecdsa::verify(kvs[0], kvd[0], kvp[0]) != kve[0];
+  bad += sigops::secp256r1_ecdsa::verify(kvs[1], kvd[1], kvp[1]) != kve[1];
+  nv += kve[0].size() + kve[1].size();
+  bad += sigops::ed25519_eddsa::ecverify_strict(sts, stm, stk) != ste;
+  bool verify_mismatch = false;
+  try { sigops::secp256k1_ecdsa::verify(kvs[0], kvd[0], {}); } catch (const std::invalid_argument&) { verify_mismatch = true; }
+  bad += !verify_mismatch;
   bad += !sigops::secp256k1_ecdsa::ecrecover({}, {}, t2, 13).empty();
   int thrown = 0;
   try { sigops::secp256k1_ecdsa::ecrecover(k1s, {}, t2, 13); } catch (const std::invalid_argument&) { thrown++; }
   try { sigops::secp256k1_ecdsa::ecrecover_single_shader(k1s, k1d, 9); } catch (const std::invalid_argument&) { thrown++; }
   try { sigops::secp256r1_ecdsa::ecrecover(r1s, r1d, std::vector<uint32_t>(3), 13); } catch (const std::invalid_argument&) { thrown++; }
   bad += thrown != 3;
-  std::cout << "mirror: k1 " << r.size() << " r1 " << q.size() << " ed " << v.size() << " mismatches " << bad
-            << std::endl;
+  std::cout << "mirror: k1 " << r.size() << " r1 " << q.size() << " ed " << v.size() << " secp verify " << nv
+            << " ed strict " << ste.size() << " mismatches " << bad << std::endl;
   return bad ? 1 : 0;
 }

@@ -88,9 +88,35 @@ int recover_one(const unsigned char* sig, const unsigned char* dig, unsigned cha
   memcpy(out, o, 64);
   return exc ? 0x100 : 0;  // flag that the careful path ran (status OK)
 }
+template <class C>
+int verify_one(const unsigned char* sig, const unsigned char* dig, const unsigned char* pub, const u32* gtab,
+               int force_careful) {
+  u32 sr[16], dr[8], pr[16];
+  memcpy(sr, sig, 64);
+  memcpy(dr, dig, 32);
+  memcpy(pr, pub, 64);
+  secp_pro pro;
+  secp_prologue_verify<C>(sr, dr, pr, pro);
+  GTabHost<typename C::Fp> gt{gtab};
+  if (pro.status != ST_OK) return (int)pro.status;
+  bool exc = false;
+  JTabHost tab;
+  jac P = secp_ladder<C>(pro, tab, gt, exc);
+  if (!exc && !force_careful) return (int)secp_verify_x<C>(P, sr);
+  jacf Pc = secp_ladder_careful<C>(pro, gt);
+  const int st = Pc.inf ? (int)ST_INFINITY : (int)secp_verify_x<C>(Pc.P, sr);
+  return st | (exc ? 0x100 : 0);
+}
 }  // namespace
 
 extern "C" {
+// verification with a known key: status; bit 8 set when the fast ladder met an exceptional addition
+int dc_secp_verify(int curve, const unsigned char* sig, const unsigned char* dig, const unsigned char* pub,
+                   int force_careful) {
+  if (curve == 0) return verify_one<CurveK1>(sig, dig, pub, K1_GTABLE, force_careful);
+  return verify_one<CurveR1>(sig, dig, pub, R1_GTABLE, force_careful);
+}
+
 // returns status; bit 8 set when the fast ladder met an exceptional addition
 int dc_secp_recover(int curve, const unsigned char* sig, const unsigned char* dig, unsigned char* out,
                     int force_careful) {
@@ -98,7 +124,8 @@ int dc_secp_recover(int curve, const unsigned char* sig, const unsigned char* di
   return recover_one<CurveR1>(sig, dig, out, R1_GTABLE, force_careful);
 }
 
-int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const unsigned char* msg, unsigned len) {
+int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const unsigned char* msg, unsigned len,
+                      int strict) {
   u32 sr[16], pr[8];
   memcpy(sr, sig, 64);
   memcpy(pr, pk, 32);
@@ -108,7 +135,8 @@ int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const u
   MemHost mem{buf};
   CTabHost tab;
   BTabHost bt;
-  u32 st = ed25519_verify_item(sr, pr, mem, 0, len, tab, bt);
+  u32 st = strict ? ed25519_verify_item<true>(sr, pr, mem, 0, len, tab, bt)
+                  : ed25519_verify_item<false>(sr, pr, mem, 0, len, tab, bt);
   delete[] buf;
   return (int)st;
 }

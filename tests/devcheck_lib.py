@@ -27,13 +27,14 @@ def lib():
         L = ctypes.CDLL(build())
         c_p = ctypes.c_char_p
         L.dc_secp_recover.argtypes = [ctypes.c_int, c_p, c_p, c_p, ctypes.c_int]
-        L.dc_ed25519_verify.argtypes = [c_p, c_p, c_p, ctypes.c_uint]
+        L.dc_ed25519_verify.argtypes = [c_p, c_p, c_p, ctypes.c_uint, ctypes.c_int]
         L.dc_field.argtypes = [ctypes.c_int, ctypes.c_int, c_p, c_p, c_p]
         L.dc_sha512_ed.argtypes = [c_p, c_p, ctypes.c_uint, c_p]
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.dc_mont_raw.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p]
         L.dc_glv_split.argtypes = [u32p, u32p, u32p]
         L.dc_sha256.argtypes = [c_p, ctypes.c_uint, ctypes.c_uint, c_p]
+        L.dc_secp_verify.argtypes = [ctypes.c_int, c_p, c_p, c_p, ctypes.c_int]
         L.dc_sha256_64.argtypes = [c_p, c_p]
         _lib = L
     return _lib
@@ -45,8 +46,8 @@ def secp_recover(curve, sig, dig, force_careful=False):
     return st, out.raw
 
 
-def ed25519_verify(sig, pk, msg):
-    return lib().dc_ed25519_verify(sig, pk, msg, len(msg))
+def ed25519_verify(sig, pk, msg, strict=False):
+    return lib().dc_ed25519_verify(sig, pk, msg, len(msg), int(strict))
 
 
 def field(op, mod, a, b=0):
@@ -79,6 +80,12 @@ def glv_split(k):
     h1 = sum(v << (32 * i) for i, v in enumerate(H1))
     h2 = sum(v << (32 * i) for i, v in enumerate(H2))
     return (-h1 if sg & 1 else h1), (-h2 if sg & 2 else h2)
+
+
+def secp_verify(curve, sig, dig, pub, force_careful=False):
+    """(status, careful path ran) of the device verification built for the host"""
+    st = lib().dc_secp_verify(curve, bytes(sig), bytes(dig), bytes(pub), int(force_careful))
+    return st & 0xFF, bool(st >> 8)
 
 
 def sha256(msg, at=0):

@@ -115,6 +115,26 @@ def test_sha256_message_new_and_address_forms():
         assert D.sha256_64(pk) == hashlib.sha256(pk).digest()
 
 
+@pytest.mark.parametrize("curve", [0, 1])
+def test_verify_random_and_edges(curve):
+    """device verification (host build, bound checker on) against the oracle: random signatures
+    with corruption, every edge case, and the careful path forced on ordinary inputs"""
+    sigs, dig, pubs, cor = O.gen_secp(curve, 12, 48, corrupt_ppm=300000)
+    for i in range(48):
+        st, _ = D.secp_verify(curve, sigs[i], dig[i], pubs[i])
+        assert st == O.secp_verify(curve, bytes(sigs[i]), bytes(dig[i]), bytes(pubs[i]))
+    hits = 0
+    for (name, sig, d, pub), exp in zip(edge_cases.secp_verify_cases(curve),
+                                        edge_cases.secp_verify_expected(curve, edge_cases.secp_verify_cases(curve))):
+        st, careful = D.secp_verify(curve, sig, d, pub)
+        hits += careful
+        assert st == exp, name
+    assert hits > 0  # the careful path was exercised
+    for i in range(6):
+        st, _ = D.secp_verify(curve, sigs[i], dig[i], pubs[i], force_careful=True)
+        assert st == O.secp_verify(curve, bytes(sigs[i]), bytes(dig[i]), bytes(pubs[i]))
+
+
 def test_sha512_preimage_stream():
     rnd = random.Random(2)
     for L in list(range(0, 260)) + [1000, 4096]:
@@ -152,3 +172,16 @@ def test_ed25519_random_and_edges():
     for i in range(48):
         m = bytes(msgs[off[i]:off[i + 1]])
         assert (D.ed25519_verify(bytes(sigs[i]), bytes(pks[i]), m) == 0) == (not cor[i])
+
+
+def test_ed25519_strict_cases():
+    """device verify_strict (host build) against the oracle's restatement on the strict edge cases:
+    small-order keys and R, undecodable and non-canonical R, and the ordinary cases"""
+    cases = edge_cases.ed_strict_cases()
+    codes = set()
+    for (name, sig, pk, m), exp in zip(cases, edge_cases.ed_strict_expected(cases)):
+        st = D.ed25519_verify(sig, pk, m, strict=True)
+        codes.add(st)
+        assert (st == 0) == exp == O.ed25519_verify_strict(sig, pk, m), name
+        assert (D.ed25519_verify(sig, pk, m) == 0) == O.ed25519_verify(sig, pk, m), name
+    assert {0, 1, 2, 3, 4, 5} <= codes

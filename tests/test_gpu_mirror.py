@@ -1,7 +1,8 @@
 """The C++ mirror of the reference API (include/sigops.hpp) run on the GPU through the C ABI:
 the reference's secp256k1 known-answer vector, generated signatures of both curves (valid and
 corrupted, expected values from the oracle) and Ed25519 with 32-byte messages as the reference
-passes them (RFC-style signing of SHA-256 digests), plus the mirror's error behaviour."""
+passes them (RFC-style signing of SHA-256 digests), plus the mirror's error behaviour; and the
+predicates next to the path: secp verification with known keys and Ed25519 verify_strict."""
 import hashlib
 import json
 import os
@@ -26,6 +27,13 @@ def test_cpp_mirror(tmp_path):
         out, st = O.recover_batch(curve, sigs, dig)
         for i in range(300):
             lines.append(f"{tag} {bytes(sigs[i]).hex()} {bytes(dig[i]).hex()} {bytes(out[i]).hex()}")
+        # verification with the generator's keys: oracle verdicts (corrupted items mostly reject)
+        vst = O.secp_verify_batch(curve, sigs, dig, pubs)
+        for i in range(300):
+            lines.append(f"{tag}v {bytes(sigs[i]).hex()} {bytes(dig[i]).hex()} {bytes(pubs[i]).hex()} {int(vst[i] == 0)}")
+    import edge_cases
+    for name, sig, pk, m in edge_cases.ed_strict_cases():
+        lines.append(f"eds {sig.hex()} {pk.hex()} {m.hex() or '-'} {int(O.ed25519_verify_strict(sig, pk, m))}")
     for i in range(60):
         seed = hashlib.sha256(b"seed%d" % i).digest()
         msg = hashlib.sha256(b"message%d" % i).digest()  # fuel Message::new: SHA-256 of the data

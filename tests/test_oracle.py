@@ -225,3 +225,99 @@ def test_ed25519_generator():
         m = bytes(msgs[off[i]:off[i + 1]])
         assert bool(valid[i]) == pyref.ed25519_verify(bytes(sigs[i]), bytes(pks[i]), m)
         assert bool(valid[i]) == (not cor[i])
+
+
+# ---- verification with a known key (fuel Signature::verify) and dalek verify_strict ----
+@pytest.mark.parametrize("curve", [0, 1])
+def test_secp_verify_edge_cases_vs_closed_form(curve):
+    cases = edge_cases.secp_verify_cases(curve)
+    exp = edge_cases.secp_verify_expected(curve, cases)
+    for (name, sig, dig, pub), e in zip(cases, exp):
+        assert O.secp_verify(curve, sig, dig, pub) == e, name
+    want = {0, 1, 2, 4, 6, 7} | ({5} if curve == 0 else set())
+    assert set(exp) == want
+    got = dict(zip((c[0] for c in cases), exp))
+    assert got["x_ge_n"] == 0 and got["valid_recid_bit_ignored"] == 0
+    assert got["s_half"] == 0 and got["s_half_plus_1"] == (5 if curve == 0 else 0)
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_secp_verify_vs_openssl(curve):
+    """OpenSSL ECDSA_do_verify (p256 rules: either s) agrees with the oracle, except that
+    secp256k1 rejects s > (n-1)/2 with its own status"""
+    c = pyref.CURVES[curve]
+    n, half = c["n"], (c["n"] - 1) // 2
+    rnd = random.Random(40 + curve)
+    for _ in range(10):
+        sk = rnd.randrange(1, n)
+        d = bytes(rnd.randrange(256) for _ in range(32))
+        r, s = S.ecdsa_sign(curve, sk, d)
+        pub = S.pubkey(curve, sk)
+        for ss in (s, n - s):
+            if ss >= 2**255:
+                continue  # not representable in fuel's format (bit 255 holds the recid)
+            sig = r.to_bytes(32, "big") + ss.to_bytes(32, "big")
+            assert S.ecdsa_verify(curve, pub, d, r, ss) is True
+            assert O.secp_verify(curve, sig, d, pub) == (5 if curve == 0 and ss > half else 0)
+            d2 = bytes([d[0] ^ 1]) + d[1:]
+            assert S.ecdsa_verify(curve, pub, d2, r, ss) is False
+            assert O.secp_verify(curve, sig, d2, pub) in ((5,) if curve == 0 and ss > half else (7,))
+        other = S.pubkey(curve, rnd.randrange(1, n))
+        sig = r.to_bytes(32, "big") + min(s, n - s).to_bytes(32, "big")
+        assert S.ecdsa_verify(curve, other, d, r, min(s, n - s)) is False
+        assert O.secp_verify(curve, sig, d, other) == 7
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_secp_verify_agrees_with_recovery(curve):
+    """for lower-S signatures, verify(sig, z, Q) accepts exactly when one of the two recovery
+    parities recovers Q (verification ignores the recid bit)"""
+    sigs, dig, pubs, cor = O.gen_secp(curve, 77, 64, corrupt_ppm=400000)
+    st = O.secp_verify_batch(curve, sigs, dig, pubs, nthreads=2)
+    n = pyref.CURVES[curve]["n"]
+    for i in range(64):
+        sig = bytearray(sigs[i].tobytes())
+        s = int.from_bytes(bytes([sig[32] & 0x7F]) + bytes(sig[33:]), "big")
+        if curve == 0 and s > (n - 1) // 2:
+            assert st[i] == 5
+            continue
+        rec = set()
+        for odd in (0, 1):
+            sig[32] = (sig[32] & 0x7F) | (odd << 7)
+            rs, q = O.secp_recover(curve, bytes(sig), dig[i].tobytes())
+            if rs == 0:
+                rec.add(q)
+        assert (st[i] == 0) == (pubs[i].tobytes() in rec), i
+        assert int(st[i]) == pyref.secp_verify(curve, sigs[i].tobytes(), dig[i].tobytes(), pubs[i].tobytes())
+
+
+def test_ed25519_small_order_points():
+    tors = pyref.ed25519_torsion()
+    assert len(tors) == 8 and (0, 1) in tors and (0, pyref.EP - 1) in tors
+    orders = sorted(next(k for k in (1, 2, 4, 8) if pyref.e_mul(T, k) == (0, 1)) for T in tors)
+    assert orders == [1, 2, 4, 4, 8, 8, 8, 8]  # the cyclic group of order 8
+    for T in tors:
+        assert O.ed25519_is_small_order(pyref.e_compress(T))
+    assert not O.ed25519_is_small_order(pyref.e_compress(pyref.EB))
+    mixed = pyref.e_add(pyref.e_mul(pyref.EB, 12345), tors[-1])
+    assert not O.ed25519_is_small_order(pyref.e_compress(mixed))
+    # an encoding that does not decompress is not a small-order point
+    y = next(y for y in range(2, 100) if pyref.recover_x(y, 0) is None)
+    assert not O.ed25519_is_small_order(y.to_bytes(32, "little"))
+
+
+def test_ed25519_verify_strict_cases():
+    for t in json.load(open(os.path.join(HERE, "golden", "rfc8032_ed25519.json")))["vectors"]:
+        pk, m, sig = (bytes.fromhex(t[k]) for k in ("public", "message", "signature"))
+        assert O.ed25519_verify_strict(sig, pk, m)
+    cases = edge_cases.ed_strict_cases()
+    exp = edge_cases.ed_strict_expected(cases)
+    loose = 0
+    for (name, sig, pk, m), e in zip(cases, exp):
+        assert O.ed25519_verify_strict(sig, pk, m) == e, name
+        plain = O.ed25519_verify(sig, pk, m)
+        assert not (e and not plain), name  # strict never accepts what verify rejects
+        loose += plain and not e
+    names = [c[0] for c, e in zip(cases, exp)]
+    assert sum(n.startswith("R_small_order") for n in names) >= 3
+    assert loose >= 8  # small-order keys and small-order R that plain verify accepts

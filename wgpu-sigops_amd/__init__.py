@@ -77,9 +77,15 @@ def lib():
             for name in ("sigops_secp256k1_recover", "sigops_secp256r1_recover"):
                 getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
             L.sigops_ed25519_verify.argtypes = [vp, c_p, c_p, c_p, c_p, sz, c_p, c_p]
+            for name in ("sigops_secp256k1_verify", "sigops_secp256r1_verify"):
+                getattr(L, name).argtypes = [vp, c_p, c_p, c_p, sz, c_p, c_p]
+            for name in ("sigops_secp256k1_verify_dev", "sigops_secp256r1_verify_dev"):
+                getattr(L, name).argtypes = [vp, vp, vp, vp, sz, vp, vp]
             for name in ("sigops_secp256k1_recover_dev", "sigops_secp256r1_recover_dev"):
                 getattr(L, name).argtypes = [vp, vp, vp, sz, vp, vp, vp]
             L.sigops_ed25519_verify_dev.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp]
+            L.sigops_ed25519_verify_strict.argtypes = [vp, c_p, c_p, c_p, c_p, sz, c_p, c_p]
+            L.sigops_ed25519_verify_strict_dev.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp]
             L.sigops_batch.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, sz, c_p, c_p]
             L.sigops_batch_dev.argtypes = [vp, vp, vp, vp, vp, vp, sz, vp, vp, vp]
             L.sigops_message_new.argtypes = [vp, c_p, c_p, sz, c_p]
@@ -199,8 +205,37 @@ class Context:
     def secp256r1_recover(self, sigs, digests):
         return self._secp(lib().sigops_secp256r1_recover, sigs, digests)
 
-    def ed25519_verify(self, sigs, pubkeys, msgs, msg_off):
-        """(n,64) R||s, (n,32) keys, concatenated messages + (n+1) offsets -> ((n,) bool, (n,) status)"""
+    def _secp_verify(self, fn, sigs, digests, pubkeys):
+        sigs = _u8(sigs, 64, "sigs")
+        digests = _u8(digests, 32, "digests")
+        pubkeys = _u8(pubkeys, 64, "pubkeys")
+        n = sigs.shape[0]
+        if digests.shape[0] != n or pubkeys.shape[0] != n:
+            raise ValueError("sigs, digests and pubkeys differ in length")
+        valid = np.zeros(n, np.uint8)
+        st = np.zeros(n, np.uint8)
+        self._check(fn(self._h, _ptr(sigs), _ptr(digests), _ptr(pubkeys), n, _ptr(valid), _ptr(st)))
+        return valid.astype(bool), st
+
+    def secp256k1_verify(self, sigs, digests, pubkeys):
+        """fuel Signature::verify: (n,64) signatures, (n,32) digests, (n,64) X||Y keys
+        -> ((n,) bool, (n,) status)"""
+        return self._secp_verify(lib().sigops_secp256k1_verify, sigs, digests, pubkeys)
+
+    def secp256r1_verify(self, sigs, digests, pubkeys):
+        return self._secp_verify(lib().sigops_secp256r1_verify, sigs, digests, pubkeys)
+
+    def secp256k1_verify_dev(self, d_sigs, d_digests, d_pubkeys, n, d_status, stream=0):
+        self._check(lib().sigops_secp256k1_verify_dev(self._h, d_sigs, d_digests, d_pubkeys, n, d_status,
+                                                      stream or None))
+
+    def secp256r1_verify_dev(self, d_sigs, d_digests, d_pubkeys, n, d_status, stream=0):
+        self._check(lib().sigops_secp256r1_verify_dev(self._h, d_sigs, d_digests, d_pubkeys, n, d_status,
+                                                      stream or None))
+
+    def ed25519_verify(self, sigs, pubkeys, msgs, msg_off, strict=False):
+        """(n,64) R||s, (n,32) keys, concatenated messages + (n+1) offsets -> ((n,) bool, (n,) status);
+        strict: ed25519-dalek verify_strict rules"""
         sigs = _u8(sigs, 64, "sigs")
         pubkeys = _u8(pubkeys, 32, "pubkeys")
         n = sigs.shape[0]
@@ -212,9 +247,17 @@ class Context:
             raise ValueError("msg_off points past the message buffer")
         valid = np.zeros(n, np.uint8)
         st = np.zeros(n, np.uint8)
-        self._check(lib().sigops_ed25519_verify(self._h, _ptr(sigs), _ptr(pubkeys), _ptr(msgs) if msgs.size else None,
-                                                _ptr(msg_off), n, _ptr(valid), _ptr(st)))
+        fn = lib().sigops_ed25519_verify_strict if strict else lib().sigops_ed25519_verify
+        self._check(fn(self._h, _ptr(sigs), _ptr(pubkeys), _ptr(msgs) if msgs.size else None, _ptr(msg_off), n,
+                       _ptr(valid), _ptr(st)))
         return valid.astype(bool), st
+
+    def ed25519_verify_strict(self, sigs, pubkeys, msgs, msg_off):
+        return self.ed25519_verify(sigs, pubkeys, msgs, msg_off, strict=True)
+
+    def ed25519_verify_strict_dev(self, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream=0):
+        self._check(lib().sigops_ed25519_verify_strict_dev(self._h, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status,
+                                                           stream or None))
 
     def batch(self, schemes, sigs, aux, msgs, msg_off):
         """mixed batch: (n,) scheme bytes, (n,64) sigs, (n,32) digest-or-key, messages + (n+1) offsets

@@ -30,3 +30,22 @@ def ecrecover_impl(curve, signatures, messages, ctx):
     fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
     out, st = fn(sigs, digs)
     return [bytes(row) for row in out], st
+
+
+def verify_impl(curve, signatures, messages, public_keys, ctx):
+    """fuel Signature::verify over a batch -> (list of bool, numpy status codes)"""
+    from . import default_context
+
+    if not (len(signatures) == len(messages) == len(public_keys)):
+        raise ValueError("signatures, messages and public keys differ in length")
+    if len(signatures) > 256 * 256 * 256 * 64:
+        raise ValueError("too many signatures")
+    if len(signatures) == 0:
+        return [], np.zeros(0, np.uint8)
+    sigs = _as_bytes_array(signatures, 64, "signature")
+    digs = _as_bytes_array(messages, 32, "message")
+    pubs = _as_bytes_array(public_keys, 64, "public key")
+    ctx = ctx or default_context()
+    fn = ctx.secp256k1_verify if curve == 0 else ctx.secp256r1_verify
+    valid, st = fn(sigs, digs, pubs)
+    return [bool(v) for v in valid], st

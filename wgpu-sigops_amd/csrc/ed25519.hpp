@@ -131,7 +131,45 @@ DEV bool sqrt_ratio_i(const fe& u, const fe& v, fe& r_out) {
   return correct || flipped;
 }
 
-enum : u32 { ED_OK = 0, ED_BAD_S = 1, ED_BAD_KEY = 2, ED_MISMATCH = 3 };
+// ED_BAD_R and ED_SMALL_ORDER come from verify_strict only
+enum : u32 { ED_OK = 0, ED_BAD_S = 1, ED_BAD_KEY = 2, ED_MISMATCH = 3, ED_BAD_R = 4, ED_SMALL_ORDER = 5 };
+
+// dalek CompressedEdwardsY::decompress: y = the 255 low bits taken mod p (non-canonical y is
+// accepted), x = sqrt_ratio_i(y^2 - 1, d y^2 + 1), negated when the sign bit is set (also for
+// x = 0). enc: 8 raw little-endian words. Returns false if there is no such x.
+DEV bool ed_decompress(const u32 enc[8], fe& x, fe& y) {
+  u32 yw[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) yw[i] = enc[i];
+  const bool x_sign = (yw[7] >> 31) != 0;
+  yw[7] &= 0x7fffffffu;
+  if (!words_lt(yw, ED_P32)) {
+    u32 t[8];
+    words_sub(yw, ED_P32, t);
+#pragma unroll
+    for (int i = 0; i < 8; i++) yw[i] = t[i];
+  }
+  y = to_mont<Fe>(fe_from_words(yw));
+  const fe one = fe_from_const(Fe::ONE);
+  const fe yy = mont_sqr<Fe>(y);
+  const fe u = fsub<Fe>(yy, one);
+  const fe v = fadd<Fe>(mont_mul<Fe>(yy, fe_from_const(ED_D)), one);
+  const bool ok = sqrt_ratio_i(u, v, x);
+  if (x_sign) x = fneg<Fe>(x);
+  return ok;
+}
+
+// dalek EdwardsPoint::is_small_order: [8]P is the identity (X = 0 and Y = Z after three doublings)
+DEV bool ed_small_order(const fe& x, const fe& y) {
+  ext P;
+  P.X = x;
+  P.Y = y;
+  P.Z = fe_from_const(Fe::ONE);
+  P.T = fe_zero();  // doubling does not read T
+#pragma unroll 1
+  for (int i = 0; i < 3; i++) P = edbl(P, false);
+  return fis_zero<Fe>(P.X) && feq<Fe>(P.Y, P.Z);
+}
 
 // sig_raw: 16 raw words (R || s); pk_raw: 8 raw words; message in mem at msg_off, len bytes.
 // k = SHA-512(R || A || M) mod l as 8 little-endian words
@@ -152,7 +190,9 @@ DEV void ed25519_challenge(const u32 sig_raw[16], const u32 pk_raw[8], const Mem
 
 // verification given the challenge k (from ed25519_challenge).
 // BTab: load(idx, ypx, ymx, t2d) for (2 idx + 1) B. Tab: per-lane store(e, cached) / load(e).
-template <class Tab, class BTab>
+// STRICT: dalek VerifyingKey::verify_strict, which also requires that R decompresses and that
+// neither R nor A has small order; the final comparison stays on the compressed bytes of R.
+template <bool STRICT = false, class Tab, class BTab>
 DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u32 k[8], Tab& tab, const BTab& bt) {
   u32 status = ED_OK;
   // s (little-endian bytes = little-endian words)
@@ -160,27 +200,17 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
 #pragma unroll
   for (int i = 0; i < 8; i++) s[i] = sig_raw[8 + i];
   if (!words_lt(s, ED_L32)) status = ED_BAD_S;
-  // A: y with bit 255 cleared, reduced mod p
-  u32 yw[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) yw[i] = pk_raw[i];
-  const bool x_sign = (yw[7] >> 31) != 0;
-  yw[7] &= 0x7fffffffu;
-  if (!words_lt(yw, ED_P32)) {
-    u32 t[8];
-    words_sub(yw, ED_P32, t);
-#pragma unroll
-    for (int i = 0; i < 8; i++) yw[i] = t[i];
-  }
-  const fe y = to_mont<Fe>(fe_from_words(yw));
-  const fe one = fe_from_const(Fe::ONE);
-  const fe yy = mont_sqr<Fe>(y);
-  const fe u = fsub<Fe>(yy, one);
-  const fe v = fadd<Fe>(mont_mul<Fe>(yy, fe_from_const(ED_D)), one);
-  fe x;
-  const bool ok = sqrt_ratio_i(u, v, x);
+  fe x, y;
+  const bool ok = ed_decompress(pk_raw, x, y);
   if (!ok && status == ED_OK) status = ED_BAD_KEY;
-  if (x_sign) x = fneg<Fe>(x);
+  const fe one = fe_from_const(Fe::ONE);
+  if constexpr (STRICT) {
+    fe rx, ry;
+    const bool r_ok = ed_decompress(sig_raw, rx, ry);
+    if (!r_ok && status == ED_OK) status = ED_BAD_R;
+    const bool small = ed_small_order(rx, ry) || ed_small_order(x, y);
+    if (small && status == ED_OK) status = ED_SMALL_ORDER;
+  }
   // odd representatives: s even -> l - s with B negated; k even -> k + 1 and a final +A
   const bool b_neg = (s[0] & 1u) == 0;
   const bool k_even = (k[0] & 1u) == 0;
@@ -265,12 +295,12 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
   return status;
 }
 
-template <class Tab, class BTab, class Mem>
+template <bool STRICT = false, class Tab, class BTab, class Mem>
 DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Mem& mem, u64 msg_off, u32 len,
                             Tab& tab, const BTab& bt) {
   u32 k[8];
   ed25519_challenge(sig_raw, pk_raw, mem, msg_off, len, k);
-  return ed25519_verify_core(sig_raw, pk_raw, k, tab, bt);
+  return ed25519_verify_core<STRICT>(sig_raw, pk_raw, k, tab, bt);
 }
 
 }  // namespace sigops

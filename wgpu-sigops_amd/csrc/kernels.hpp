@@ -27,9 +27,10 @@ hipError_t launch_secp_verify_CurveR1(const void* sigs, const void* digs, const 
                                      const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
                                      uint32_t* pend_count, uint32_t grid, hipStream_t s);
 
+// strict: ed25519-dalek verify_strict instead of verify
 hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* msgs, const uint64_t* msg_off,
                                  void* status, uint32_t n, const void* btab, void* ctab, uint32_t slots, void* kbuf,
-                                 uint32_t grid, hipStream_t s);
+                                 uint32_t grid, hipStream_t s, bool strict = false);
 
 // ---- mixed batch ----
 constexpr uint32_t BUCKET_K1 = 0, BUCKET_R1 = 1, BUCKET_ED0 = 2;

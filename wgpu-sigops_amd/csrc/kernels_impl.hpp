@@ -290,7 +290,8 @@ static __global__ __launch_bounds__(256) void k_ed25519_hash(const uint4* __rest
   }
 }
 
-static __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_ed25519_verify(
+template <bool STRICT>
+__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_ed25519_verify(
     const uint4* __restrict__ sigs, const uint4* __restrict__ pks, const uint4* __restrict__ kbuf,
     u8* __restrict__ status, u32 n, const uint4* __restrict__ btab, uint4* __restrict__ ctab, u32 slots) {
   __shared__ uint4 sb[BTAB_ENTRIES * 7];
@@ -308,7 +309,7 @@ static __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_ed25519_ver
     load_words(sigs + 4 * (u64)i, sr, 4);
     load_words(pks + 2 * (u64)i, pr, 2);
     load_words(kbuf + 2 * (u64)i, k, 2);
-    status[i] = (u8)ed25519_verify_core(sr, pr, k, tab, bt);
+    status[i] = (u8)ed25519_verify_core<STRICT>(sr, pr, k, tab, bt);
   }
 }
 

@@ -323,18 +323,29 @@ int sigops_secp256r1_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d
   return secp_verify_dev(c, 1, d_sigs, d_digests, d_pubkeys, n, d_status, (hipStream_t)stream);
 }
 
-int sigops_ed25519_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
-                              const uint64_t* d_msg_off, size_t n, void* d_status, void* stream) {
+// Ed25519 verification, dalek `verify` rules or (strict) `verify_strict` rules
+static int ed_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
+                         const uint64_t* d_msg_off, size_t n, void* d_status, void* stream, bool strict) {
   if (!c) return SIGOPS_E_ARG;
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
   HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_msgs, d_msg_off, d_status, (uint32_t)n, c->d_btab,
-                                          c->d_lane_tab, c->slots, c->d_kbuf, c->grid, (hipStream_t)stream));
+                                          c->d_lane_tab, c->slots, c->d_kbuf, c->grid, (hipStream_t)stream, strict));
   return SIGOPS_OK;
 }
 
-int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
-                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status) {
+int sigops_ed25519_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
+                              const uint64_t* d_msg_off, size_t n, void* d_status, void* stream) {
+  return ed_verify_dev(c, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream, false);
+}
+
+int sigops_ed25519_verify_strict_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
+                                     const uint64_t* d_msg_off, size_t n, void* d_status, void* stream) {
+  return ed_verify_dev(c, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream, true);
+}
+
+static int ed_verify_host(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status, bool strict) {
   if (!c) return SIGOPS_E_ARG;
   c->err.clear();
   if (n == 0) return SIGOPS_OK;
@@ -360,7 +371,7 @@ int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
   HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
-  if ((rc = sigops_ed25519_verify_dev(c, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->st.p, s)))
+  if ((rc = ed_verify_dev(c, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->st.p, s, strict)))
     return rc;
   uint8_t* st = status;
   if (!st) {
@@ -369,11 +380,22 @@ int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
   }
   HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  const uint8_t last = strict ? SIGOPS_ED_SMALL_ORDER : SIGOPS_ED_MISMATCH;
   for (size_t i = 0; i < n; i++) {
-    if (st[i] > SIGOPS_ED_MISMATCH) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
+    if (st[i] > last) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
     valid[i] = st[i] == SIGOPS_ED_OK;
   }
   return SIGOPS_OK;
+}
+
+int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status) {
+  return ed_verify_host(c, sigs, pubkeys, msgs, msg_off, n, valid, status, false);
+}
+
+int sigops_ed25519_verify_strict(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                                 const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status) {
+  return ed_verify_host(c, sigs, pubkeys, msgs, msg_off, n, valid, status, true);
 }
 
 int sigops_gen_secp_dev(sigops_ctx* c, int curve, uint64_t seed, size_t n, uint32_t corrupt_ppm, void* d_sigs,

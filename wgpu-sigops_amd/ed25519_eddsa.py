@@ -5,7 +5,9 @@
 signatures, 32-byte messages (fuel ``Message`` digests, which is what the reference hashes,
 src/ed25519_eddsa.rs:39) and 32-byte verifying keys, and return a list of bools under
 ed25519-dalek ``verify`` rules. ``ecverify_bytes`` is the variable-length extension (messages of
-any length). Errors as in the reference: ShaderFailureError when results are missing, [] for no
+any length). ``ecverify_strict`` applies dalek ``verify_strict`` rules instead (R must decompress,
+neither R nor the key may have small order), the adjacent predicate the reference names
+(src/curve_algos/ed25519_eddsa.rs:197-198). Errors as in the reference: ShaderFailureError when results are missing, [] for no
 input, IndexError when fewer keys than signatures are given (the reference indexes, :38).
 """
 import numpy as np
@@ -14,7 +16,7 @@ from . import precompute
 from ._secp_common import _as_bytes_array
 
 
-def _verify(signatures, messages, verifying_keys, ctx, fixed_len):
+def _verify(signatures, messages, verifying_keys, ctx, fixed_len, strict=False):
     from . import default_context
 
     if len(signatures) != len(messages):
@@ -36,7 +38,7 @@ def _verify(signatures, messages, verifying_keys, ctx, fixed_len):
         off[1:] = np.cumsum([len(p) for p in parts])
         msgs = np.frombuffer(b"".join(parts), dtype=np.uint8)
     ctx = ctx or default_context()
-    valid, _ = ctx.ed25519_verify(sigs, pks, msgs, off)
+    valid, _ = (ctx.ed25519_verify_strict if strict else ctx.ed25519_verify)(sigs, pks, msgs, off)
     return [bool(v) for v in valid]
 
 
@@ -53,3 +55,8 @@ def ecverify_single(signatures, messages, verifying_keys, log_limb_size, ctx=Non
 def ecverify_bytes(signatures, messages, verifying_keys, ctx=None):
     """Variable-length messages (each message is any bytes object)."""
     return _verify(signatures, messages, verifying_keys, ctx, False)
+
+
+def ecverify_strict(signatures, messages, verifying_keys, ctx=None):
+    """ed25519-dalek ``verify_strict`` for each item (messages of any length): a list of bools."""
+    return _verify(signatures, messages, verifying_keys, ctx, False, strict=True)

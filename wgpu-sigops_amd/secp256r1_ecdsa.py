@@ -11,7 +11,7 @@ bytes (the reference has no per-item error channel; ``recover_with_status`` expo
 import numpy as np
 
 from . import precompute
-from ._secp_common import ecrecover_impl
+from ._secp_common import ecrecover_impl, verify_impl
 
 CURVE = 1
 
@@ -34,4 +34,17 @@ def recover_with_status(signatures, messages, ctx=None):
     return ecrecover_impl(CURVE, signatures, messages, ctx)
 
 
-__all__ = ["ecrecover", "ecrecover_single_shader", "ecrecover_single", "recover_with_status", "np"]
+def verify(signatures, messages, public_keys, ctx=None):
+    """fuel ``Signature::verify(&public_key, &message)`` for each item: a list of bools.
+    Rules of p256 0.13 / ecdsa 0.16 verify_prehash (either s); the recovery parity bit is ignored.
+    No counterpart among the reference's entry points."""
+    return verify_impl(CURVE, signatures, messages, public_keys, ctx)[0]
+
+
+def verify_with_status(signatures, messages, public_keys, ctx=None):
+    """(list of bools, numpy status codes SIGOPS_ST_*)"""
+    return verify_impl(CURVE, signatures, messages, public_keys, ctx)
+
+
+__all__ = ["ecrecover", "ecrecover_single_shader", "ecrecover_single", "recover_with_status", "verify",
+           "verify_with_status", "np"]
