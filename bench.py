@@ -13,8 +13,9 @@ Also reported, each on the same mixed workload:
     and the batched/plain ratio;
   * transfers: the batched path through the host API (H2D + graph + D2H), the reference's
     benchmark method (src/benchmarks/secp256k1_ecdsa.rs:126-138);
-  * roofline: algorithmic u32 multiply-accumulates per signature (canonical schedule, see
-    DESIGN.md) divided by the launch time, against the measured v_mad_u64_u32 issue rate;
+  * roofline: algorithmic u32 multiply-accumulates per signature (the canonical textbook schedule:
+    fixed-window Fermat inversions, 252 doublings, dense 2 x 8^2 MACs per 256-bit product) divided
+    by the launch time, against the measured v_mad_u64_u32 issue rate;
   * cpu_baseline: the CPU oracle timed on a bounded sample of the same items on this host's cores,
     also used to check the GPU outputs of that sample bit for bit.
 Prints ONE JSON line.
@@ -31,7 +32,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-# canonical multiply-accumulates per signature (u32 x u32 -> u64), see DESIGN.md section "Roofline"
+# canonical multiply-accumulates per signature (u32 x u32 -> u64) of the textbook schedule above
 MAC_PER_SIG = {"secp256k1": 595072, "secp256r1": 627456, "ed25519_per_block": 24 * 128, "ed25519_base": 3850 * 128}
 LIMBS_MAC = 128  # MACs per canonical 256-bit modular multiplication
 
@@ -96,6 +97,41 @@ def fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, steps, dev):
     return res
 
 
+def predicates(ctx, mx, stream, s_ptr, steps, dev):
+    """The predicates next to the path on the batch's own items, each scheme timed alone with events
+    on the launch stream: secp verification with the generator's public keys (fuel
+    Signature::verify) and Ed25519 verify_strict. Every uncorrupted item must come out valid."""
+    import torch
+    k1, r1, ed = mx.parts
+    res = {}
+    for name, it in (("secp256k1_verify", k1), ("secp256r1_verify", r1), ("ed25519_verify_strict", ed)):
+        st = torch.empty(it.n, dtype=torch.uint8, device=dev)
+
+        def run():
+            if it is k1 or it is r1:
+                fn = ctx.secp256k1_verify_dev if it is k1 else ctx.secp256r1_verify_dev
+                fn(it.sigs.data_ptr(), it.aux.data_ptr(), it.pubs.data_ptr(), it.n, st.data_ptr(), s_ptr)
+            else:
+                ctx.ed25519_verify_strict_dev(it.sigs.data_ptr(), it.aux.data_ptr(), it.msgs.data_ptr(),
+                                              it.msg_off.data_ptr(), it.n, st.data_ptr(), s_ptr)
+
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        st_h, cor = st.cpu().numpy(), it.corrupted.cpu().numpy()
+        ok = bool((st_h[cor == 0] == 0).all())
+        res[name] = {"n": it.n, "ms": round(ms, 3), "sigs_per_s": round(it.n / (ms / 1e3), 1),
+                     "uncorrupted_all_valid": ok}
+        log(f"{name}: n={it.n} {ms:.3f} ms, {it.n / ms / 1e3:.2f} M sigs/s, uncorrupted all valid: {ok}")
+    return res
+
+
 def mixed_mac_per_sig(schemes, lens):
     """average canonical MACs per signature of this exact batch"""
     k1 = (schemes == 0).sum()
@@ -146,7 +182,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every signature is independent, "
-              "so more GPUs would shard the batch without any collective (see DESIGN.md).", file=sys.stderr)
+              "so more GPUs would shard the batch by item ranges without any collective.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -214,6 +250,7 @@ def main():
     log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
 
     fuel = fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, args.steps, dev)
+    preds = predicates(ctx, mx, stream, s_ptr, args.steps, dev)
 
     # correctness of the timed output by the round-trip property (all 2^20 items)
     out_h, st_h, cor = out.cpu().numpy(), st.cpu().numpy(), mx.corrupted.cpu().numpy()
@@ -421,6 +458,7 @@ def main():
         "roundtrip_all_items_ok": roundtrip_ok,
         "served": served,
         "fuel_hashing": fuel,
+        "predicates": preds,
         "probe": {"mad_per_s": mad_per_s, "fma64_per_s": fma_per_s},
         "setup_ms": round(setup_ms, 1),
     }

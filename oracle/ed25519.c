@@ -7,7 +7,8 @@
  * src/wgsl/ed25519_eddsa.wgsl:6-63, src/wgsl/ed25519_utils.wgsl:27-118,
  * src/wgsl/main/ed25519_eddsa_main_0.wgsl:23-86 / _5.wgsl:20-46.
  * Acceptance rules are dalek's (the tests' oracle is `verifying_key.verify`,
- * src/tests/ed25519_eddsa.rs:26), which the reference WGSL does not fully implement:
+ * src/tests/ed25519_eddsa.rs:26), which the reference WGSL does not fully implement (no s < l check,
+ * src/wgsl/main/ed25519_eddsa_main_0.wgsl:51-54):
  *   - s must be canonical (s < l), else reject;
  *   - A must decompress: y = bytes with bit 255 cleared, taken mod p (non-canonical y accepted),
  *     x from sqrt_ratio_i(y^2-1, d y^2+1), negated when the sign bit is set (also for x = 0);

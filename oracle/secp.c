@@ -7,7 +7,7 @@
  * decode the recid parity from bit 7 of s's first byte (src/wgsl/signature.wgsl:6-21,
  * src/tests/mod.rs:151-163), lift x = r to R = (r, y) with the requested parity, then
  * Q = (-z/r)*G + (s/r)*R. Acceptance rules are those of the reference's CPU path rather than of
- * its WGSL (which has no checks): fuel-crypto 0.49 over libsecp256k1
+ * its WGSL (which has no range checks, src/wgsl/secp256k1_ecdsa.wgsl:29-37): fuel-crypto 0.49 over libsecp256k1
  * (secp256k1_ecdsa_recoverable_signature_parse_compact / secp256k1_ecdsa_sig_recover) for k1 and
  * fuel-crypto's p256 backend over ecdsa 0.16 recover_from_prehash for r1:
  *   r in [1, n-1], s in [1, n-1] (recid bit cleared), z = digest mod n,

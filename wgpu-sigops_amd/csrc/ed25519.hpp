@@ -5,7 +5,7 @@
 // src/wgsl/ed25519_utils.wgsl:27-118 (pow_p58, sqrt_ratio_i, decompression),
 // src/wgsl/main/ed25519_eddsa_main_0.wgsl:23-86 (decoding, A.y mod p, preimage) and _5.wgsl:20-46
 // (compress and byte-compare with R). Acceptance rules follow dalek (the reference test oracle,
-// src/tests/ed25519_eddsa.rs:26) rather than the WGSL, which skips the s < l check:
+// src/tests/ed25519_eddsa.rs:26) rather than the WGSL, which skips the s < l check (src/wgsl/main/ed25519_eddsa_main_0.wgsl:51-54):
 //   s >= l -> reject; A must decompress (y taken mod p, sign bit applied also to x = 0);
 //   k = SHA-512(R || A || M) mod l over the given bytes; accept iff compress([s]B - [k]A) == R.
 //

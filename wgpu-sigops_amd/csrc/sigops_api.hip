@@ -2,7 +2,7 @@
 //
 // Replaces the reference's per-call wgpu life cycle (src/gpu.rs:5-186: adapter + device creation,
 // WGSL compilation, buffer creation, dispatch, staging readback and device.destroy on every call,
-// SURVEY Q8) with a context that is created once: code objects are loaded with the library, the
+// src/secp256k1_ecdsa.rs:79-80) with a context that is created once: code objects are loaded with the library, the
 // generator tables are uploaded to HBM once, and per-lane scratch plus I/O buffers are reused.
 #include <hip/hip_runtime.h>
 
