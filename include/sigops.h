@@ -119,6 +119,27 @@ int sigops_secp256k1_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t*
 int sigops_secp256r1_verify(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, const uint8_t* pubkeys,
                             size_t n, uint8_t* valid, uint8_t* status);
 
+/* Caller-supplied fixed-base tables (`table_limbs` / `log_limb_size` of the reference's ecrecover
+   and ecverify, src/secp256k1_ecdsa.rs:61-66, src/secp256r1_ecdsa.rs:62-67, src/ed25519_eddsa.rs:67-73).
+   The table is the reference's layout (src/precompute.rs:12-69): [P, 2P, ..., 16P] affine, each
+   coordinate x * 2^(nl*l) mod p in nl = 256 / l + 1 little-endian limbs of l = log_limb_size bits
+   (11..15); secp entries (x, y), Ed25519 entries (x, y, x*y). The reference's stage-1 shaders use it
+   as the fixed base, so the result is u1 * P + u2 * R (secp) or [s]P - [k]A (Ed25519) for the
+   table's point P; these entry points convert it on the device (a 128-lane kernel that also builds
+   the odd multiples) and run the same kernels on it. table_len is the number of u32 limbs. A table
+   that is not [P, ..., 16P] for a curve point P (for Ed25519: of prime order) fails the call with
+   SIGOPS_E_ARG; for the tables precompute::*_bases builds (P = G, B) the results equal the plain
+   entry points'. */
+int sigops_secp256k1_recover_table(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                   const uint32_t* table_limbs, size_t table_len, uint32_t log_limb_size,
+                                   uint8_t* out_pub, uint8_t* status);
+int sigops_secp256r1_recover_table(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                   const uint32_t* table_limbs, size_t table_len, uint32_t log_limb_size,
+                                   uint8_t* out_pub, uint8_t* status);
+int sigops_ed25519_verify_table(sigops_ctx* ctx, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                                const uint64_t* msg_off, size_t n, const uint32_t* table_limbs, size_t table_len,
+                                uint32_t log_limb_size, uint8_t* valid, uint8_t* status);
+
 /* ---- device-resident variants: pointers are device memory, work is enqueued on `stream`
    (a hipStream_t, NULL for the default stream) and the call returns without synchronising.
    The context's scratch is shared, so calls on one context must not overlap. ---- */

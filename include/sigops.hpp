@@ -11,8 +11,8 @@
 // ed25519_eddsa::ecverify_strict (dalek verify_strict)
 // Rust's Result<_, ShaderFailureError> (src/lib.rs:12-14) becomes a thrown sigops::ShaderFailureError;
 // the reference's panics on bad arguments (length mismatch, src/secp256k1_ecdsa.rs:21) become
-// std::invalid_argument. The table arguments are checked for shape only: the library keeps its own
-// generator tables resident in HBM.
+// std::invalid_argument. A table argument is converted on the device and used as the fixed base, as
+// the reference uses it (sigops_*_table); one that is not [P, 2P, ..., 16P] throws std::runtime_error.
 #pragma once
 #include <array>
 #include <cstdint>
@@ -68,9 +68,11 @@ inline void check_table(const std::vector<uint32_t>& t, size_t coords, uint32_t 
   check_log_limb_size(l);
   if (t.size() != 16 * coords * (256 / l + 1)) throw std::invalid_argument("table_limbs has the wrong size");
 }
+// table: NULL (the resident generator table) or the caller's table_limbs in the reference's layout
 template <class Sig>
 std::vector<std::vector<uint8_t>> recover(int curve, const std::vector<Sig>& signatures,
-                                          const std::vector<Message>& messages, Context& ctx) {
+                                          const std::vector<Message>& messages, Context& ctx,
+                                          const std::vector<uint32_t>* table = nullptr, uint32_t log_limb_size = 0) {
   if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
   const size_t n = signatures.size();
   if (n > (size_t(1) << 30)) throw std::invalid_argument("at most 2^30 signatures");  // init's assert, :21-22
@@ -80,8 +82,12 @@ std::vector<std::vector<uint8_t>> recover(int curve, const std::vector<Sig>& sig
     memcpy(&sigs[i * 64], signatures[i].data(), 64);
     memcpy(&digs[i * 32], messages[i].data(), 32);
   }
-  ctx.check(curve == 0 ? sigops_secp256k1_recover(ctx.get(), sigs.data(), digs.data(), n, out.data(), nullptr)
-                       : sigops_secp256r1_recover(ctx.get(), sigs.data(), digs.data(), n, out.data(), nullptr));
+  if (table)
+    ctx.check((curve == 0 ? sigops_secp256k1_recover_table : sigops_secp256r1_recover_table)(
+        ctx.get(), sigs.data(), digs.data(), n, table->data(), table->size(), log_limb_size, out.data(), nullptr));
+  else
+    ctx.check(curve == 0 ? sigops_secp256k1_recover(ctx.get(), sigs.data(), digs.data(), n, out.data(), nullptr)
+                         : sigops_secp256r1_recover(ctx.get(), sigs.data(), digs.data(), n, out.data(), nullptr));
   std::vector<std::vector<uint8_t>> res(n);
   for (size_t i = 0; i < n; i++) res[i].assign(out.begin() + i * 64, out.begin() + i * 64 + 64);
   return res;
@@ -120,7 +126,7 @@ inline std::vector<std::vector<uint8_t>> ecrecover(const std::vector<Signature>&
                                                    const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
                                                    Context& ctx = default_context()) {
   detail::check_table(table_limbs, 2, log_limb_size);
-  return detail::recover(0, signatures, messages, ctx);
+  return detail::recover(0, signatures, messages, ctx, &table_limbs, log_limb_size);
 }
 inline std::vector<std::vector<uint8_t>> ecrecover_single_shader(const std::vector<Signature>& signatures,
                                                                  const std::vector<Message>& messages,
@@ -147,7 +153,7 @@ inline std::vector<std::vector<uint8_t>> ecrecover(const std::vector<Bytes64>& s
                                                    const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
                                                    Context& ctx = default_context()) {
   detail::check_table(table_limbs, 2, log_limb_size);
-  return detail::recover(1, signatures, messages, ctx);
+  return detail::recover(1, signatures, messages, ctx, &table_limbs, log_limb_size);
 }
 inline std::vector<std::vector<uint8_t>> ecrecover_single_shader(const std::vector<Bytes64>& signatures,
                                                                  const std::vector<Message>& messages,
@@ -166,7 +172,8 @@ inline std::vector<std::vector<uint8_t>> ecrecover_single(const std::vector<Byte
 namespace ed25519_eddsa {
 namespace detail_ed {
 inline std::vector<bool> verify(const std::vector<Ed25519Signature>& signatures, const std::vector<Message>& messages,
-                                const std::vector<VerifyingKey>& verifying_keys, Context& ctx) {
+                                const std::vector<VerifyingKey>& verifying_keys, Context& ctx,
+                                const std::vector<uint32_t>* table = nullptr, uint32_t log_limb_size = 0) {
   if (signatures.size() != messages.size()) throw std::invalid_argument("signatures and messages differ in length");
   const size_t n = signatures.size();
   if (n == 0) return {};
@@ -180,7 +187,12 @@ inline std::vector<bool> verify(const std::vector<Ed25519Signature>& signatures,
     off[i] = 32 * i;
   }
   off[n] = 32 * n;
-  ctx.check(sigops_ed25519_verify(ctx.get(), sigs.data(), pks.data(), msgs.data(), off.data(), n, valid.data(), nullptr));
+  if (table)
+    ctx.check(sigops_ed25519_verify_table(ctx.get(), sigs.data(), pks.data(), msgs.data(), off.data(), n,
+                                          table->data(), table->size(), log_limb_size, valid.data(), nullptr));
+  else
+    ctx.check(
+        sigops_ed25519_verify(ctx.get(), sigs.data(), pks.data(), msgs.data(), off.data(), n, valid.data(), nullptr));
   std::vector<bool> res(n);
   for (size_t i = 0; i < n; i++) res[i] = valid[i] == 1;
   return res;
@@ -192,7 +204,7 @@ inline std::vector<bool> ecverify(const std::vector<Ed25519Signature>& signature
                                   const std::vector<uint32_t>& table_limbs, uint32_t log_limb_size,
                                   Context& ctx = default_context()) {
   detail::check_table(table_limbs, 3, log_limb_size);
-  return detail_ed::verify(signatures, messages, verifying_keys, ctx);
+  return detail_ed::verify(signatures, messages, verifying_keys, ctx, &table_limbs, log_limb_size);
 }
 inline std::vector<bool> ecverify_single(const std::vector<Ed25519Signature>& signatures,
                                          const std::vector<Message>& messages,

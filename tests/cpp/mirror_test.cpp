@@ -4,6 +4,7 @@
 //   ed <sig hex> <pubkey hex> <message hex (32 bytes)> <0|1>
 //   k1v|r1v <sig hex> <digest hex> <pub hex> <0|1>      (verification with a known key)
 //   eds <sig hex> <pubkey hex> <message hex or -> <0|1> (verify_strict, any message length)
+//   tk1|tr1|ted <limb> <limb> ...                       (the reference's 13-bit tables, precompute::*_bases)
 // and checks every result; also checks the error behaviour (length mismatch, bad limb size).
 #include <cstdio>
 #include <fstream>
@@ -43,8 +44,16 @@ int main(int argc, char** argv) {
   std::vector<sigops::VerifyingKey> stk;
   std::vector<std::vector<uint8_t>> stm;
   std::vector<bool> ste;
+  std::vector<uint32_t> t2k1, t2r1, t3;
   std::string line;
   while (std::getline(in, line)) {
+    if (line.rfind("tk1 ", 0) == 0 || line.rfind("tr1 ", 0) == 0 || line.rfind("ted ", 0) == 0) {
+      std::istringstream ts(line.substr(4));
+      std::vector<uint32_t>& t = line[1] == 'k' ? t2k1 : (line[1] == 'r' ? t2r1 : t3);
+      uint32_t v;
+      while (ts >> v) t.push_back(v);
+      continue;
+    }
     std::istringstream ss(line);
     std::string kind, a, b, c, d;
     ss >> kind >> a >> b >> c;
@@ -63,13 +72,14 @@ int main(int argc, char** argv) {
     }
   }
   // the tables as the reference builds them: 16 points x {x,y} (x {x,y,t}) x 20 limbs at 13 bits
-  std::vector<uint32_t> t2(16 * 2 * 20), t3(16 * 3 * 20);
   int bad = 0;
+  bad += t2k1.size() != 16 * 2 * 20 || t2r1.size() != 16 * 2 * 20 || t3.size() != 16 * 3 * 20;
+  const std::vector<uint32_t>& t2 = t2k1;
   auto r = sigops::secp256k1_ecdsa::ecrecover(k1s, k1d, t2, 13);
   for (size_t i = 0; i < r.size(); i++) bad += r[i] != k1e[i];
   auto r2 = sigops::secp256k1_ecdsa::ecrecover_single(k1s, k1d, 13);
   bad += r2 != r;
-  auto q = sigops::secp256r1_ecdsa::ecrecover(r1s, r1d, t2, 13);
+  auto q = sigops::secp256r1_ecdsa::ecrecover(r1s, r1d, t2r1, 13);
   for (size_t i = 0; i < q.size(); i++) bad += q[i] != r1e[i];
   auto v = sigops::ed25519_eddsa::ecverify(eds, edm, edk, t3, 13);
   for (size_t i = 0; i < v.size(); i++) bad += v[i] != ede[i];
@@ -94,6 +104,11 @@ int main(int argc, char** argv) {
   try { sigops::secp256k1_ecdsa::ecrecover_single_shader(k1s, k1d, 9); } catch (const std::invalid_argument&) { thrown++; }
   try { sigops::secp256r1_ecdsa::ecrecover(r1s, r1d, std::vector<uint32_t>(3), 13); } catch (const std::invalid_argument&) { thrown++; }
   bad += thrown != 3;
+  // a table of the right shape that is not [P, ..., 16P] is refused by the library
+  bool refused = false;
+  try { sigops::secp256k1_ecdsa::ecrecover(k1s, k1d, std::vector<uint32_t>(16 * 2 * 20), 13); }
+  catch (const std::runtime_error&) { refused = true; }
+  bad += !refused;
   std::cout << "mirror: k1 " << r.size() << " r1 " << q.size() << " ed " << v.size() << " secp verify " << nv
             << " ed strict " << ste.size() << " mismatches " << bad << std::endl;
   return bad ? 1 : 0;

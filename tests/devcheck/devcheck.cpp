@@ -7,6 +7,7 @@
 #include "gen.hpp"
 #include "secp.hpp"
 #include "sha256.hpp"
+#include "table.hpp"
 
 using namespace sigops;
 
@@ -107,9 +108,58 @@ int verify_one(const unsigned char* sig, const unsigned char* dig, const unsigne
   const int st = Pc.inf ? (int)ST_INFINITY : (int)secp_verify_x<C>(Pc.P, sr);
   return st | (exc ? 0x100 : 0);
 }
+// the 128-lane workgroup of kern_table.hip, lane by lane: check entries 0..15, then build
+template <class C>
+u32 table_secp_host(const u32* limbs, u32 lbits, u32* out) {
+  typedef typename C::Fp F;
+  const u32 nl = 256 / lbits + 1;
+  fe xs[16], ys[16];
+  u32 bad = 0;
+  for (u32 j = 0; j < 16; j++) tab_secp_check_entry<C>(j, limbs, nl, lbits, xs[j], ys[j], bad);
+  u32 p0[2 * NL];
+  const fe xc = fcanon<F>(xs[0]), yc = fcanon<F>(ys[0]);
+  for (int k = 0; k < NL; k++) {
+    p0[k] = xc.v[k];
+    p0[NL + k] = yc.v[k];
+  }
+  for (u32 j = 0; j < 128; j++)
+    tab_secp_build_entry<C>(j, p0, j < 16 ? xs[j] : fe_zero(), j < 16 ? ys[j] : fe_zero(), bad, out + 20 * j);
+  return bad;
+}
+
+u32 table_ed_host(const u32* limbs, u32 lbits, u32* out) {
+  const u32 nl = 256 / lbits + 1;
+  fe xs[16], ys[16];
+  u32 bad = 0;
+  for (u32 j = 0; j < 16; j++) tab_ed_check_entry(j, limbs, nl, lbits, xs[j], ys[j], bad);
+  u32 p0[2 * NL];
+  const fe xc = fcanon<ModPed>(xs[0]), yc = fcanon<ModPed>(ys[0]);
+  for (int k = 0; k < NL; k++) {
+    p0[k] = xc.v[k];
+    p0[NL + k] = yc.v[k];
+  }
+  for (u32 j = 0; j < 128; j++)
+    tab_ed_build_entry(j, p0, j < 16 ? xs[j] : fe_zero(), j < 16 ? ys[j] : fe_zero(), bad, out + 28 * j);
+  return bad;
+}
 }  // namespace
 
 extern "C" {
+// caller-table conversion (host build of kern_table.hip): returns the TAB_BAD_* bits, writes the
+// 128-entry table (20 words per secp entry, 28 per Ed25519 entry); which: 0 k1, 1 r1, 2 ed25519
+unsigned dc_table(int which, const u32* limbs, unsigned lbits, u32* out) {
+  if (which == 0) return table_secp_host<CurveK1>(limbs, lbits, out);
+  if (which == 1) return table_secp_host<CurveR1>(limbs, lbits, out);
+  return table_ed_host(limbs, lbits, out);
+}
+
+// the resident generator tables as uploaded to HBM (same layout as dc_table's output)
+void dc_resident_table(int which, u32* out) {
+  if (which == 0) memcpy(out, K1_GTABLE, sizeof K1_GTABLE);
+  else if (which == 1) memcpy(out, R1_GTABLE, sizeof R1_GTABLE);
+  else memcpy(out, ED_BTABLE, sizeof ED_BTABLE);
+}
+
 // verification with a known key: status; bit 8 set when the fast ladder met an exceptional addition
 int dc_secp_verify(int curve, const unsigned char* sig, const unsigned char* dig, const unsigned char* pub,
                    int force_careful) {

@@ -35,6 +35,9 @@ def lib():
         L.dc_glv_split.argtypes = [u32p, u32p, u32p]
         L.dc_sha256.argtypes = [c_p, ctypes.c_uint, ctypes.c_uint, c_p]
         L.dc_secp_verify.argtypes = [ctypes.c_int, c_p, c_p, c_p, ctypes.c_int]
+        L.dc_table.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
+        L.dc_table.restype = ctypes.c_uint
+        L.dc_resident_table.argtypes = [ctypes.c_int, ctypes.c_void_p]
         L.dc_sha256_64.argtypes = [c_p, c_p]
         _lib = L
     return _lib
@@ -86,6 +89,23 @@ def secp_verify(curve, sig, dig, pub, force_careful=False):
     """(status, careful path ran) of the device verification built for the host"""
     st = lib().dc_secp_verify(curve, bytes(sig), bytes(dig), bytes(pub), int(force_careful))
     return st & 0xFF, bool(st >> 8)
+
+
+def table(which, limbs, lbits):
+    """caller-table conversion (0 k1, 1 r1, 2 ed25519): (TAB_BAD_* bits, (128, 20|28) uint32 table)"""
+    import numpy as np
+    width = 28 if which == 2 else 20
+    limbs = np.ascontiguousarray(limbs, dtype=np.uint32)
+    out = np.zeros((128, width), np.uint32)
+    bad = lib().dc_table(which, limbs.ctypes.data, lbits, out.ctypes.data)
+    return bad, out
+
+
+def resident_table(which):
+    import numpy as np
+    out = np.zeros((128, 28 if which == 2 else 20), np.uint32)
+    lib().dc_resident_table(which, out.ctypes.data)
+    return out
 
 
 def sha256(msg, at=0):

@@ -185,3 +185,54 @@ def test_ed25519_strict_cases():
         assert (st == 0) == exp == O.ed25519_verify_strict(sig, pk, m), name
         assert (D.ed25519_verify(sig, pk, m) == 0) == O.ed25519_verify(sig, pk, m), name
     assert {0, 1, 2, 3, 4, 5} <= codes
+
+
+def _mont_limbs(v, p):
+    """an engine table word group: v * 2^261 mod p as nine 29-bit limbs"""
+    m = v * 2**261 % p
+    return [(m >> (29 * k)) & (2**29 - 1) for k in range(9)]
+
+
+@pytest.mark.parametrize("lbits", [11, 12, 13, 14, 15])
+def test_caller_tables_canonical(lbits):
+    """the reference's own tables (precompute::*_bases) convert to exactly the resident tables"""
+    from sigops_pkg import sigops
+    P = sigops.precompute
+    for which, tab in ((0, P.secp256k1_bases(lbits)), (1, P.secp256r1_bases(lbits)), (2, P.ed25519_bases(lbits))):
+        bad, out = D.table(which, tab, lbits)
+        assert bad == 0, (which, lbits)
+        assert np.array_equal(out, D.resident_table(which)), (which, lbits)
+
+
+def test_caller_tables_other_points_and_bad_tables():
+    from sigops_pkg import sigops
+    P = sigops.precompute
+    for curve in (0, 1):
+        c = pyref.CURVES[curve]
+        Q = pyref.w_mul(c, c["G"], 12345)
+        bad, out = D.table(curve, P.secp_table_for_point(curve, Q, 13), 13)
+        assert bad == 0
+        for j in (0, 1, 17, 127):  # odd multiple (2j+1) Q, closed form
+            x, y = pyref.w_mul(c, Q, 2 * j + 1)
+            assert list(out[j, :9]) == _mont_limbs(x, c["p"]) and list(out[j, 9:18]) == _mont_limbs(y, c["p"])
+        good = np.array(P.secp_table_for_point(curve, Q, 13), np.uint32)
+        t = good.copy()
+        t[5] ^= 1  # one limb of entry 0's x: off the curve
+        assert D.table(curve, t, 13)[0] & 1
+        t = good.copy()
+        t[40 * 3:40 * 4], t[40 * 4:40 * 5] = good[40 * 4:40 * 5], good[40 * 3:40 * 4]  # entries 3 and 4 swapped
+        assert D.table(curve, t, 13)[0] == 2
+        t = good.copy()
+        t[7] = 1 << 13  # a limb out of range
+        assert D.table(curve, t, 13)[0] & 1
+    # Ed25519: another point of the prime-order subgroup, and one with a torsion component
+    B5 = pyref.e_mul(pyref.EB, 5)
+    bad, out = D.table(2, P.ed25519_table_for_point(B5, 12), 12)
+    assert bad == 0
+    x, y = pyref.e_mul(B5, 7)  # entry j = 3 holds 7 * B5 as (y+x, y-x, 2d x y)
+    p = pyref.EP
+    assert list(out[3, :9]) == _mont_limbs((y + x) % p, p) and list(out[3, 9:18]) == _mont_limbs((y - x) % p, p)
+    assert list(out[3, 18:27]) == _mont_limbs(2 * pyref.ED * x * y % p, p)
+    T = pyref.ed25519_torsion()[-1]
+    mixed = pyref.e_add(B5, T)
+    assert D.table(2, P.ed25519_table_for_point(mixed, 12), 12)[0] == 8

@@ -19,7 +19,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def test_cpp_mirror(tmp_path):
     exe = os.path.join(HERE, "cpp", "mirror_test")
     assert os.path.exists(exe), "tests/cpp/mirror_test is built by __graft_entry__.build()"
-    lines = []
+    from sigops_pkg import sigops
+    P = sigops.precompute
+    lines = [f"{tag} " + " ".join(str(v) for v in tab)
+             for tag, tab in (("tk1", P.secp256k1_bases(13)), ("tr1", P.secp256r1_bases(13)),
+                              ("ted", P.ed25519_bases(13)))]
     kat = json.load(open(os.path.join(HERE, "golden", "secp256k1_kat.json")))
     lines.append(f"k1 {kat['signature']} {kat['digest']} {kat['pubkey']}")
     for curve, tag in ((0, "k1"), (1, "r1")):

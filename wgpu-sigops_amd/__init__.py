@@ -77,6 +77,9 @@ def lib():
             for name in ("sigops_secp256k1_recover", "sigops_secp256r1_recover"):
                 getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
             L.sigops_ed25519_verify.argtypes = [vp, c_p, c_p, c_p, c_p, sz, c_p, c_p]
+            for name in ("sigops_secp256k1_recover_table", "sigops_secp256r1_recover_table"):
+                getattr(L, name).argtypes = [vp, c_p, c_p, sz, vp, sz, ctypes.c_uint32, c_p, c_p]
+            L.sigops_ed25519_verify_table.argtypes = [vp, c_p, c_p, c_p, c_p, sz, vp, sz, ctypes.c_uint32, c_p, c_p]
             for name in ("sigops_secp256k1_verify", "sigops_secp256r1_verify"):
                 getattr(L, name).argtypes = [vp, c_p, c_p, c_p, sz, c_p, c_p]
             for name in ("sigops_secp256k1_verify_dev", "sigops_secp256r1_verify_dev"):
@@ -204,6 +207,34 @@ class Context:
 
     def secp256r1_recover(self, sigs, digests):
         return self._secp(lib().sigops_secp256r1_recover, sigs, digests)
+
+    def secp_recover_table(self, curve, sigs, digests, table_limbs, log_limb_size):
+        """recovery with the caller's fixed-base table in the reference's layout: u1 * P + u2 * R for
+        the table's point P (raises SigopsError when the table is not [P, ..., 16P])"""
+        tab = np.ascontiguousarray(table_limbs, dtype=np.uint32).reshape(-1)
+        fn = lib().sigops_secp256k1_recover_table if curve == 0 else lib().sigops_secp256r1_recover_table
+
+        def call(h, ps, pd, n, po, pst):
+            return fn(h, ps, pd, n, tab.ctypes.data, tab.size, int(log_limb_size), po, pst)
+        return self._secp(call, sigs, digests)
+
+    def ed25519_verify_table(self, sigs, pubkeys, msgs, msg_off, table_limbs, log_limb_size):
+        """verification with the caller's fixed-base table (reference layout, x, y, x*y per entry)"""
+        sigs = _u8(sigs, 64, "sigs")
+        pubkeys = _u8(pubkeys, 32, "pubkeys")
+        n = sigs.shape[0]
+        msg_off = np.ascontiguousarray(msg_off, dtype=np.uint64)
+        if pubkeys.shape[0] != n or msg_off.shape[0] != n + 1:
+            raise ValueError("sigs, pubkeys and msg_off (n+1) differ in length")
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint8).reshape(-1)
+        tab = np.ascontiguousarray(table_limbs, dtype=np.uint32).reshape(-1)
+        valid = np.zeros(n, np.uint8)
+        st = np.zeros(n, np.uint8)
+        self._check(lib().sigops_ed25519_verify_table(self._h, _ptr(sigs), _ptr(pubkeys),
+                                                      _ptr(msgs) if msgs.size else None, _ptr(msg_off), n,
+                                                      tab.ctypes.data, tab.size, int(log_limb_size), _ptr(valid),
+                                                      _ptr(st)))
+        return valid.astype(bool), st
 
     def _secp_verify(self, fn, sigs, digests, pubkeys):
         sigs = _u8(sigs, 64, "sigs")

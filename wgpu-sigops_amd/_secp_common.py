@@ -15,7 +15,9 @@ def _as_bytes_array(items, width, what):
     return np.frombuffer(bytes(buf), dtype=np.uint8).reshape(-1, width)
 
 
-def ecrecover_impl(curve, signatures, messages, ctx):
+def ecrecover_impl(curve, signatures, messages, ctx, table=None):
+    """table: None (the resident generator table) or (table_limbs, log_limb_size) in the
+    reference's layout, converted and used as the fixed base on the device"""
     from . import default_context
 
     if len(signatures) != len(messages):
@@ -27,8 +29,11 @@ def ecrecover_impl(curve, signatures, messages, ctx):
     sigs = _as_bytes_array(signatures, 64, "signature")
     digs = _as_bytes_array(messages, 32, "message")
     ctx = ctx or default_context()
-    fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
-    out, st = fn(sigs, digs)
+    if table is not None:
+        out, st = ctx.secp_recover_table(curve, sigs, digs, table[0], table[1])
+    else:
+        fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
+        out, st = fn(sigs, digs)
     return [bytes(row) for row in out], st
 
 

@@ -188,12 +188,33 @@ DEV void ed25519_challenge(const u32 sig_raw[16], const u32 pk_raw[8], const Mem
   fe_to_words(fcanon<Fl>(fadd<Fl>(hi, lo)), k);
 }
 
+// verify_strict's extra rules (dalek VerifyingKey::verify_strict): R must decompress and neither R
+// nor A may have small order. Returns ED_OK, ED_BAD_R or ED_SMALL_ORDER. They are evaluated in
+// the hash pass, which has registers to spare, and travel to the verifier in bits 29..31 of the
+// challenge's top word: k < l < 2^253 leaves those bits zero.
+DEV u32 ed25519_strict_checks(const u32 sig_raw[16], const u32 pk_raw[8]) {
+  fe ax, ay, rx, ry;
+  (void)ed_decompress(pk_raw, ax, ay);  // an undecodable A is rejected by the verifier first
+  if (!ed_decompress(sig_raw, rx, ry)) return ED_BAD_R;
+  return (ed_small_order(rx, ry) || ed_small_order(ax, ay)) ? ED_SMALL_ORDER : ED_OK;
+}
+constexpr u32 ED_K_STRICT_SHIFT = 29;
+
 // verification given the challenge k (from ed25519_challenge).
 // BTab: load(idx, ypx, ymx, t2d) for (2 idx + 1) B. Tab: per-lane store(e, cached) / load(e).
-// STRICT: dalek VerifyingKey::verify_strict, which also requires that R decompresses and that
-// neither R nor A has small order; the final comparison stays on the compressed bytes of R.
+// STRICT: dalek VerifyingKey::verify_strict; the status of its extra rules arrives in the top bits
+// of k (ed25519_strict_checks). The final comparison stays on the compressed bytes of R.
 template <bool STRICT = false, class Tab, class BTab>
-DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u32 k[8], Tab& tab, const BTab& bt) {
+DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u32 k_in[8], Tab& tab,
+                            const BTab& bt) {
+  u32 k[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) k[i] = k_in[i];
+  u32 strict_status = ED_OK;
+  if constexpr (STRICT) {
+    strict_status = k[7] >> ED_K_STRICT_SHIFT;
+    k[7] &= (1u << ED_K_STRICT_SHIFT) - 1u;
+  }
   u32 status = ED_OK;
   // s (little-endian bytes = little-endian words)
   u32 s[8];
@@ -204,13 +225,7 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
   const bool ok = ed_decompress(pk_raw, x, y);
   if (!ok && status == ED_OK) status = ED_BAD_KEY;
   const fe one = fe_from_const(Fe::ONE);
-  if constexpr (STRICT) {
-    fe rx, ry;
-    const bool r_ok = ed_decompress(sig_raw, rx, ry);
-    if (!r_ok && status == ED_OK) status = ED_BAD_R;
-    const bool small = ed_small_order(rx, ry) || ed_small_order(x, y);
-    if (small && status == ED_OK) status = ED_SMALL_ORDER;
-  }
+  if (STRICT && status == ED_OK) status = strict_status;
   // odd representatives: s even -> l - s with B negated; k even -> k + 1 and a final +A
   const bool b_neg = (s[0] & 1u) == 0;
   const bool k_even = (k[0] & 1u) == 0;
@@ -300,6 +315,7 @@ DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Me
                             Tab& tab, const BTab& bt) {
   u32 k[8];
   ed25519_challenge(sig_raw, pk_raw, mem, msg_off, len, k);
+  if constexpr (STRICT) k[7] |= ed25519_strict_checks(sig_raw, pk_raw) << ED_K_STRICT_SHIFT;
   return ed25519_verify_core<STRICT>(sig_raw, pk_raw, k, tab, bt);
 }
 

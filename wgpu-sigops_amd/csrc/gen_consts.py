@@ -457,6 +457,15 @@ def main():
     o.write(arr("SHA256_K", K2))
     o.write(arr("SHA256_IV", IV2))
     o.write(arr("SHA256_PAD64_KW", sha256_pad_kw(512)))  # second block of every 64-byte message
+    # caller-supplied tables in the reference's layout hold x * 2^(nl*l) mod p in nl = 256 // l + 1
+    # limbs of l = 11..15 bits (src/tests/mod.rs:136-149); one Montgomery product with
+    # C_l = 2^(-nl*l) * 2^522 mod p turns that value into x * 2^261 mod p (this engine's form)
+    for tag, p in [("K1", P_K1), ("R1", P_R1), ("ED", P_ED)]:
+        vals = []
+        for lb in range(11, 16):
+            nl = 256 // lb + 1
+            vals += limbs(pow(2, -nl * lb, p) * 2**522 % p)
+        o.write(arr(f"REFCONV_{tag}", vals))
     o.write("\n// host-side generator tables: entries of 20 u32 (x[9], y[9], pad[2]) for secp,\n")
     o.write("// 28 u32 (ypx[9], ymx[9], t2d[9], pad) for ed25519; Montgomery form\n")
     ntab = 1 << (WG - 1)

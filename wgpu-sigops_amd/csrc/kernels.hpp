@@ -70,6 +70,12 @@ hipError_t launch_message_new(const void* msgs, const uint64_t* off, uint32_t n,
 hipError_t launch_fuel_address(const void* pubs, const void* status, uint32_t n, void* out, uint32_t grid,
                                hipStream_t s);
 
+// caller-supplied generator tables in the reference's layout -> this engine's table (flags[0] = 0
+// when the table passes its checks); lbits in 11..15
+hipError_t launch_table_secp(int curve, const uint32_t* limbs, uint32_t lbits, void* gtab_out, uint32_t* flags,
+                             hipStream_t s);
+hipError_t launch_table_ed(const uint32_t* limbs, uint32_t lbits, void* btab_out, uint32_t* flags, hipStream_t s);
+
 hipError_t launch_gen_secp(int curve, uint64_t seed, uint32_t n, uint32_t ppm, const void* gtab, void* sigs,
                            void* digs, void* pubs, void* corrupted, uint32_t grid, hipStream_t s);
 hipError_t launch_gen_ed25519(uint64_t seed, uint32_t n, uint32_t ppm, const void* btab, const uint64_t* msg_off,

@@ -269,7 +269,9 @@ __global__ __launch_bounds__(64) void k_secp_verify_careful(const uint4* __restr
 // k = SHA-512(R || A || M) mod l per item; a pass of its own so that the verifier's register
 // budget carries no hash state (the reference also hashes in a separate stage,
 // src/wgsl/main/ed25519_eddsa_main_1.wgsl). `order` (may be NULL) lists the items to hash.
-static __global__ __launch_bounds__(256) void k_ed25519_hash(const uint4* __restrict__ sigs,
+// STRICT: also evaluates verify_strict's extra rules into the top bits of k (ed25519_strict_checks)
+template <bool STRICT>
+__global__ __launch_bounds__(256) void k_ed25519_hash(const uint4* __restrict__ sigs,
                                                              const uint4* __restrict__ pks,
                                                              const u32* __restrict__ msgs,
                                                              const u64* __restrict__ msg_off,
@@ -285,6 +287,7 @@ static __global__ __launch_bounds__(256) void k_ed25519_hash(const uint4* __rest
     load_words(pks + 2 * (u64)i, pr, 2);
     const u64 off = msg_off[i];
     ed25519_challenge(sr, pr, mem, off, (u32)(msg_off[i + 1] - off), k);
+    if constexpr (STRICT) k[7] |= ed25519_strict_checks(sr, pr) << ED_K_STRICT_SHIFT;
     kbuf[2 * (u64)i] = make_uint4(k[0], k[1], k[2], k[3]);
     kbuf[2 * (u64)i + 1] = make_uint4(k[4], k[5], k[6], k[7]);
   }

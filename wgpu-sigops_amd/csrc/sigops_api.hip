@@ -21,6 +21,8 @@
 namespace {
 
 constexpr uint8_t CANARY = 0xEE;
+// largest caller table: 16 entries x 3 coordinates x 24 limbs (l = 11)
+constexpr size_t TABLE_LIMBS_MAX = 16 * 3 * 24;
 
 struct DevBuf {
   void* p = nullptr;
@@ -48,6 +50,10 @@ struct sigops_ctx {
   uint32_t* d_order = nullptr;
   uint32_t* d_n = nullptr;
   uint32_t* d_hq = nullptr;  // work counter of the message-hashing kernel
+  // caller-supplied tables in the reference's layout: limbs in, converted table out, check flags
+  uint32_t* d_tlimbs = nullptr;
+  void* d_ctab = nullptr;
+  uint32_t* d_tflags = nullptr;
   std::map<std::tuple<const void*, const void*, const void*, const void*, const void*, void*, void*>,
            hipGraphExec_t>
       graphs;
@@ -97,23 +103,54 @@ __global__ void k_fill_bytes(uint8_t* p, uint8_t v, size_t n) {
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
+// gtab: the fixed-base table to use (NULL: the resident table of G)
 int secp_recover_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_digs, size_t n, void* d_out,
-                     void* d_status, hipStream_t s) {
+                     void* d_status, hipStream_t s, const void* gtab = nullptr) {
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
   if (curve == 0)
-    HIPCHK(c, sigops::launch_secp_recover_CurveK1(d_sigs, d_digs, d_out, d_status, (uint32_t)n, c->d_gtab_k1,
-                                                 c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+    HIPCHK(c, sigops::launch_secp_recover_CurveK1(d_sigs, d_digs, d_out, d_status, (uint32_t)n,
+                                                 gtab ? gtab : c->d_gtab_k1, c->d_lane_tab, c->slots, c->d_pend + 1,
+                                                 c->d_pend, c->grid, s));
   else
-    HIPCHK(c, sigops::launch_secp_recover_CurveR1(d_sigs, d_digs, d_out, d_status, (uint32_t)n, c->d_gtab_r1,
-                                                 c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+    HIPCHK(c, sigops::launch_secp_recover_CurveR1(d_sigs, d_digs, d_out, d_status, (uint32_t)n,
+                                                 gtab ? gtab : c->d_gtab_r1, c->d_lane_tab, c->slots, c->d_pend + 1,
+                                                 c->d_pend, c->grid, s));
+  return SIGOPS_OK;
+}
+
+// A caller-supplied table in the reference's layout (which: 0 k1, 1 r1, 2 ed25519) converted into
+// c->d_ctab on the context's stream; fails with SIGOPS_E_ARG unless it is [P, 2P, ..., 16P] for a
+// curve point P (Ed25519: of prime order) in l-bit Montgomery limbs.
+int load_caller_table(sigops_ctx* c, int which, const uint32_t* table_limbs, size_t table_len, uint32_t lbits) {
+  if (lbits < 11 || lbits > 15) return fail(c, SIGOPS_E_ARG, "log_limb_size must be in 11..15");
+  const size_t coords = which == 2 ? 3 : 2;
+  const size_t want = 16 * coords * (256 / lbits + 1);
+  if (!table_limbs || table_len != want)
+    return fail(c, SIGOPS_E_ARG, "table_limbs must hold 16 x " + std::to_string(coords) + " x " +
+                                     std::to_string(256 / lbits + 1) + " limbs");
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->d_tlimbs, table_limbs, want * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  if (which == 2)
+    HIPCHK(c, sigops::launch_table_ed(c->d_tlimbs, lbits, c->d_ctab, c->d_tflags, s));
+  else
+    HIPCHK(c, sigops::launch_table_secp(which, c->d_tlimbs, lbits, c->d_ctab, c->d_tflags, s));
+  uint32_t flags = 0;
+  HIPCHK(c, hipMemcpyAsync(&flags, c->d_tflags, sizeof flags, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (flags)
+    return fail(c, SIGOPS_E_ARG,
+                std::string("table_limbs is not [P, 2P, ..., 16P] for a curve point P") +
+                    ((flags & 1) ? " (a limb, value or curve equation is wrong)" : "") +
+                    ((flags & 2) ? " (an entry is not the next multiple)" : "") +
+                    ((flags & 8) ? " (P is not in the prime-order subgroup)" : ""));
   return SIGOPS_OK;
 }
 
 // out_pub: n x 64 recovered keys; or, with out_addr set instead, n x 32 addresses SHA-256(X || Y)
 // computed on the device after the recovery (only the addresses travel back)
 int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8_t* digs, size_t n,
-                      uint8_t* out_pub, uint8_t* status, uint8_t* out_addr = nullptr) {
+                      uint8_t* out_pub, uint8_t* status, uint8_t* out_addr = nullptr, const void* gtab = nullptr) {
   if (!c) return SIGOPS_E_ARG;
   c->err.clear();
   if (n == 0) return SIGOPS_OK;
@@ -128,7 +165,7 @@ int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8
   HIPCHK(c, hipMemcpyAsync(c->aux.p, digs, n * 32, hipMemcpyHostToDevice, s));
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
-  if ((rc = secp_recover_dev(c, curve, c->in.p, c->aux.p, n, c->out.p, c->st.p, s))) return rc;
+  if ((rc = secp_recover_dev(c, curve, c->in.p, c->aux.p, n, c->out.p, c->st.p, s, gtab))) return rc;
   uint8_t* st = status;
   if (!st) {
     c->host_status.resize(n);
@@ -258,6 +295,10 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
     return bad(e, "hipMalloc batch order");
   if ((e = hipMalloc((void**)&c->d_n, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc n");
   if ((e = hipMalloc((void**)&c->d_hq, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc hash queue");
+  if ((e = hipMalloc((void**)&c->d_tlimbs, TABLE_LIMBS_MAX * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&c->d_ctab, sizeof(sigops::ED_BTABLE))) != hipSuccess ||
+      (e = hipMalloc((void**)&c->d_tflags, sizeof(uint32_t))) != hipSuccess)
+    return bad(e, "hipMalloc caller tables");
   if ((e = hipMalloc(&c->d_kbuf, (size_t)c->max_batch * 32)) != hipSuccess) return bad(e, "hipMalloc challenges");
   return c;
 }
@@ -271,7 +312,8 @@ void sigops_destroy(sigops_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
                   c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
-                  (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, c->d_kbuf})
+                  (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, (void*)c->d_tlimbs, c->d_ctab,
+                  (void*)c->d_tflags, c->d_kbuf})
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -323,14 +365,17 @@ int sigops_secp256r1_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d
   return secp_verify_dev(c, 1, d_sigs, d_digests, d_pubkeys, n, d_status, (hipStream_t)stream);
 }
 
-// Ed25519 verification, dalek `verify` rules or (strict) `verify_strict` rules
+// Ed25519 verification, dalek `verify` rules or (strict) `verify_strict` rules; btab: the
+// fixed-base table (NULL: the resident table of B)
 static int ed_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
-                         const uint64_t* d_msg_off, size_t n, void* d_status, void* stream, bool strict) {
+                         const uint64_t* d_msg_off, size_t n, void* d_status, void* stream, bool strict,
+                         const void* btab = nullptr) {
   if (!c) return SIGOPS_E_ARG;
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
-  HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_msgs, d_msg_off, d_status, (uint32_t)n, c->d_btab,
-                                          c->d_lane_tab, c->slots, c->d_kbuf, c->grid, (hipStream_t)stream, strict));
+  HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_msgs, d_msg_off, d_status, (uint32_t)n,
+                                          btab ? btab : c->d_btab, c->d_lane_tab, c->slots, c->d_kbuf, c->grid,
+                                          (hipStream_t)stream, strict));
   return SIGOPS_OK;
 }
 
@@ -345,7 +390,8 @@ int sigops_ed25519_verify_strict_dev(sigops_ctx* c, const void* d_sigs, const vo
 }
 
 static int ed_verify_host(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
-                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status, bool strict) {
+                          const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status, bool strict,
+                          const void* btab = nullptr) {
   if (!c) return SIGOPS_E_ARG;
   c->err.clear();
   if (n == 0) return SIGOPS_OK;
@@ -371,7 +417,7 @@ static int ed_verify_host(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
   HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
-  if ((rc = ed_verify_dev(c, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->st.p, s, strict)))
+  if ((rc = ed_verify_dev(c, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->st.p, s, strict, btab)))
     return rc;
   uint8_t* st = status;
   if (!st) {
@@ -396,6 +442,39 @@ int sigops_ed25519_verify(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
 int sigops_ed25519_verify_strict(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
                                  const uint64_t* msg_off, size_t n, uint8_t* valid, uint8_t* status) {
   return ed_verify_host(c, sigs, pubkeys, msgs, msg_off, n, valid, status, true);
+}
+
+int sigops_ed25519_verify_table(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pubkeys, const uint8_t* msgs,
+                                const uint64_t* msg_off, size_t n, const uint32_t* table_limbs, size_t table_len,
+                                uint32_t log_limb_size, uint8_t* valid, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  int rc;
+  if (n == 0) return SIGOPS_OK;  // nothing to do, as the reference returns early (src/secp256k1_ecdsa.rs:71-73)
+  if ((rc = load_caller_table(c, 2, table_limbs, table_len, log_limb_size))) return rc;
+  return ed_verify_host(c, sigs, pubkeys, msgs, msg_off, n, valid, status, false, c->d_ctab);
+}
+
+int sigops_secp256k1_recover_table(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                   const uint32_t* table_limbs, size_t table_len, uint32_t log_limb_size,
+                                   uint8_t* out_pub, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  int rc;
+  if (n == 0) return SIGOPS_OK;  // nothing to do, as the reference returns early (src/secp256k1_ecdsa.rs:71-73)
+  if ((rc = load_caller_table(c, 0, table_limbs, table_len, log_limb_size))) return rc;
+  return secp_recover_host(c, 0, sigs, digests, n, out_pub, status, nullptr, c->d_ctab);
+}
+
+int sigops_secp256r1_recover_table(sigops_ctx* c, const uint8_t* sigs, const uint8_t* digests, size_t n,
+                                   const uint32_t* table_limbs, size_t table_len, uint32_t log_limb_size,
+                                   uint8_t* out_pub, uint8_t* status) {
+  if (!c) return SIGOPS_E_ARG;
+  c->err.clear();
+  int rc;
+  if (n == 0) return SIGOPS_OK;  // nothing to do, as the reference returns early (src/secp256k1_ecdsa.rs:71-73)
+  if ((rc = load_caller_table(c, 1, table_limbs, table_len, log_limb_size))) return rc;
+  return secp_recover_host(c, 1, sigs, digests, n, out_pub, status, nullptr, c->d_ctab);
 }
 
 int sigops_gen_secp_dev(sigops_ctx* c, int curve, uint64_t seed, size_t n, uint32_t corrupt_ppm, void* d_sigs,

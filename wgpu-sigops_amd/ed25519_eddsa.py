@@ -16,7 +16,7 @@ from . import precompute
 from ._secp_common import _as_bytes_array
 
 
-def _verify(signatures, messages, verifying_keys, ctx, fixed_len, strict=False):
+def _verify(signatures, messages, verifying_keys, ctx, fixed_len, strict=False, table=None):
     from . import default_context
 
     if len(signatures) != len(messages):
@@ -38,13 +38,16 @@ def _verify(signatures, messages, verifying_keys, ctx, fixed_len, strict=False):
         off[1:] = np.cumsum([len(p) for p in parts])
         msgs = np.frombuffer(b"".join(parts), dtype=np.uint8)
     ctx = ctx or default_context()
-    valid, _ = (ctx.ed25519_verify_strict if strict else ctx.ed25519_verify)(sigs, pks, msgs, off)
+    if table is not None:
+        valid, _ = ctx.ed25519_verify_table(sigs, pks, msgs, off, table[0], table[1])
+    else:
+        valid, _ = (ctx.ed25519_verify_strict if strict else ctx.ed25519_verify)(sigs, pks, msgs, off)
     return [bool(v) for v in valid]
 
 
 def ecverify(signatures, messages, verifying_keys, table_limbs, log_limb_size, ctx=None):
     precompute.check_table(table_limbs, 3, log_limb_size)
-    return _verify(signatures, messages, verifying_keys, ctx, True)
+    return _verify(signatures, messages, verifying_keys, ctx, True, table=(table_limbs, log_limb_size))
 
 
 def ecverify_single(signatures, messages, verifying_keys, log_limb_size, ctx=None):

@@ -1,9 +1,11 @@
 """Generator tables in the reference's layout (src/precompute.rs:12-69).
 
 The reference uploads these tables with every call (``table_limbs`` argument of ``ecrecover`` /
-``ecverify``). This engine keeps its own wider tables resident in HBM, so the mirrored functions
-only check a caller's table for shape; these builders exist so that code written against the
-reference keeps working unchanged. Layout: [G, 2G, ..., 16G] (precompute_table,
+``ecverify``) and uses them as the fixed base. The mirrored functions pass a caller's table to the
+library, which converts it on the device into its own wider odd-multiple table (the resident
+tables of G and B are what the reference's own tables convert to) and refuses a table that is not
+[P, 2P, ..., 16P] for a curve point P. ``secp_table_for_point`` / ``ed25519_table_for_point``
+build the layout for any point. Layout: [G, 2G, ..., 16G] (precompute_table,
 src/curve_algos/precompute.rs:3-17), affine, each coordinate times R = 2^(num_limbs*log_limb_size)
 mod p, split into little-endian limbs of log_limb_size bits (src/tests/mod.rs:94-149); secp tables
 hold (x, y), the Ed25519 table (x, y, x*y).
@@ -73,12 +75,21 @@ def _ed_base():
     return (_P_ED - x if x & 1 else x, y)
 
 
+def secp_table_for_point(curve, P, log_limb_size):
+    """the reference's table layout for the multiples of an arbitrary curve point P = (x, y)"""
+    return _secp_table(P, (_P_K1, _P_R1)[curve], (0, -3)[curve], log_limb_size)
+
+
 def ed25519_bases(log_limb_size):
+    return ed25519_table_for_point(_ed_base(), log_limb_size)
+
+
+def ed25519_table_for_point(B, log_limb_size):
+    """the reference's Ed25519 table layout (x, y, x*y per entry) for the multiples of B = (x, y)"""
     check_log_limb_size(log_limb_size)
     n = calc_num_limbs(log_limb_size)
     r = 2 ** (n * log_limb_size)
     p = _P_ED
-    B = _ed_base()
     out, cur = [], B
     for i in range(1 << WINDOW_SIZE):
         if i:

@@ -18,7 +18,7 @@ CURVE = 0
 
 def ecrecover(signatures, messages, table_limbs, log_limb_size, ctx=None):
     precompute.check_table(table_limbs, 2, log_limb_size)
-    return ecrecover_impl(CURVE, signatures, messages, ctx)[0]
+    return ecrecover_impl(CURVE, signatures, messages, ctx, table=(table_limbs, log_limb_size))[0]
 
 
 def ecrecover_single_shader(signatures, messages, log_limb_size, ctx=None):
