@@ -100,11 +100,13 @@ def fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, steps, dev):
 def predicates(ctx, mx, stream, s_ptr, steps, dev):
     """The predicates next to the path on the batch's own items, each scheme timed alone with events
     on the launch stream: secp verification with the generator's public keys (fuel
-    Signature::verify) and Ed25519 verify_strict. Every uncorrupted item must come out valid."""
+    Signature::verify) and Ed25519 verify_strict, with plain Ed25519 verification of the same items
+    beside it. Every uncorrupted item must come out valid."""
     import torch
     k1, r1, ed = mx.parts
     res = {}
-    for name, it in (("secp256k1_verify", k1), ("secp256r1_verify", r1), ("ed25519_verify_strict", ed)):
+    for name, it in (("secp256k1_verify", k1), ("secp256r1_verify", r1), ("ed25519_verify", ed),
+                     ("ed25519_verify_strict", ed)):
         st = torch.empty(it.n, dtype=torch.uint8, device=dev)
 
         def run():
@@ -112,8 +114,9 @@ def predicates(ctx, mx, stream, s_ptr, steps, dev):
                 fn = ctx.secp256k1_verify_dev if it is k1 else ctx.secp256r1_verify_dev
                 fn(it.sigs.data_ptr(), it.aux.data_ptr(), it.pubs.data_ptr(), it.n, st.data_ptr(), s_ptr)
             else:
-                ctx.ed25519_verify_strict_dev(it.sigs.data_ptr(), it.aux.data_ptr(), it.msgs.data_ptr(),
-                                              it.msg_off.data_ptr(), it.n, st.data_ptr(), s_ptr)
+                fn = ctx.ed25519_verify_strict_dev if name.endswith("strict") else ctx.ed25519_verify_dev
+                fn(it.sigs.data_ptr(), it.aux.data_ptr(), it.msgs.data_ptr(), it.msg_off.data_ptr(), it.n,
+                   st.data_ptr(), s_ptr)
 
         run()
         torch.cuda.synchronize()
