@@ -171,6 +171,13 @@ DEV bool ed_small_order(const fe& x, const fe& y) {
   return fis_zero<Fe>(P.X) && feq<Fe>(P.Y, P.Z);
 }
 
+// k = the 512-bit little-endian digest h (16 words) mod l, as 8 little-endian words
+DEV void ed25519_k_from_digest(const u32 h[16], u32 k[8]) {
+  const fe hi = mont_mul<Fl>(fe_from_words(h + 8), fe_from_const(L_C256));
+  const fe lo = mont_mul<Fl>(fe_from_words(h), fe_from_const(L_C0));
+  fe_to_words(fcanon<Fl>(fadd<Fl>(hi, lo)), k);
+}
+
 // sig_raw: 16 raw words (R || s); pk_raw: 8 raw words; message in mem at msg_off, len bytes.
 // k = SHA-512(R || A || M) mod l as 8 little-endian words
 template <class Mem>
@@ -183,15 +190,13 @@ DEV void ed25519_challenge(const u32 sig_raw[16], const u32 pk_raw[8], const Mem
     hdr[8 + i] = pk_raw[i];
   }
   sha512_ed(hdr, mem, msg_off, len, h);
-  const fe hi = mont_mul<Fl>(fe_from_words(h + 8), fe_from_const(L_C256));
-  const fe lo = mont_mul<Fl>(fe_from_words(h), fe_from_const(L_C0));
-  fe_to_words(fcanon<Fl>(fadd<Fl>(hi, lo)), k);
+  ed25519_k_from_digest(h, k);
 }
 
 // verify_strict's extra rules (dalek VerifyingKey::verify_strict): R must decompress and neither R
-// nor A may have small order. Returns ED_OK, ED_BAD_R or ED_SMALL_ORDER. They are evaluated in
-// the hash pass, which has registers to spare, and travel to the verifier in bits 29..31 of the
-// challenge's top word: k < l < 2^253 leaves those bits zero.
+// nor A may have small order. Returns ED_OK, ED_BAD_R or ED_SMALL_ORDER. They are evaluated in a
+// pass of their own between the hash and the verifier (registers to spare) and travel to the
+// verifier in bits 29..31 of the challenge's top word: k < l < 2^253 leaves those bits zero.
 DEV u32 ed25519_strict_checks(const u32 sig_raw[16], const u32 pk_raw[8]) {
   fe ax, ay, rx, ry;
   (void)ed_decompress(pk_raw, ax, ay);  // an undecodable A is rejected by the verifier first

@@ -38,8 +38,12 @@ struct BatchArgs {
   const uint4* kbuf;  // Ed25519 challenges, written by the hash pass
 };
 
+// scheme == NULL: every item is an Ed25519 verification (the plain Ed25519 path hashes through
+// the same bucketing, so that its wavefronts hash equally long messages)
+DEV u32 item_scheme(const BatchArgs& a, u32 i) { return a.scheme ? a.scheme[i] : 2u; }
+
 DEV u32 item_bucket(const BatchArgs& a, u32 i) {
-  const u32 s = a.scheme[i];
+  const u32 s = item_scheme(a, i);
   if (s == 0) return BUCKET_K1;
   if (s == 1) return BUCKET_R1;
   const u64 len = a.msg_off[i + 1] - a.msg_off[i];
@@ -54,7 +58,7 @@ __global__ __launch_bounds__(256) void k_bucket_hist(BatchArgs a) {
   __syncthreads();
   const u32 n = *a.n_ptr;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (a.scheme[i] > 2) continue;
+    if (item_scheme(a, i) > 2) continue;
     atomicAdd(&h[item_bucket(a, i)], 1u);
   }
   __syncthreads();
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
     const u32 i = c0 + threadIdx.x;
     u32 b = NBUCKETS, rank = 0;
     if (i < n) {
-      if (a.scheme[i] > 2) {
+      if (item_scheme(a, i) > 2) {
         a.status[i] = 0xFD;  // unknown scheme: reported as a canary failure by the host
       } else {
         b = item_bucket(a, i);
@@ -122,7 +126,7 @@ __global__ __launch_bounds__(256) void k_batch_hash(BatchArgs a) {
   const u32 total = a.tile_start[NBUCKETS] * a.tile_items;
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
     const u32 i = a.order[j];
-    if (i == DUMMY || a.scheme[i] != 2) continue;
+    if (i == DUMMY || item_scheme(a, i) != 2) continue;
     u32 sr[16], pr[8], k[8];
     load_words(a.sigs + 4 * (u64)i, sr, 4);
     load_words(a.aux + 2 * (u64)i, pr, 2);
@@ -251,7 +255,7 @@ static __global__ void k_zero_u32(u32* p, u32 n) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
+static BatchArgs batch_args(const BatchLaunch& L) {
   BatchArgs a;
   a.scheme = (const u8*)L.scheme;
   a.sigs = (const uint4*)L.sigs;
@@ -271,6 +275,23 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   a.rank = L.rank;
   a.tile_items = 64u * L.tile_rounds;
   a.kbuf = (const uint4*)L.kbuf;
+  return a;
+}
+
+// Ed25519 challenges of a plain (all-Ed25519, L.scheme == NULL) batch in bucket order: the
+// bucketing and hash passes of launch_batch without the persistent kernel
+hipError_t launch_ed_hash_bucketed(const BatchLaunch& L, hipStream_t s) {
+  const BatchArgs a = batch_args(L);
+  k_zero_u32<<<1, 256, 0, s>>>(L.work, BATCH_WORK_WORDS);
+  k_bucket_hist<<<L.aux_grid, 256, 0, s>>>(a);
+  k_bucket_scan<<<1, 64, 0, s>>>(a);
+  k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
+  k_batch_hash<<<L.grid, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
+  const BatchArgs a = batch_args(L);
   // stage events exist only on the eager (timed) path, never under graph capture
   auto mark = [&](int i) -> hipError_t { return L.stage_ev ? hipEventRecord(L.stage_ev[i], s) : hipSuccess; };
   hipError_t e;

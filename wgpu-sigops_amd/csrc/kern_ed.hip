@@ -1,26 +1,23 @@
-// kern_ed.hip -- instantiates the Ed25519 verification kernel and provides its launch helper.
+// kern_ed.hip -- instantiates the Ed25519 verification kernels and provides their launch helper
+// (the challenges come from the bucketed hash pass of kern_batch.hip).
 #include "kernels_impl.hpp"
 
 namespace sigops {
 
-hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* msgs, const u64* msg_off,
-                                 void* status, u32 n, const void* btab, void* ctab, u32 slots, void* kbuf, u32 grid,
-                                 hipStream_t s, bool strict) {
+// the challenges are in kbuf already (launch_ed_hash_bucketed); strict adds verify_strict's pass
+hipError_t launch_ed25519_verify(const void* sigs, const void* pks, void* status, u32 n, const void* btab, void* ctab,
+                                 u32 slots, void* kbuf, u32 grid, hipStream_t s, bool strict) {
   if (n == 0) return hipSuccess;
   const u32 want = (n + 255) / 256;
   const u32 g = want < grid ? want : grid;
-  if (strict)
-    k_ed25519_hash<true><<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, (const u32*)msgs, msg_off, nullptr,
-                                          nullptr, n, (uint4*)kbuf);
-  else
-    k_ed25519_hash<false><<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, (const u32*)msgs, msg_off, nullptr,
-                                           nullptr, n, (uint4*)kbuf);
-  if (strict)
+  if (strict) {
+    k_ed25519_strict<<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, n, (uint4*)kbuf);
     k_ed25519_verify<true><<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, (const uint4*)kbuf, (u8*)status,
                                             n, (const uint4*)btab, (uint4*)ctab, slots);
-  else
+  } else {
     k_ed25519_verify<false><<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, (const uint4*)kbuf, (u8*)status,
                                              n, (const uint4*)btab, (uint4*)ctab, slots);
+  }
   return hipGetLastError();
 }
 

@@ -27,10 +27,11 @@ hipError_t launch_secp_verify_CurveR1(const void* sigs, const void* digs, const 
                                      const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
                                      uint32_t* pend_count, uint32_t grid, hipStream_t s);
 
+// Ed25519 verification given the challenges in kbuf (launch_ed_hash_bucketed writes them);
 // strict: ed25519-dalek verify_strict instead of verify
-hipError_t launch_ed25519_verify(const void* sigs, const void* pks, const void* msgs, const uint64_t* msg_off,
-                                 void* status, uint32_t n, const void* btab, void* ctab, uint32_t slots, void* kbuf,
-                                 uint32_t grid, hipStream_t s, bool strict = false);
+hipError_t launch_ed25519_verify(const void* sigs, const void* pks, void* status, uint32_t n, const void* btab,
+                                 void* ctab, uint32_t slots, void* kbuf, uint32_t grid, hipStream_t s,
+                                 bool strict = false);
 
 // ---- mixed batch ----
 constexpr uint32_t BUCKET_K1 = 0, BUCKET_R1 = 1, BUCKET_ED0 = 2;
@@ -62,6 +63,8 @@ struct BatchLaunch {
 };
 
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
+// bucketing and hash passes only, for an all-Ed25519 batch (L.scheme == NULL): challenges into L.kbuf
+hipError_t launch_ed_hash_bucketed(const BatchLaunch& L, hipStream_t s);
 
 // SHA-256: digests of messages (Fuel Message::new) and addresses of 64-byte public keys
 // queue: one device word the message kernel uses as its work counter (cleared by the launch)

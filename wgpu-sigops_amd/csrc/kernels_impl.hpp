@@ -266,30 +266,18 @@ __global__ __launch_bounds__(64) void k_secp_verify_careful(const uint4* __restr
   }
 }
 
-// k = SHA-512(R || A || M) mod l per item; a pass of its own so that the verifier's register
-// budget carries no hash state (the reference also hashes in a separate stage,
-// src/wgsl/main/ed25519_eddsa_main_1.wgsl). `order` (may be NULL) lists the items to hash.
-// STRICT: also evaluates verify_strict's extra rules into the top bits of k (ed25519_strict_checks)
-template <bool STRICT>
-__global__ __launch_bounds__(256) void k_ed25519_hash(const uint4* __restrict__ sigs,
-                                                             const uint4* __restrict__ pks,
-                                                             const u32* __restrict__ msgs,
-                                                             const u64* __restrict__ msg_off,
-                                                             const u32* __restrict__ order, const u32* n_ptr, u32 n_arg,
-                                                             uint4* __restrict__ kbuf) {
-  const MemDev mem{msgs};
-  const u32 n = n_ptr ? *n_ptr : n_arg;
-  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const u32 i = order ? order[j] : j;
-    if (i == 0xFFFFFFFFu) continue;
-    u32 sr[16], pr[8], k[8];
+// verify_strict: the extra rules of every item into the top bits of its k (ed25519_strict_checks)
+static __global__ __launch_bounds__(256) void k_ed25519_strict(const uint4* __restrict__ sigs,
+                                                               const uint4* __restrict__ pks, u32 n,
+                                                               uint4* __restrict__ kbuf) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    u32 sr[16], pr[8];
     load_words(sigs + 4 * (u64)i, sr, 4);
     load_words(pks + 2 * (u64)i, pr, 2);
-    const u64 off = msg_off[i];
-    ed25519_challenge(sr, pr, mem, off, (u32)(msg_off[i + 1] - off), k);
-    if constexpr (STRICT) k[7] |= ed25519_strict_checks(sr, pr) << ED_K_STRICT_SHIFT;
-    kbuf[2 * (u64)i] = make_uint4(k[0], k[1], k[2], k[3]);
-    kbuf[2 * (u64)i + 1] = make_uint4(k[4], k[5], k[6], k[7]);
+    const u32 st = ed25519_strict_checks(sr, pr);
+    uint4 top = kbuf[2 * (u64)i + 1];
+    top.w |= st << ED_K_STRICT_SHIFT;
+    kbuf[2 * (u64)i + 1] = top;
   }
 }
 

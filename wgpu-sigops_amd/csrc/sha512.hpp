@@ -4,8 +4,8 @@
 // hard-coded at :122-123) because its messages are fixed 32-byte digests; here M has any length,
 // so the padded preimage is generated word by word: 64 header bytes (R || A) from registers, the
 // message from memory, then 0x80, zeros and the 128-bit bit length. Lanes of a wavefront with
-// fewer blocks go idle while the longest message finishes, which is why the batched kernel groups
-// Ed25519 items by block count.
+// fewer blocks would go idle while the longest message finishes, which is why the kernels hash
+// Ed25519 items grouped by block count.
 #pragma once
 #include "field.hpp"
 
@@ -54,6 +54,15 @@ DEV void sha512_compress(u64 h[8], u64 w[16]) {
 
 // 8 little-endian bytes at byte offset off of a 4-byte-aligned buffer (reads 12 bytes starting at
 // the aligned word below off; the buffer carries >= 16 bytes of slack after the last message).
+DEV void sha512_out(const u64 h[8], u32 out[16]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u64 le = bswap64(h[i]);
+    out[2 * i] = (u32)le;
+    out[2 * i + 1] = (u32)(le >> 32);
+  }
+}
+
 template <class Mem>
 DEV u64 load8_le(const Mem& mem, u64 off) {
   const u64 wa = off >> 2;
@@ -64,6 +73,38 @@ DEV u64 load8_le(const Mem& mem, u64 off) {
   return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
 }
 
+// blocks of the padded preimage of total bytes: the bytes, 0x80, zeros, the 128-bit bit length
+DEV u32 sha512_nblocks(u64 total) { return (u32)((total + 1 + 16 + 127) >> 7); }
+
+// Schedule words 0..15 of block b (of nblk) of the padded preimage hdr[0..HL) || msg[0..len),
+// hdr given as 16 raw u32 words (only the first HL/4 are used), msg at byte offset msg_off.
+template <int HL, class Mem>
+DEV void sha512_block_words(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 b, u32 nblk, u64 w[16]) {
+  const u64 total = (u64)HL + len;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const u64 o = (u64)b * 128 + 8 * q;  // byte offset in the preimage
+    u64 word;
+    if (o < (u64)HL) {
+      const int hq = (int)(o >> 3) & 7;
+      word = bswap64((u64)hdr[2 * hq] | ((u64)hdr[2 * hq + 1] << 32));
+    } else {
+      const u64 mo = o - HL;
+      u64 le = 0;
+      if (mo < len) le = load8_le(mem, msg_off + mo);
+      const i64 keep = (i64)len - (i64)mo;  // message bytes in this word
+      if (keep < 8) {
+        const u64 m = keep <= 0 ? 0ull : ((1ull << (8 * keep)) - 1ull);
+        le &= m;
+        if (keep >= 0) le |= 0x80ull << (8 * keep);
+      }
+      word = bswap64(le);
+      if (b == nblk - 1 && q == 15) word = total * 8ull;
+    }
+    w[q] = word;
+  }
+}
+
 // SHA-512(hdr[0..HL) || msg[0..len)) for a header of HL = 32 or 64 bytes, given as 16 raw u32
 // words (only the first HL/4 are used). Output: the 64 digest bytes as 16 little-endian u32 words.
 template <int HL, class Mem>
@@ -72,40 +113,13 @@ DEV void sha512_hdr(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32
   u64 h[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) h[i] = SHA512_IV[i];
-  const u64 total = (u64)HL + len;
-  const u32 nblk = (u32)((total + 1 + 16 + 127) >> 7);
+  const u32 nblk = sha512_nblocks((u64)HL + len);
   for (u32 b = 0; b < nblk; b++) {
     u64 w[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const u64 o = (u64)b * 128 + 8 * q;  // byte offset in the preimage
-      u64 word;
-      if (o < (u64)HL) {
-        const int hq = (int)(o >> 3);
-        word = bswap64((u64)hdr[2 * hq] | ((u64)hdr[2 * hq + 1] << 32));
-      } else {
-        const u64 mo = o - HL;
-        u64 le = 0;
-        if (mo < len) le = load8_le(mem, msg_off + mo);
-        const i64 keep = (i64)len - (i64)mo;  // message bytes in this word
-        if (keep < 8) {
-          const u64 m = keep <= 0 ? 0ull : ((1ull << (8 * keep)) - 1ull);
-          le &= m;
-          if (keep >= 0) le |= 0x80ull << (8 * keep);
-        }
-        word = bswap64(le);
-        if (b == nblk - 1 && q == 15) word = total * 8ull;
-      }
-      w[q] = word;
-    }
+    sha512_block_words<HL>(hdr, mem, msg_off, len, b, nblk, w);
     sha512_compress(h, w);
   }
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const u64 le = bswap64(h[i]);
-    out[2 * i] = (u32)le;
-    out[2 * i + 1] = (u32)(le >> 32);
-  }
+  sha512_out(h, out);
 }
 
 // SHA-512(R || A || M): hdr = R then A (16 raw words)

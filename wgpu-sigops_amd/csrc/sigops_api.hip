@@ -373,9 +373,29 @@ static int ed_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkey
   if (!c) return SIGOPS_E_ARG;
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
-  HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_msgs, d_msg_off, d_status, (uint32_t)n,
-                                          btab ? btab : c->d_btab, c->d_lane_tab, c->slots, c->d_kbuf, c->grid,
-                                          (hipStream_t)stream, strict));
+  hipStream_t s = (hipStream_t)stream;
+  // challenges through the batch's bucketing (all items Ed25519): each wavefront hashes messages
+  // of one SHA-512 block count, instead of waiting for the longest message among its 64 lanes
+  sigops::BatchLaunch L{};
+  L.scheme = nullptr;
+  L.sigs = d_sigs;
+  L.aux = d_pubkeys;
+  L.msgs = d_msgs;
+  L.msg_off = d_msg_off;
+  L.status = d_status;
+  L.n_ptr = c->d_n;
+  L.work = c->d_work;
+  L.order = c->d_order;
+  L.rank = c->d_rank;
+  L.kbuf = c->d_kbuf;
+  L.grid = c->grid;
+  L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
+  L.tile_rounds = 1;
+  k_set_u32<<<1, 1, 0, s>>>(c->d_n, (uint32_t)n);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, sigops::launch_ed_hash_bucketed(L, s));
+  HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_status, (uint32_t)n, btab ? btab : c->d_btab,
+                                          c->d_lane_tab, c->slots, c->d_kbuf, c->grid, s, strict));
   return SIGOPS_OK;
 }
 
