@@ -193,11 +193,13 @@ DEV constexpr u64 col_init(int k) {
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// Column offsets read from constant memory: the compiler cannot fold a load into the code, so each
-// offset stays a 64-bit scalar operand of its column's first multiply-add instead of becoming a
-// separate 64-bit add after it (16 fewer instructions per secp256k1 / 2^255-19 product).
-static __constant__ u64 g_col_offsets[3] = {COL_E, COL_E - (COL_E >> SIGOPS_LIMB_BITS),
-                                            (u64)0 - (COL_E >> SIGOPS_LIMB_BITS)};
+// Column offsets read from constant memory: the compiler does not fold the load of an externally
+// visible object, so each offset stays a 64-bit scalar operand of its column's first multiply-add
+// instead of becoming a separate 64-bit add after it (16 fewer instructions per secp256k1 /
+// 2^255-19 product). With internal linkage (`static`) the compiler proves the array constant and
+// folds it back into immediates, so it stays extern; each code object holds its own copy.
+__constant__ u64 g_col_offsets[3] = {COL_E, COL_E - (COL_E >> SIGOPS_LIMB_BITS),
+                                     (u64)0 - (COL_E >> SIGOPS_LIMB_BITS)};
 #endif
 
 template <class F>

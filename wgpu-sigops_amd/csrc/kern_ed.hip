@@ -4,6 +4,21 @@
 
 namespace sigops {
 
+// verify_strict: the extra rules of every item into the top bits of its k (ed25519_strict_checks)
+static __global__ __launch_bounds__(256) void k_ed25519_strict(const uint4* __restrict__ sigs,
+                                                               const uint4* __restrict__ pks, u32 n,
+                                                               uint4* __restrict__ kbuf) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    u32 sr[16], pr[8];
+    load_words(sigs + 4 * (u64)i, sr, 4);
+    load_words(pks + 2 * (u64)i, pr, 2);
+    const u32 st = ed25519_strict_checks(sr, pr);
+    uint4 top = kbuf[2 * (u64)i + 1];
+    top.w |= st << ED_K_STRICT_SHIFT;
+    kbuf[2 * (u64)i + 1] = top;
+  }
+}
+
 // the challenges are in kbuf already (launch_ed_hash_bucketed); strict adds verify_strict's pass
 hipError_t launch_ed25519_verify(const void* sigs, const void* pks, void* status, u32 n, const void* btab, void* ctab,
                                  u32 slots, void* kbuf, u32 grid, hipStream_t s, bool strict) {
