@@ -48,10 +48,14 @@ struct BTabHost {
     canonical_bounds<ModPed>(t2d);
   }
 };
-struct JTabHost {
+struct STabHost {
   jac t[8];
-  void store(int e, const jac& p) { t[e] = p; }
-  jac load(u32 e) const { return t[e]; }
+  fe before[8];
+  fe ax[8], ay[8];
+  void store_full(int e, const jac& p, const fe& b) { t[e] = p; before[e] = b; }
+  void load_full(u32 e, jac& p, fe& b) const { p = t[e]; b = before[e]; }
+  void store_affine(int e, const fe& x, const fe& y) { ax[e] = x; ay[e] = y; }
+  void load_affine(u32 e, fe& x, fe& y) const { x = ax[e]; y = ay[e]; }
 };
 struct CTabHost {
   cached t[8];
@@ -75,7 +79,7 @@ int recover_one(const unsigned char* sig, const unsigned char* dig, unsigned cha
   memset(out, 0, 64);
   if (pro.status != ST_OK) return (int)pro.status;
   bool exc = false;
-  JTabHost tab;
+  STabHost tab;
   jac Q = secp_ladder<C>(pro, tab, gt, exc);
   int careful = exc || force_careful;
   if (!careful) {
@@ -101,7 +105,7 @@ int verify_one(const unsigned char* sig, const unsigned char* dig, const unsigne
   GTabHost<typename C::Fp> gt{gtab};
   if (pro.status != ST_OK) return (int)pro.status;
   bool exc = false;
-  JTabHost tab;
+  STabHost tab;
   jac P = secp_ladder<C>(pro, tab, gt, exc);
   if (!exc && !force_careful) return (int)secp_verify_x<C>(P, sr);
   jacf Pc = secp_ladder_careful<C>(pro, gt);

@@ -144,7 +144,7 @@ DEV void store16(uint4* p, const u32 o[16]) {
 }
 
 template <class C, class GT>
-DEV void batch_secp_item(const BatchArgs& a, u32 i, JTabDev& tab, const GT& gt) {
+DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) {
   u32 sr[16], dr[8], o[16];
   load_words(a.sigs + 4 * (u64)i, sr, 4);
   load_words(a.aux + 2 * (u64)i, dr, 2);
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArg
   __syncthreads();
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 lane = threadIdx.x & 63u;
-  JTabDev jt;
+  STabDev jt;
   jt.base = lane_tab;
   jt.slots = slots;
   jt.slot = slot;

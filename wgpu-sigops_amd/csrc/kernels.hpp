@@ -9,7 +9,7 @@ typedef uint8_t u8;
 
 constexpr uint32_t GTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of G
 constexpr uint32_t BTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of B
-constexpr uint32_t SECP_LANE_TAB_BYTES = 8 * 7 * 16;  // 8 Jacobian entries x 7 uint4
+constexpr uint32_t SECP_LANE_TAB_BYTES = 8 * 9 * 16;  // 8 entries x 9 uint4 (Jacobian + Z product, then affine)
 constexpr uint32_t ED_LANE_TAB_BYTES = 8 * 9 * 16;    // 8 cached entries x 9 uint4
 
 hipError_t launch_secp_recover_CurveK1(const void* sigs, const void* digs, void* out, void* status, uint32_t n,

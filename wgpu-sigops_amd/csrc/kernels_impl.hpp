@@ -63,30 +63,56 @@ struct LaneTab {
     for (int c = 0; c < CH; c++)
       base[((u64)e * CH + c) * slots + slot] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
   }
-  DEV void get(u32 e, u32* w) const {
+  DEV void get(u32 e, u32* w) const { get_n<CH>(e, w); }
+  // the first N chunks of entry e
+  template <int N>
+  DEV void put_n(int e, const u32* w) {
 #pragma unroll
-    for (int c = 0; c < CH; c++) {
+    for (int c = 0; c < N; c++)
+      base[((u64)e * CH + c) * slots + slot] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
+  }
+  template <int N>
+  DEV void get_n(u32 e, u32* w) const {
+#pragma unroll
+    for (int c = 0; c < N; c++) {
       const uint4 q = base[((u64)e * CH + c) * slots + slot];
       w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
     }
   }
 };
 
-struct JTabDev : LaneTab<7> {  // Jacobian point: 27 words + pad
-  DEV void store(int e, const jac& p) {
-    u32 w[28];
+// secp R-table (secp_build_table): while building, entry e holds a Jacobian point and the running
+// product of the Z coordinates before it (36 words, 9 chunks); afterwards its first 5 chunks hold
+// the affine (x, y) the ladder reads
+struct STabDev : LaneTab<9> {
+  DEV void store_full(int e, const jac& p, const fe& before) {
+    u32 w[36];
 #pragma unroll
-    for (int k = 0; k < NL; k++) { w[k] = p.X.v[k]; w[9 + k] = p.Y.v[k]; w[18 + k] = p.Z.v[k]; }
-    w[27] = 0;
+    for (int k = 0; k < NL; k++) {
+      w[k] = p.X.v[k]; w[9 + k] = p.Y.v[k]; w[18 + k] = p.Z.v[k]; w[27 + k] = before.v[k];
+    }
     put(e, w);
   }
-  DEV jac load(u32 e) const {
-    u32 w[28];
+  DEV void load_full(u32 e, jac& p, fe& before) const {
+    u32 w[36];
     get(e, w);
-    jac p;
 #pragma unroll
-    for (int k = 0; k < NL; k++) { p.X.v[k] = w[k]; p.Y.v[k] = w[9 + k]; p.Z.v[k] = w[18 + k]; }
-    return p;
+    for (int k = 0; k < NL; k++) {
+      p.X.v[k] = w[k]; p.Y.v[k] = w[9 + k]; p.Z.v[k] = w[18 + k]; before.v[k] = w[27 + k];
+    }
+  }
+  DEV void store_affine(int e, const fe& x, const fe& y) {
+    u32 w[20];
+#pragma unroll
+    for (int k = 0; k < NL; k++) { w[k] = x.v[k]; w[9 + k] = y.v[k]; }
+    w[18] = w[19] = 0;
+    put_n<5>(e, w);
+  }
+  DEV void load_affine(u32 e, fe& x, fe& y) const {
+    u32 w[20];
+    get_n<5>(e, w);
+#pragma unroll
+    for (int k = 0; k < NL; k++) { x.v[k] = w[k]; y.v[k] = w[9 + k]; }
   }
 };
 
@@ -141,7 +167,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_recover(const
   __syncthreads();
   const GTabLds gt{sg};
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
-  JTabDev tab;
+  STabDev tab;
   tab.base = rtab;
   tab.slots = slots;
   tab.slot = slot;
@@ -216,7 +242,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_verify(
   __syncthreads();
   const GTabLds gt{sg};
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
-  JTabDev tab;
+  STabDev tab;
   tab.base = rtab;
   tab.slots = slots;
   tab.slot = slot;

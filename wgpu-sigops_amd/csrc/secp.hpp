@@ -11,7 +11,8 @@
 //   * both scalars in the signed all-odd digit form k = sum (2c_i - (2^w-1)) 2^(wi) with
 //     C = (k >> 1) | 2^255 for odd k (even k: use n - k and negate the base), so every window
 //     performs exactly one addition and no lane ever branches on a zero digit;
-//   * u2*R: w = 4, table of odd multiples R..15R (Jacobian) in per-lane memory;
+//   * u2*R: w = 4, table of odd multiples R..15R in per-lane memory, made affine with one
+//     simultaneous inversion so that every addition of the R stream is a mixed addition;
 //   * u1*G: w = 8, table of odd multiples G..255G (affine) staged in LDS by the kernel;
 //   * Jacobian formulas dbl-2009-l (a = 0) / dbl-2001-b (a = -3), add-2007-bl, madd-2007-bl.
 //     These are incomplete: an addition of P and +-P is detected (H == 0) and the lane is marked
@@ -464,13 +465,19 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
   typedef typename C::Fp F;
   const fe beta = fe_from_const(K1_BETA);
   // top windows: every stream's top digit is positive; only the stream signs apply
-  jac acc = tab.load(nibble(o.cr1, 32) - 8);
-  acc.Y = fneg_if<F>(o.sr1, acc.Y);
+  // the accumulator enters jmadd as a subtrahend, so its coordinates are folded to < (1 + 2^-19) M
+  jac acc;
   {
-    jac E = tab.load(nibble(o.cr2, 32) - 8);
-    E.X = mont_mul<F>(E.X, beta);
-    E.Y = fneg_if<F>(o.sr2, E.Y);
-    acc = jadd<C>(acc, E, exc);
+    fe x, y;
+    tab.load_affine(nibble(o.cr1, 32) - 8, x, y);
+    acc.X = fnorm_red<F>(x);
+    acc.Y = fnorm_red<F>(fneg_if<F>(o.sr1, y));
+    acc.Z = fe_from_const(F::ONE);
+  }
+  {
+    fe ex, ey;
+    tab.load_affine(nibble(o.cr2, 32) - 8, ex, ey);
+    acc = jmadd<C>(acc, mont_mul<F>(ex, beta), fneg_if<F>(o.sr2, ey), exc);
   }
 #pragma unroll 1
   for (int i = 32; i >= 0; i--) {
@@ -478,15 +485,13 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
       // the first table load is issued before the doublings and the second before the first
       // addition; each is hidden behind that work without keeping both entries live
       const u32 c1 = nibble(o.cr1, i), c2 = nibble(o.cr2, i);
-      jac E1 = tab.load((c1 < 8 ? ~c1 : c1) & 7u);
+      fe x1, y1, x2, y2;
+      tab.load_affine((c1 < 8 ? ~c1 : c1) & 7u, x1, y1);
 #pragma unroll 1
       for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
-      jac E2 = tab.load((c2 < 8 ? ~c2 : c2) & 7u);
-      E1.Y = fneg_if<F>((c1 < 8) != o.sr1, E1.Y);
-      acc = jadd<C>(acc, E1, exc);
-      E2.X = mont_mul<F>(E2.X, beta);
-      E2.Y = fneg_if<F>((c2 < 8) != o.sr2, E2.Y);
-      acc = jadd<C>(acc, E2, exc);
+      tab.load_affine((c2 < 8 ? ~c2 : c2) & 7u, x2, y2);
+      acc = jmadd<C>(acc, x1, fneg_if<F>((c1 < 8) != o.sr1, y1), exc);
+      acc = jmadd<C>(acc, mont_mul<F>(x2, beta), fneg_if<F>((c2 < 8) != o.sr2, y2), exc);
     }
     if ((i & 1) == 0) {
       fe gx, gy;
@@ -502,36 +507,70 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
   return acc;
 }
 
-// R-table stored per lane; Tab provides store(e, jac) / load(e) -> jac.
-// GTab provides load(idx, fe& x, fe& y) for the odd multiples (2 idx + 1) G.
-template <class C, class Tab, class GTab>
-DEV jac secp_ladder(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
+// Per-lane table of the odd multiples (2e + 1) P, e = 0..7, made affine with one inversion
+// (Montgomery's simultaneous inversion), so that the ladder adds its entries with the mixed formula
+// (8M + 3S instead of 11M + 5S per addition). The entries are built in Jacobian form with the
+// running product of the Z coordinates before them stored alongside; one inversion of the full
+// product then yields every 1/Z on the way back down, and each entry is rewritten in place as
+// (x, y). Tab: store_full(e, jac, before) / load_full(e, jac&, before&) and store_affine(e, x, y)
+// / load_affine(e, x&, y&).
+template <class C, class Tab>
+DEV void secp_build_table(const secp_pro& o, Tab& tab, bool& exc) {
   typedef typename C::Fp F;
   jac P;
   P.X = o.rx;
   P.Y = o.ry;
   P.Z = fe_from_const(F::ONE);
   const jac P2 = jdbl<C>(P);
-  tab.store(0, P);
+  tab.store_affine(0, P.X, P.Y);
   jac T = P;
+  fe before = P.Z;  // Z_1 * ... * Z_(e-1)
 #pragma unroll 1
   for (int e = 1; e < 8; e++) {
     T = jadd<C>(T, P2, exc);
-    tab.store(e, T);
+    tab.store_full(e, T, before);
+    before = mont_mul<F>(before, T.Z);
   }
+  fe inv = finv<F>(before);  // 1 / (Z_1 * ... * Z_7)
+#pragma unroll 1
+  for (int e = 7; e >= 1; e--) {
+    jac E;
+    fe b;
+    tab.load_full(e, E, b);
+    const fe zi = mont_mul<F>(inv, b);  // 1 / Z_e
+    inv = mont_mul<F>(inv, E.Z);        // 1 / (Z_1 * ... * Z_(e-1))
+    const fe zi2 = mont_sqr<F>(zi);
+    tab.store_affine(e, mont_mul<F>(E.X, zi2), mont_mul<F>(E.Y, mont_mul<F>(zi2, zi)));
+  }
+}
+
+// R-table stored per lane (secp_build_table). GTab provides load(idx, fe& x, fe& y) for the odd
+// multiples (2 idx + 1) G.
+template <class C, class Tab, class GTab>
+DEV jac secp_ladder(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
+  typedef typename C::Fp F;
+  secp_build_table<C>(o, tab, exc);
   if constexpr (C::GLV) return secp_ladder_glv<C>(o, tab, gt, exc);
-  // top R digit is always positive (C's bit 255 is set)
-  jac acc = tab.load(nibble(o.c2, 63) - 8);
+  // top R digit is always positive (C's bit 255 is set); the accumulator enters jmadd as a
+  // subtrahend, so its coordinates are folded to < (1 + 2^-19) M
+  jac acc;
+  {
+    fe x, y;
+    tab.load_affine(nibble(o.c2, 63) - 8, x, y);
+    acc.X = fnorm_red<F>(x);
+    acc.Y = fnorm_red<F>(y);
+    acc.Z = fe_from_const(F::ONE);
+  }
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
     // issue the HBM load of this window's table entry before the doublings, which hide its latency
     const u32 c = nibble(o.c2, i);
     const bool neg = c < 8;
-    jac E = tab.load((neg ? ~c : c) & 7u);
+    fe ex, ey;
+    tab.load_affine((neg ? ~c : c) & 7u, ex, ey);
 #pragma unroll 1
     for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
-    E.Y = fneg_if<F>(neg, E.Y);
-    acc = jadd<C>(acc, E, exc);
+    acc = jmadd<C>(acc, ex, fneg_if<F>(neg, ey), exc);
     if ((i & 1) == 0) {
       const u32 cg = byte8(o.c1, i >> 1);
       const bool gneg = (cg < 128) != o.g_neg;
