@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the Fuel hashing and verification-predicate timings (shorter profiled runs)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--tile-rounds", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
@@ -252,8 +254,10 @@ def main():
     stage_ms = {k: sum(r[k] for r in stage_runs) / len(stage_runs) for k in stage_runs[0]}
     log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
 
-    fuel = fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, args.steps, dev)
-    preds = predicates(ctx, mx, stream, s_ptr, args.steps, dev)
+    fuel = preds = None
+    if not args.no_extras:
+        fuel = fuel_hashing(ctx, n, mx, out, st, stream, s_ptr, args.steps, dev)
+        preds = predicates(ctx, mx, stream, s_ptr, args.steps, dev)
 
     # correctness of the timed output by the round-trip property (all 2^20 items)
     out_h, st_h, cor = out.cpu().numpy(), st.cpu().numpy(), mx.corrupted.cpu().numpy()
