@@ -346,10 +346,12 @@ def main():
         capacity = n / (incl_ms / 1e3)
         with sigops.Server(slots=3) as srv:
             warm = True
-            for load in (0.5, 0.25, 0.5, 0.8):  # the first pass warms up (buffers, graphs) and is dropped
+            # the first pass warms up and is dropped: at the highest load, so that every slot's buffers
+            # reach their largest size (and their graphs are captured) before anything is measured
+            for load in (0.8, 0.25, 0.5, 0.8):
                 rate_items = load * capacity
                 mean_req = (4096 - 1) / np.log(4096)
-                nreq = int(max(2000, (0.3 if warm else args.serve_seconds) * rate_items / mean_req))
+                nreq = int(max(2000, (0.5 if warm else args.serve_seconds) * rate_items / mean_req))
                 sizes = np.floor(np.exp(rng.random(nreq) * np.log(4096))).astype(np.int64).clip(1, 4096)
                 starts = rng.integers(0, n - 4096, nreq)
                 # requests are contiguous slices of the pool; the replay takes one offsets array, so
