@@ -211,10 +211,14 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArg
       } else if (b == BUCKET_R1) {
         batch_secp_item<CurveR1>(a, i, jt, GTabLds{s_r1});
       } else {
+        // the challenge is hashed here, inside the persistent kernel: the tile's items have equal
+        // SHA-512 block counts (bucketed), and the hash's memory latency hides behind the other
+        // waves' field arithmetic instead of idling the GPU in a pass of its own
         u32 sr[16], pr[8], kk[8];
         load_words(a.sigs + 4 * (u64)i, sr, 4);
         load_words(a.aux + 2 * (u64)i, pr, 2);
-        load_words(a.kbuf + 2 * (u64)i, kk, 2);
+        const u64 off = a.msg_off[i];
+        ed25519_challenge(sr, pr, MemDev{a.msgs}, off, (u32)(a.msg_off[i + 1] - off), kk);
         a.status[i] = (u8)ed25519_verify_core(sr, pr, kk, ct, BTabLds{s_b});
         const u32 zero[16] = {0};
         store16(a.out + 4 * (u64)i, zero);
@@ -302,9 +306,7 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   k_bucket_scan<<<1, 64, 0, s>>>(a);
   k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
   if ((e = mark(1)) != hipSuccess) return e;
-  // Ed25519 challenges, hashing the items in bucket order (equal SHA-512 block counts per wavefront);
-  // secp entries of the order array are skipped by the scheme test inside
-  k_batch_hash<<<L.grid, 256, 0, s>>>(a);
+  // (the Ed25519 challenges are hashed inside k_batch_run; the stage stays empty)
   if ((e = mark(2)) != hipSuccess) return e;
   k_batch_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                      (uint4*)L.lane_tab, L.slots);
