@@ -279,7 +279,8 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
     acc = edbl(acc, false);
     acc = edbl(acc, true);
     E = neg_cached_if(neg, E);
-    acc = eadd_cached(acc, E, true);
+    // T is needed only by a following addition: doublings do not read it (uniform branches)
+    acc = eadd_cached(acc, E, (i & 1) == 0);
     if ((i & 1) == 0) {
       const u32 cb = byte8(c1, i >> 1);
       const bool bneg = (cb < 128) != b_neg;
@@ -287,7 +288,7 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
       bt.load(((cb < 128) ? ~cb : cb) & 127u, ypx, ymx, t2d);
       const u32 m = lane_mask(bneg);
       acc = eadd_niels(acc, fsel_mask(m, ymx, ypx), fsel_mask(m, ypx, ymx),
-                       fsel_mask(m, fsub_nr<Fe, 2, 1>(fe_zero(), t2d), t2d), true);
+                       fsel_mask(m, fsub_nr<Fe, 2, 1>(fe_zero(), t2d), t2d), i == 0);
     }
   }
   {
