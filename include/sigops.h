@@ -232,10 +232,11 @@ int sigops_server_wait(sigops_server* srv, int64_t ticket, uint8_t* out, uint8_t
                        uint64_t* t_submit_ns, uint64_t* t_done_ns);
 /* batches launched and items processed so far */
 int sigops_server_stats(sigops_server* srv, uint64_t* batches, uint64_t* items);
-/* Load replay for latency measurement: a native client submits request r (items req_off[r] ..
-   req_off[r+1] of the given arrays, messages through the global msg_off) at arrival_ns[r] after the
-   start, collects every result into out / status (may be NULL) and writes latency_ns[r] =
-   completion - submission. */
+/* Load replay for latency measurement: native clients (4 threads, request r on thread r mod 4)
+   submit request r (items req_off[r] .. req_off[r+1] of the given arrays, messages through the
+   global msg_off) at arrival_ns[r] after the start; every result is collected into out / status
+   (may be NULL) and latency_ns[r] = completion - scheduled arrival, so a client behind its
+   schedule adds to the latency. */
 int sigops_server_replay(sigops_server* srv, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
                          const uint8_t* msgs, const uint64_t* msg_off, const uint64_t* req_off,
                          const uint64_t* arrival_ns, size_t n_requests, uint64_t* latency_ns, uint8_t* out,

@@ -14,8 +14,10 @@
 // depend on what it was batched with.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdint>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -26,6 +28,7 @@
 #include <vector>
 
 #include "../../include/sigops.h"
+#include "host_xfer.hpp"
 
 namespace {
 
@@ -118,6 +121,7 @@ struct sigops_server {
   bool stopping = false;
   bool dispatcher_done = false;
   std::thread dispatcher, completer;
+  std::unique_ptr<sigops_host::Pool> pack_pool;  // used by the dispatcher only
   std::atomic<uint64_t> n_batches{0}, n_items{0};
   int fatal = 0;
 
@@ -139,16 +143,37 @@ int sigops_server::launch(Slot* s) {
   if (!s->d_sch.ensure(n) || !s->d_sig.ensure(n * 64) || !s->d_aux.ensure(n * 32) || !s->d_msg.ensure(msg + 16) ||
       !s->d_off.ensure(n + 1) || !s->d_out.ensure(n * 64) || !s->d_st.ensure(n))
     return SIGOPS_E_NOMEM;
-  size_t i0 = 0, m0 = 0;
-  for (auto& r : s->reqs) {
-    memcpy(s->h_sch.p + i0, r->schemes.data(), r->n);
-    memcpy(s->h_sig.p + i0 * 64, r->sigs.data(), r->n * 64);
-    memcpy(s->h_aux.p + i0 * 32, r->aux.data(), r->n * 32);
-    if (!r->msgs.empty()) memcpy(s->h_msg.p + m0, r->msgs.data(), r->msgs.size());
-    for (size_t k = 0; k < r->n; k++) s->h_off.p[i0 + k] = m0 + r->off[k];
-    i0 += r->n;
-    m0 += r->msgs.size();
+  // item and message positions of each request, then the copies, spread over the pool by items
+  const size_t nr = s->reqs.size();
+  std::vector<size_t> ri(nr + 1), rm(nr + 1);
+  ri[0] = rm[0] = 0;
+  for (size_t k = 0; k < nr; k++) {
+    ri[k + 1] = ri[k] + s->reqs[k]->n;
+    rm[k + 1] = rm[k] + s->reqs[k]->msgs.size();
   }
+  auto pack = [&](size_t ka, size_t kb) {
+    for (size_t k = ka; k < kb; k++) {
+      const Request& r = *s->reqs[k];
+      const size_t i0 = ri[k], m0 = rm[k];
+      memcpy(s->h_sch.p + i0, r.schemes.data(), r.n);
+      memcpy(s->h_sig.p + i0 * 64, r.sigs.data(), r.n * 64);
+      memcpy(s->h_aux.p + i0 * 32, r.aux.data(), r.n * 32);
+      if (!r.msgs.empty()) memcpy(s->h_msg.p + m0, r.msgs.data(), r.msgs.size());
+      for (size_t j = 0; j < r.n; j++) s->h_off.p[i0 + j] = m0 + r.off[j];
+    }
+  };
+  const unsigned nt = pack_pool->size();
+  if (nt > 1 && n >= 8192 && nr > 1) {
+    // thread t packs the requests whose first item lies in [t n / nt, (t + 1) n / nt)
+    pack_pool->run([&](unsigned t) {
+      const size_t ka = std::lower_bound(ri.begin(), ri.begin() + nr, n * t / nt) - ri.begin();
+      const size_t kb = std::lower_bound(ri.begin(), ri.begin() + nr, n * (t + 1) / nt) - ri.begin();
+      pack(ka, kb);
+    });
+  } else {
+    pack(0, nr);
+  }
+  const size_t m0 = rm[nr];
   s->h_off.p[n] = m0;
   memset(s->h_msg.p + m0, 0, 16);
   s->items = n;
@@ -288,6 +313,7 @@ sigops_server* sigops_server_create(const sigops_server_config* cfg_in) {
   auto* S = new (std::nothrow) sigops_server();
   if (!S) return nullptr;
   S->cfg = cfg;
+  S->pack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
   for (uint32_t k = 0; k < cfg.slots; k++) {
     auto slot = std::make_unique<Slot>();
     sigops_config cc{};
@@ -401,40 +427,51 @@ int sigops_server_replay(sigops_server* S, const uint8_t* schemes, const uint8_t
                          const uint64_t* arrival_ns, size_t n_requests, uint64_t* latency_ns, uint8_t* out,
                          uint8_t* status) {
   if (!S || !req_off || !arrival_ns || !latency_ns) return SIGOPS_E_ARG;
-  std::vector<int64_t> tickets(n_requests, -1);
+  constexpr int64_t PENDING = INT64_MIN;
+  std::vector<std::atomic<int64_t>> tickets(n_requests);
+  for (auto& t : tickets) t.store(PENDING, std::memory_order_relaxed);
   std::atomic<int> err{0};
-  // collector: waits for the requests in submission order; the server stamps each completion time
-  // itself, so waiting in order does not inflate the latencies
-  std::atomic<size_t> submitted{0};
+  const uint64_t t0 = now_ns();
+  // collector: waits for the requests in arrival order; the server stamps each completion time
+  // itself, so waiting in order does not inflate the latencies, which run from the scheduled
+  // arrival (a client that falls behind its schedule shows up in them)
   std::thread collector([&] {
     for (size_t r = 0; r < n_requests; r++) {
-      while (submitted.load(std::memory_order_acquire) <= r) std::this_thread::sleep_for(std::chrono::microseconds(5));
+      int64_t tk;
+      while ((tk = tickets[r].load(std::memory_order_acquire)) == PENDING)
+        std::this_thread::sleep_for(std::chrono::microseconds(5));
       uint64_t ts = 0, td = 0;
       const size_t a = req_off[r];
-      const int rc = sigops_server_wait(S, tickets[r], out ? out + a * 64 : nullptr, status ? status + a : nullptr, -1,
-                                        &ts, &td);
+      const int rc =
+          sigops_server_wait(S, tk, out ? out + a * 64 : nullptr, status ? status + a : nullptr, -1, &ts, &td);
       if (rc) err = rc;
-      latency_ns[r] = td - ts;
+      const uint64_t due = t0 + arrival_ns[r];
+      latency_ns[r] = td > due ? td - due : 0;
     }
   });
-  const uint64_t t0 = now_ns();
-  for (size_t r = 0; r < n_requests; r++) {
-    const uint64_t due = t0 + arrival_ns[r];
-    for (;;) {
-      const uint64_t t = now_ns();
-      if (t >= due) break;
-      if (due - t > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(due - t - 100000));
-    }
-    const size_t a = req_off[r], b = req_off[r + 1];
-    const int64_t tk = sigops_server_submit(S, schemes + a, sigs + a * 64, aux + a * 32, msgs, msg_off + a, b - a);
-    if (tk < 0) {
-      err = (int)tk;
-      tickets[r] = sigops_server_submit(S, schemes, sigs, aux, msgs, msg_off, 0);  // placeholder, completes at once
-    } else {
-      tickets[r] = tk;
-    }
-    submitted.store(r + 1, std::memory_order_release);
-  }
+  // clients: request r is submitted by client r mod K at its arrival time, as requests from
+  // several independent callers would be
+  const unsigned K = (unsigned)std::min<size_t>(4, std::max<size_t>(1, n_requests));
+  std::vector<std::thread> clients;
+  for (unsigned c = 0; c < K; c++)
+    clients.emplace_back([&, c] {
+      for (size_t r = c; r < n_requests; r += K) {
+        const uint64_t due = t0 + arrival_ns[r];
+        for (;;) {
+          const uint64_t t = now_ns();
+          if (t >= due) break;
+          if (due - t > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(due - t - 100000));
+        }
+        const size_t a = req_off[r], b = req_off[r + 1];
+        int64_t tk = sigops_server_submit(S, schemes + a, sigs + a * 64, aux + a * 32, msgs, msg_off + a, b - a);
+        if (tk < 0) {
+          err = (int)tk;
+          tk = sigops_server_submit(S, schemes, sigs, aux, msgs, msg_off, 0);  // placeholder, completes at once
+        }
+        tickets[r].store(tk, std::memory_order_release);
+      }
+    });
+  for (auto& t : clients) t.join();
   collector.join();
   return err.load();
 }

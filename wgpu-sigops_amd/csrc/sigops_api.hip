@@ -7,8 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <cstring>
 #include <string>
@@ -16,6 +18,7 @@
 
 #include "../../include/sigops.h"
 #include "consts.hpp"
+#include "host_xfer.hpp"
 #include "kernels.hpp"
 
 namespace {
@@ -63,7 +66,10 @@ struct sigops_ctx {
   bool stage_recorded = false;
   hipEvent_t stage_ev[SIGOPS_STAGES + 1] = {};
   std::vector<uint8_t> host_status;
-  std::vector<uint64_t> host_off;
+  // host entry points: pinned staging of caller buffers, packed and unpacked by a thread pool
+  std::unique_ptr<sigops_host::Pool> pool;
+  sigops_host::Pinned h_up, h_down;
+  std::vector<hipEvent_t> xfer_ev;
   std::string err;
 };
 
@@ -94,6 +100,43 @@ int ensure(sigops_ctx* c, DevBuf& b, size_t bytes) {
   if (e != hipSuccess) return fail(c, SIGOPS_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   b.cap = want;
   return SIGOPS_OK;
+}
+
+sigops_host::Pool& host_pool(sigops_ctx* c) {
+  if (!c->pool) c->pool = std::make_unique<sigops_host::Pool>(sigops_host::host_threads());
+  return *c->pool;
+}
+
+// caller buffers -> device buffers, queued on the context's stream
+int put(sigops_ctx* c, const std::vector<sigops_host::Up>& segs) {
+  HIPCHK(c, sigops_host::upload(host_pool(c), c->h_up, segs, c->stream));
+  return SIGOPS_OK;
+}
+
+// device buffers -> caller buffers after the work queued on the context's stream; returns when
+// they are in place
+int get(sigops_ctx* c, const std::vector<sigops_host::Down>& segs) {
+  HIPCHK(c, sigops_host::download(host_pool(c), c->h_down, segs, c->stream, c->xfer_ev));
+  return SIGOPS_OK;
+}
+
+// pred(i) for every i in [0, n), evaluated by the pool; false if any is false
+template <class P>
+bool all_of(sigops_ctx* c, size_t n, P pred) {
+  std::atomic<bool> ok{true};
+  host_pool(c).for_range(n, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; i++)
+      if (!pred(i)) {
+        ok.store(false, std::memory_order_relaxed);
+        return;
+      }
+  });
+  return ok.load();
+}
+
+// message offsets: non-decreasing, each length below 2^32
+bool offsets_ok(sigops_ctx* c, const uint64_t* off, size_t n) {
+  return all_of(c, n, [&](size_t i) { return off[i + 1] >= off[i] && off[i + 1] - off[i] <= 0xFFFFFFFFull; });
 }
 
 __global__ void k_fill_bytes(uint8_t* p, uint8_t v, size_t n) {
@@ -161,8 +204,7 @@ int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8
       (rc = ensure(c, c->st, n)))
     return rc;
   hipStream_t s = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->aux.p, digs, n * 32, hipMemcpyHostToDevice, s));
+  if ((rc = put(c, {{c->in.p, sigs, n * 64}, {c->aux.p, digs, n * 32}}))) return rc;
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
   if ((rc = secp_recover_dev(c, curve, c->in.p, c->aux.p, n, c->out.p, c->st.p, s, gtab))) return rc;
@@ -174,14 +216,13 @@ int secp_recover_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8
   if (out_addr) {
     // the digests in aux are consumed by the recovery launched before on this stream
     HIPCHK(c, sigops::launch_fuel_address(c->out.p, c->st.p, (uint32_t)n, c->aux.p, c->grid, s));
-    HIPCHK(c, hipMemcpyAsync(out_addr, c->aux.p, n * 32, hipMemcpyDeviceToHost, s));
+    rc = get(c, {{out_addr, c->aux.p, n * 32}, {st, c->st.p, n}});
   } else {
-    HIPCHK(c, hipMemcpyAsync(out_pub, c->out.p, n * 64, hipMemcpyDeviceToHost, s));
+    rc = get(c, {{out_pub, c->out.p, n * 64}, {st, c->st.p, n}});
   }
-  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  for (size_t i = 0; i < n; i++)
-    if (st[i] > SIGOPS_ST_INFINITY) return fail(c, SIGOPS_E_CANARY, "a recovery result was not written");
+  if (rc) return rc;
+  if (!all_of(c, n, [&](size_t i) { return st[i] <= SIGOPS_ST_INFINITY; }))
+    return fail(c, SIGOPS_E_CANARY, "a recovery result was not written");
   return SIGOPS_OK;
 }
 
@@ -211,9 +252,7 @@ int secp_verify_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8_
       (rc = ensure(c, c->st, n)))
     return rc;
   hipStream_t s = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->aux.p, digs, n * 32, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->out.p, pubs, n * 64, hipMemcpyHostToDevice, s));
+  if ((rc = put(c, {{c->in.p, sigs, n * 64}, {c->aux.p, digs, n * 32}, {c->out.p, pubs, n * 64}}))) return rc;
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
   if ((rc = secp_verify_dev(c, curve, c->in.p, c->aux.p, c->out.p, n, c->st.p, s))) return rc;
@@ -222,12 +261,12 @@ int secp_verify_host(sigops_ctx* c, int curve, const uint8_t* sigs, const uint8_
     c->host_status.resize(n);
     st = c->host_status.data();
   }
-  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  for (size_t i = 0; i < n; i++) {
-    if (st[i] > SIGOPS_ST_MISMATCH) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
-    valid[i] = st[i] == SIGOPS_ST_OK;
-  }
+  if ((rc = get(c, {{st, c->st.p, n}}))) return rc;
+  if (!all_of(c, n, [&](size_t i) {
+        valid[i] = st[i] == SIGOPS_ST_OK;
+        return st[i] <= SIGOPS_ST_MISMATCH;
+      }))
+    return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
   return SIGOPS_OK;
 }
 
@@ -315,6 +354,10 @@ void sigops_destroy(sigops_ctx* c) {
                   (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, (void*)c->d_tlimbs, c->d_ctab,
                   (void*)c->d_tflags, c->d_kbuf})
     if (p) (void)hipFree(p);
+  c->pool.reset();
+  sigops_host::pinned_free(c->h_up);
+  sigops_host::pinned_free(c->h_down);
+  for (hipEvent_t e : c->xfer_ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -417,24 +460,22 @@ static int ed_verify_host(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
   if (n == 0) return SIGOPS_OK;
   if (!sigs || !pubkeys || !msg_off || !valid) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (!offsets_ok(c, msg_off, n))
+    return fail(c, SIGOPS_E_ARG, "msg_off must be non-decreasing with lengths < 2^32");
   const uint64_t base = msg_off[0];
-  for (size_t i = 0; i < n; i++)
-    if (msg_off[i + 1] < msg_off[i] || msg_off[i + 1] - msg_off[i] > 0xFFFFFFFFull)
-      return fail(c, SIGOPS_E_ARG, "msg_off must be non-decreasing with lengths < 2^32");
   const uint64_t total = msg_off[n] - base;
   if (total && !msgs) return fail(c, SIGOPS_E_ARG, "NULL msgs");
   int rc;
   if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) ||
       (rc = ensure(c, c->msg, total + 16)) || (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->st, n)))
     return rc;
-  c->host_off.resize(n + 1);
-  for (size_t i = 0; i <= n; i++) c->host_off[i] = msg_off[i] - base;
   hipStream_t s = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->aux.p, pubkeys, n * 32, hipMemcpyHostToDevice, s));
-  if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, msgs + base, total, hipMemcpyHostToDevice, s));
+  if ((rc = put(c, {{c->in.p, sigs, n * 64},
+                    {c->aux.p, pubkeys, n * 32},
+                    {c->msg.p, total ? msgs + base : nullptr, total},
+                    {c->off.p, msg_off, (n + 1) * 8, base}})))
+    return rc;
   k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
-  HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
   if ((rc = ed_verify_dev(c, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->st.p, s, strict, btab)))
@@ -444,13 +485,13 @@ static int ed_verify_host(sigops_ctx* c, const uint8_t* sigs, const uint8_t* pub
     c->host_status.resize(n);
     st = c->host_status.data();
   }
-  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
+  if ((rc = get(c, {{st, c->st.p, n}}))) return rc;
   const uint8_t last = strict ? SIGOPS_ED_SMALL_ORDER : SIGOPS_ED_MISMATCH;
-  for (size_t i = 0; i < n; i++) {
-    if (st[i] > last) return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
-    valid[i] = st[i] == SIGOPS_ED_OK;
-  }
+  if (!all_of(c, n, [&](size_t i) {
+        valid[i] = st[i] == SIGOPS_ED_OK;
+        return st[i] <= last;
+      }))
+    return fail(c, SIGOPS_E_CANARY, "a verification result was not written");
   return SIGOPS_OK;
 }
 
@@ -612,26 +653,20 @@ int sigops_message_new(sigops_ctx* c, const uint8_t* data, const uint64_t* off, 
   if (n == 0) return SIGOPS_OK;
   if (!off || !digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
+  if (!offsets_ok(c, off, n)) return fail(c, SIGOPS_E_ARG, "off must be non-decreasing with lengths < 2^32");
   const uint64_t base = off[0];
-  for (size_t i = 0; i < n; i++)
-    if (off[i + 1] < off[i] || off[i + 1] - off[i] > 0xFFFFFFFFull)
-      return fail(c, SIGOPS_E_ARG, "off must be non-decreasing with lengths < 2^32");
   const uint64_t total = off[n] - base;
   if (total && !data) return fail(c, SIGOPS_E_ARG, "NULL data");
   int rc;
   if ((rc = ensure(c, c->msg, total + 16)) || (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->aux, n * 32)))
     return rc;
-  c->host_off.resize(n + 1);
-  for (size_t i = 0; i <= n; i++) c->host_off[i] = off[i] - base;
   hipStream_t s = c->stream;
-  if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, data + base, total, hipMemcpyHostToDevice, s));
+  if ((rc = put(c, {{c->msg.p, total ? data + base : nullptr, total}, {c->off.p, off, (n + 1) * 8, base}}))) return rc;
   k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
-  HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, sigops::launch_message_new(c->msg.p, (const uint64_t*)c->off.p, (uint32_t)n, c->aux.p, c->d_hq,
                                        2 * c->grid, s));
-  HIPCHK(c, hipMemcpyAsync(digests, c->aux.p, n * 32, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  return SIGOPS_OK;
+  return get(c, {{digests, c->aux.p, n * 32}});
 }
 
 int sigops_address_from_pubkeys_dev(sigops_ctx* c, const void* d_pubs, const void* d_status, size_t n, void* d_addr,
@@ -670,12 +705,11 @@ int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, con
   if (n == 0) return SIGOPS_OK;
   if (!schemes || !sigs || !aux || !msg_off || !out) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
-  for (size_t i = 0; i < n; i++)
-    if (schemes[i] > SIGOPS_SCHEME_ED25519) return fail(c, SIGOPS_E_ARG, "unknown scheme byte");
+  if (!all_of(c, n, [&](size_t i) { return schemes[i] <= SIGOPS_SCHEME_ED25519; }))
+    return fail(c, SIGOPS_E_ARG, "unknown scheme byte");
+  if (!offsets_ok(c, msg_off, n))
+    return fail(c, SIGOPS_E_ARG, "msg_off must be non-decreasing with lengths < 2^32");
   const uint64_t base = msg_off[0];
-  for (size_t i = 0; i < n; i++)
-    if (msg_off[i + 1] < msg_off[i] || msg_off[i + 1] - msg_off[i] > 0xFFFFFFFFull)
-      return fail(c, SIGOPS_E_ARG, "msg_off must be non-decreasing with lengths < 2^32");
   const uint64_t total = msg_off[n] - base;
   if (total && !msgs) return fail(c, SIGOPS_E_ARG, "NULL msgs");
   int rc;
@@ -683,15 +717,14 @@ int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, con
       (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->out, n * 64)) || (rc = ensure(c, c->st, n)) ||
       (rc = ensure(c, c->sch, n)))
     return rc;
-  c->host_off.resize(n + 1);
-  for (size_t i = 0; i <= n; i++) c->host_off[i] = msg_off[i] - base;
   hipStream_t s = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->sch.p, schemes, n, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->in.p, sigs, n * 64, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->aux.p, aux, n * 32, hipMemcpyHostToDevice, s));
-  if (total) HIPCHK(c, hipMemcpyAsync(c->msg.p, msgs + base, total, hipMemcpyHostToDevice, s));
+  if ((rc = put(c, {{c->sch.p, schemes, n},
+                    {c->in.p, sigs, n * 64},
+                    {c->aux.p, aux, n * 32},
+                    {c->msg.p, total ? msgs + base : nullptr, total},
+                    {c->off.p, msg_off, (n + 1) * 8, base}})))
+    return rc;
   k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
-  HIPCHK(c, hipMemcpyAsync(c->off.p, c->host_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
   k_fill_bytes<<<64, 256, 0, s>>>((uint8_t*)c->st.p, CANARY, n);
   HIPCHK(c, hipGetLastError());
   if ((rc = sigops_batch_dev(c, c->sch.p, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->out.p,
@@ -702,13 +735,11 @@ int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, con
     c->host_status.resize(n);
     st = c->host_status.data();
   }
-  HIPCHK(c, hipMemcpyAsync(out, c->out.p, n * 64, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(st, c->st.p, n, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  for (size_t i = 0; i < n; i++) {
-    const uint8_t lim = schemes[i] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY;
-    if (st[i] > lim) return fail(c, SIGOPS_E_CANARY, "a batch result was not written");
-  }
+  if ((rc = get(c, {{out, c->out.p, n * 64}, {st, c->st.p, n}}))) return rc;
+  if (!all_of(c, n, [&](size_t i) {
+        return st[i] <= (schemes[i] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY);
+      }))
+    return fail(c, SIGOPS_E_CANARY, "a batch result was not written");
   return SIGOPS_OK;
 }
 
