@@ -180,8 +180,9 @@ int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs,
    Addresses (the step after it): address i = SHA-256(X || Y) of a 64-byte public key, 32 zero
    bytes where the recovery failed (status != 0). No counterpart in the reference's API. */
 int sigops_message_new(sigops_ctx* ctx, const uint8_t* data, const uint64_t* off, size_t n, uint8_t* digests);
-/* device pointers, d_data 4-byte aligned and readable 16 bytes past its last byte; n < 2^31.
-   The context's hashing work counter is shared: calls on one context must not overlap on the device. */
+/* device pointers, d_data 4-byte aligned and readable 16 bytes past its last byte; n <= max_batch.
+   Messages are hashed longest first (a counting sort by block count on the device). The context's
+   hashing work words and order are shared: calls on one context must not overlap on the device. */
 int sigops_message_new_dev(sigops_ctx* ctx, const void* d_data, const uint64_t* d_off, size_t n, void* d_digests,
                            void* stream);
 /* d_status may be NULL (every key hashed) */

@@ -35,6 +35,18 @@ def test_message_new_lengths(ctx):
     assert sigops.fuel.message_new(datas[:5], ctx=ctx) == [hashlib.sha256(d).digest() for d in datas[:5]]
 
 
+def test_message_new_single_block_count(ctx):
+    """messages of one SHA-256 block count keep their index order (the longest-first sort is
+    skipped): lengths 0..55 (one block each), all 64-byte messages (two), and a single message"""
+    rng = np.random.default_rng(23)
+    for lens in (rng.integers(0, 56, 5000), np.full(3000, 64), np.array([200])):
+        datas = [rng.integers(0, 256, int(L), dtype=np.uint8).tobytes() for L in lens]
+        _say(f"message_new one block count n={len(datas)} len0={len(datas[0])}")
+        got = ctx.message_new(datas)
+        want = np.array([np.frombuffer(hashlib.sha256(d).digest(), np.uint8) for d in datas])
+        assert np.array_equal(got, want)
+
+
 def test_message_new_dev_unaligned_offsets(ctx):
     rng = np.random.default_rng(22)
     n = 777
@@ -45,6 +57,26 @@ def test_message_new_dev_unaligned_offsets(ctx):
     d_off = torch.from_numpy(off.astype(np.int64)).cuda()
     d_out = torch.zeros((n, 32), dtype=torch.uint8, device="cuda")
     ctx.message_new_dev(d_data.data_ptr(), d_off.data_ptr(), n, d_out.data_ptr())
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy()
+    for i in range(n):
+        assert got[i].tobytes() == hashlib.sha256(data[off[i]:off[i + 1]].tobytes()).digest(), i
+
+
+def test_message_new_dev_buffer_4_byte_aligned(ctx):
+    """a data pointer 4 bytes past a 16-byte boundary (the documented minimum alignment), with
+    log-uniform lengths up to 3000 bytes: the longest-first order over many block counts"""
+    rng = np.random.default_rng(25)
+    n = 3000
+    lens = np.exp(rng.uniform(0, np.log(3000), n)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    data = rng.integers(0, 256, int(off[-1]) + 16, dtype=np.uint8)
+    d_buf = torch.from_numpy(np.concatenate([np.zeros(4, np.uint8), data])).cuda()
+    d_off = torch.from_numpy(off).cuda()
+    d_out = torch.zeros((n, 32), dtype=torch.uint8, device="cuda")
+    assert d_buf.data_ptr() % 16 == 0
+    _say(f"message_new_dev 4-byte-aligned buffer n={n}")
+    ctx.message_new_dev(d_buf.data_ptr() + 4, d_off.data_ptr(), n, d_out.data_ptr())
     torch.cuda.synchronize()
     got = d_out.cpu().numpy()
     for i in range(n):

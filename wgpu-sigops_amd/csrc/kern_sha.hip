@@ -5,16 +5,82 @@
 
 namespace sigops {
 
+// Longest messages first. A lane that takes a long message near the end of the queue would keep
+// the launch running, alone, for its remaining blocks; so the queue hands out message indices in
+// order of decreasing block count. A counting sort builds that order: bin 0 holds messages of
+// MSG_BINS or more blocks, bin b > 0 those of MSG_BINS - b blocks. The work words hold the queue
+// counter, the bin counts, the bin cursors and a flag that is set when more than one bin is used
+// (with a single length the index order is kept and the scatter does nothing).
+constexpr u32 MW_QUEUE = 0, MW_COUNT = 1, MW_CURSOR = 1 + MSG_BINS, MW_FLAG = 1 + 2 * MSG_BINS;
+
+DEV u32 msg_bin(const u64* off, u32 i) {
+  const u64 nb = (off[i + 1] - off[i] + 8) / 64 + 1;  // SHA-256 blocks, padding included
+  return nb >= MSG_BINS ? 0u : MSG_BINS - (u32)nb;
+}
+
+static __global__ __launch_bounds__(256) void k_msg_hist(const u64* __restrict__ off, u32 n, u32* __restrict__ work) {
+  __shared__ u32 h[MSG_BINS];
+  for (u32 t = threadIdx.x; t < MSG_BINS; t += blockDim.x) h[t] = 0;
+  __syncthreads();
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&h[msg_bin(off, i)], 1u);
+  __syncthreads();
+  for (u32 t = threadIdx.x; t < MSG_BINS; t += blockDim.x)
+    if (h[t]) atomicAdd(&work[MW_COUNT + t], h[t]);
+}
+
+// one lane: exclusive scan of the bin counts into the cursors, and the flag
+static __global__ __launch_bounds__(64) void k_msg_scan(u32* __restrict__ work) {
+  if (threadIdx.x != 0) return;
+  u32 pos = 0, used = 0;
+  for (u32 b = 0; b < MSG_BINS; b++) {
+    const u32 c = work[MW_COUNT + b];
+    work[MW_CURSOR + b] = pos;
+    pos += c;
+    used += c != 0;
+  }
+  work[MW_FLAG] = used > 1;
+}
+
+// ranks within each workgroup's chunk of 256 messages by LDS atomics, then one global atomic per
+// bin and chunk reserves the chunk's range in that bin
+static __global__ __launch_bounds__(256) void k_msg_scatter(const u64* __restrict__ off, u32 n, u32* __restrict__ work,
+                                                           u32* __restrict__ order) {
+  __shared__ u32 cnt[MSG_BINS];
+  __shared__ u32 base[MSG_BINS];
+  if (work[MW_FLAG] == 0) return;
+  for (u32 c0 = blockIdx.x * blockDim.x; c0 < n; c0 += gridDim.x * blockDim.x) {
+    for (u32 t = threadIdx.x; t < MSG_BINS; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
+    const u32 i = c0 + threadIdx.x;
+    u32 b = MSG_BINS, rank = 0;
+    if (i < n) {
+      b = msg_bin(off, i);
+      rank = atomicAdd(&cnt[b], 1u);
+    }
+    __syncthreads();
+    for (u32 t = threadIdx.x; t < MSG_BINS; t += blockDim.x)
+      if (cnt[t]) base[t] = atomicAdd(&work[MW_CURSOR + t], cnt[t]);
+    __syncthreads();
+    if (b < MSG_BINS) order[base[b] + rank] = i;
+    __syncthreads();
+  }
+}
+
 // digest i = SHA-256(msgs[off[i] .. off[i+1])); msgs 4-byte aligned with slack after the last byte.
 // Messages differ in length, so no lane owns one message for the whole launch: each loop trip
-// compresses one block per lane, and a lane whose message is done takes the next index from the
-// queue counter (the lanes of a wave that need one in the same trip take consecutive indices with
-// one atomic). A wave stays busy until the queue runs dry instead of idling behind its longest
-// message. queue[0] must be 0 at launch; every lane leaves once an index >= n comes out.
+// compresses one block per lane, and a lane whose message is done takes the next position from the
+// queue counter (the lanes of a wave that need one in the same trip take consecutive positions with
+// one atomic) and the message index at that position of the longest-first order. A wave stays busy
+// until the queue runs dry instead of idling behind its longest message. The queue counter must be
+// 0 at launch; every lane leaves once a position >= n comes out.
 static __global__ __launch_bounds__(256) void k_message_new(const u32* __restrict__ msgs,
                                                            const u64* __restrict__ off, u32 n,
-                                                           uint4* __restrict__ out, u32* __restrict__ queue) {
+                                                           uint4* __restrict__ out, u32* __restrict__ work,
+                                                           const u32* __restrict__ order) {
   const MemDev mem{msgs};
+  u32* queue = work + MW_QUEUE;
+  const bool sorted = work[MW_FLAG] != 0;
   u32 i = 0, b = 0, nblk = 0, len = 0;
   u64 o = 0;
   u32 h[8];
@@ -27,8 +93,9 @@ static __global__ __launch_bounds__(256) void k_message_new(const u32* __restric
       if (__lane_id() == leader) first = atomicAdd(queue, (u32)__popcll(want));
       first = __builtin_amdgcn_readlane(first, leader);
       if (need) {
-        i = first + __builtin_amdgcn_mbcnt_hi((u32)(want >> 32), __builtin_amdgcn_mbcnt_lo((u32)want, 0u));
-        if (i >= n) break;
+        const u32 j = first + __builtin_amdgcn_mbcnt_hi((u32)(want >> 32), __builtin_amdgcn_mbcnt_lo((u32)want, 0u));
+        if (j >= n) break;
+        i = sorted ? order[j] : j;
         o = off[i];
         len = (u32)(off[i + 1] - o);
         nblk = sha256_nblocks(len);
@@ -66,11 +133,14 @@ static u32 grid_for(u32 n, u32 grid) {
   return want < grid ? want : grid;
 }
 
-hipError_t launch_message_new(const void* msgs, const u64* off, u32 n, void* out, u32* queue, u32 grid,
+hipError_t launch_message_new(const void* msgs, const u64* off, u32 n, void* out, u32* work, u32* order, u32 grid,
                               hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_fill_u32<<<1, 64, 0, s>>>(queue, 0u, 1u);
-  k_message_new<<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out, queue);
+  k_fill_u32<<<1, 256, 0, s>>>(work, 0u, MSG_WORK_WORDS);
+  k_msg_hist<<<grid_for(n, grid), 256, 0, s>>>(off, n, work);
+  k_msg_scan<<<1, 64, 0, s>>>(work);
+  k_msg_scatter<<<grid_for(n, grid), 256, 0, s>>>(off, n, work, order);
+  k_message_new<<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out, work, order);
   return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ struct sigops_ctx {
   uint32_t* d_work = nullptr;
   uint32_t* d_order = nullptr;
   uint32_t* d_n = nullptr;
-  uint32_t* d_hq = nullptr;  // work counter of the message-hashing kernel
+  uint32_t* d_hq = nullptr;      // work words of the message-hashing launch (MSG_WORK_WORDS)
+  uint32_t* d_horder = nullptr;  // max_batch words: longest-first order of the messages
   // caller-supplied tables in the reference's layout: limbs in, converted table out, check flags
   uint32_t* d_tlimbs = nullptr;
   void* d_ctab = nullptr;
@@ -333,7 +334,9 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
       hipSuccess)
     return bad(e, "hipMalloc batch order");
   if ((e = hipMalloc((void**)&c->d_n, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc n");
-  if ((e = hipMalloc((void**)&c->d_hq, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc hash queue");
+  if ((e = hipMalloc((void**)&c->d_hq, sigops::MSG_WORK_WORDS * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc((void**)&c->d_horder, (size_t)c->max_batch * sizeof(uint32_t))) != hipSuccess)
+    return bad(e, "hipMalloc message hashing");
   if ((e = hipMalloc((void**)&c->d_tlimbs, TABLE_LIMBS_MAX * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc(&c->d_ctab, sizeof(sigops::ED_BTABLE))) != hipSuccess ||
       (e = hipMalloc((void**)&c->d_tflags, sizeof(uint32_t))) != hipSuccess)
@@ -351,7 +354,7 @@ void sigops_destroy(sigops_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
                   c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
-                  (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, (void*)c->d_tlimbs, c->d_ctab,
+                  (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, (void*)c->d_horder, (void*)c->d_tlimbs, c->d_ctab,
                   (void*)c->d_tflags, c->d_kbuf})
     if (p) (void)hipFree(p);
   c->pool.reset();
@@ -639,10 +642,10 @@ int sigops_message_new_dev(sigops_ctx* c, const void* d_data, const uint64_t* d_
                            void* stream) {
   if (!c) return SIGOPS_E_ARG;
   if (n == 0) return SIGOPS_OK;
-  if (n > 0x7FFFFFFFull) return fail(c, SIGOPS_E_ARG, "n too large");
+  if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (!d_data || !d_off || !d_digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (((uintptr_t)d_data & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_data must be 4-byte aligned");
-  HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, c->d_hq, 2 * c->grid,
+  HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, c->d_hq, c->d_horder, 2 * c->grid,
                                        (hipStream_t)stream));
   return SIGOPS_OK;
 }
@@ -664,7 +667,7 @@ int sigops_message_new(sigops_ctx* c, const uint8_t* data, const uint64_t* off, 
   if ((rc = put(c, {{c->msg.p, total ? data + base : nullptr, total}, {c->off.p, off, (n + 1) * 8, base}}))) return rc;
   k_fill_bytes<<<1, 64, 0, s>>>((uint8_t*)c->msg.p + total, 0, 16);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, sigops::launch_message_new(c->msg.p, (const uint64_t*)c->off.p, (uint32_t)n, c->aux.p, c->d_hq,
+  HIPCHK(c, sigops::launch_message_new(c->msg.p, (const uint64_t*)c->off.p, (uint32_t)n, c->aux.p, c->d_hq, c->d_horder,
                                        2 * c->grid, s));
   return get(c, {{digests, c->aux.p, n * 32}});
 }
