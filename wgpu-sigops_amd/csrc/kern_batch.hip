@@ -4,7 +4,8 @@
 // of any length. The reference has one kernel chain per scheme and fixed 32-byte messages; here:
 //   1. k_bucket_hist / k_bucket_scan / k_bucket_scatter sort item indices into buckets (k1, r1, and
 //      Ed25519 by SHA-512 block count, so the lanes of a wavefront hash equally long messages), each
-//      bucket padded to whole tiles; tiles are ordered by estimated cost, longest first;
+//      bucket padded to whole tiles; the buckets are processed in the context's order (its rank
+//      table: secp256r1, secp256k1, then Ed25519 by decreasing block count);
 //   2. k_batch_run is persistent: every wavefront pulls the next tile (64 x tile_rounds items of
 //      one bucket) from an atomic counter, so the scheme is uniform across the wavefront and load
 //      balances dynamically; it runs the same per-item code as the plain kernels;

@@ -315,15 +315,18 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
   if (cfg->tile_rounds) c->tile_rounds = std::min<uint32_t>(cfg->tile_rounds, sigops::MAX_TILE_ROUNDS);
   c->use_graph = cfg->no_graph == 0;
   {
-    // processing order of the buckets: estimated cost per item, most expensive first (an Ed25519
-    // verification costs ~0.85 of a recovery plus ~0.6% per SHA-512 block)
-    std::vector<std::pair<double, uint32_t>> cost;
-    cost.push_back({1.0, sigops::BUCKET_K1});
-    cost.push_back({1.01, sigops::BUCKET_R1});
-    for (uint32_t b = 0; b < sigops::ED_BINS; b++) cost.push_back({0.85 + 0.006 * (b + 1), sigops::BUCKET_ED0 + b});
-    std::stable_sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
+    // processing order of the buckets (higher key first): secp256r1, then secp256k1, then Ed25519
+    // by decreasing SHA-512 block count. Measured on MI355X, alternating builds on one box, for the
+    // 2^20 mixed batch: 16.30 ms in this order; 16.55 ms for an estimated-cost order that put the
+    // few longest Ed25519 buckets first and the short ones after secp256k1; 16.67 ms with
+    // secp256k1 before secp256r1; about 17 ms with the Ed25519 buckets first.
+    std::vector<std::pair<double, uint32_t>> key;
+    key.push_back({2.0, sigops::BUCKET_K1});
+    key.push_back({3.0, sigops::BUCKET_R1});
+    for (uint32_t b = 0; b < sigops::ED_BINS; b++) key.push_back({1.0 + 0.001 * (b + 1), sigops::BUCKET_ED0 + b});
+    std::stable_sort(key.begin(), key.end(), [](auto& x, auto& y) { return x.first > y.first; });
     uint8_t rank[sigops::NBUCKETS];
-    for (uint32_t r = 0; r < sigops::NBUCKETS; r++) rank[cost[r].second] = (uint8_t)r;
+    for (uint32_t r = 0; r < sigops::NBUCKETS; r++) rank[key[r].second] = (uint8_t)r;
     if ((e = hipMalloc((void**)&c->d_rank, sizeof rank)) != hipSuccess) return bad(e, "hipMalloc rank");
     if ((e = hipMemcpy(c->d_rank, rank, sizeof rank, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "upload");
   }
