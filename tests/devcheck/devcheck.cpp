@@ -295,15 +295,23 @@ void dc_field(int op, int mod, const unsigned char* a, const unsigned char* b, u
   memcpy(out, ro, 32);
 }
 
-void dc_sha512_ed(const unsigned char* hdr64, const unsigned char* msg, unsigned len, unsigned char* out) {
-  u32 hdr[16], h[16];
-  memcpy(hdr, hdr64, 64);
-  u32* buf = new u32[(len + 16) / 4 + 4]();
-  memcpy(buf, msg, len);
-  MemHost mem{buf};
-  sha512_ed(hdr, mem, 0, len, h);
+// SHA-512(hdr[0..hl) || msg) with hl = 32 or 64 and msg placed at byte offset `at` of a word
+// buffer; returns -1 if the hash read a word other than those holding message bytes (for an empty
+// message: the word at its offset)
+int dc_sha512(const unsigned char* hdr_bytes, int hl, const unsigned char* msg, unsigned len, unsigned at,
+              unsigned char* out) {
+  u32 hdr[16] = {}, h[16];
+  memcpy(hdr, hdr_bytes, hl);
+  u32* buf = new u32[(at + len + 16) / 4 + 4]();
+  memcpy((unsigned char*)buf + at, msg, len);
+  MemChecked mem{buf, at / 4u, (at + len - (len != 0)) / 4u, false};
+  if (hl == 64)
+    sha512_ed(hdr, mem, at, len, h);
+  else
+    sha512_hdr<32>(hdr, mem, at, len, h);
   memcpy(out, h, 64);
   delete[] buf;
+  return mem.outside ? -1 : 0;
 }
 
 void dc_gen_secp(int curve, unsigned long long seed, unsigned long long i, unsigned ppm, unsigned char* sig,

@@ -29,7 +29,7 @@ def lib():
         L.dc_secp_recover.argtypes = [ctypes.c_int, c_p, c_p, c_p, ctypes.c_int]
         L.dc_ed25519_verify.argtypes = [c_p, c_p, c_p, ctypes.c_uint, ctypes.c_int]
         L.dc_field.argtypes = [ctypes.c_int, ctypes.c_int, c_p, c_p, c_p]
-        L.dc_sha512_ed.argtypes = [c_p, c_p, ctypes.c_uint, c_p]
+        L.dc_sha512.argtypes = [c_p, ctypes.c_int, c_p, ctypes.c_uint, ctypes.c_uint, c_p]
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.dc_mont_raw.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p]
         L.dc_glv_split.argtypes = [u32p, u32p, u32p]
@@ -59,10 +59,18 @@ def field(op, mod, a, b=0):
     return int.from_bytes(out.raw, "big")
 
 
-def sha512_ed(hdr, msg):
+def sha512(hdr, msg, at=0):
+    """SHA-512(hdr || msg) by the device code built for the host, hdr of 32 or 64 bytes, msg at byte
+    offset `at` of its buffer; raises if the hash read memory outside the message's words"""
     out = ctypes.create_string_buffer(64)
-    lib().dc_sha512_ed(hdr, msg, len(msg), out)
+    rc = lib().dc_sha512(bytes(hdr), len(hdr), bytes(msg), len(msg), at, out)
+    if rc != 0:
+        raise AssertionError(f"SHA-512 of {len(msg)} bytes at offset {at} read outside the message's words")
     return out.raw
+
+
+def sha512_ed(hdr, msg):
+    return sha512(hdr, msg)
 
 
 def mont_raw(op, mod, a_limbs, b_limbs):

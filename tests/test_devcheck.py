@@ -136,11 +136,17 @@ def test_verify_random_and_edges(curve):
 
 
 def test_sha512_preimage_stream():
+    """SHA-512 of a 64-byte (R || A) or 32-byte header and a message of every length 0..259 and a
+    few long ones, the message at every byte alignment; the reader touches no word outside the
+    message"""
     rnd = random.Random(2)
-    for L in list(range(0, 260)) + [1000, 4096]:
-        hdr = bytes(rnd.randrange(256) for _ in range(64))
+    for L in list(range(0, 260)) + [1000, 4095, 4096]:
         msg = bytes(rnd.randrange(256) for _ in range(L))
-        assert D.sha512_ed(hdr, msg) == hashlib.sha512(hdr + msg).digest()
+        for hl in (64, 32):
+            hdr = bytes(rnd.randrange(256) for _ in range(hl))
+            want = hashlib.sha512(hdr + msg).digest()
+            for at in (0, 1, 2, 3) if L < 140 or L > 999 else (L % 4,):
+                assert D.sha512(hdr, msg, at) == want, (L, hl, at)
 
 
 @pytest.mark.parametrize("curve", [0, 1])

@@ -604,6 +604,15 @@ DEV void fe_to_words(const fe& a, u32 w[8]) {  // a < 2^256
 // big-endian bytes (as 8 u32 loaded little-endian from memory) <-> words
 DEV u32 bswap32(u32 x) { return __builtin_bswap32(x); }
 
+// the low 32 bits of (hi:lo) >> sh, sh in {0, 8, 16, 24} (v_alignbit_b32)
+DEV u32 funnel32(u32 hi, u32 lo, u32 sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, sh);
+#else
+  return (u32)((((u64)hi << 32) | lo) >> sh);
+#endif
+}
+
 DEV void words_from_be_bytes(const u32 raw[8], u32 w[8]) {
 #pragma unroll
   for (int i = 0; i < 8; i++) w[i] = bswap32(raw[7 - i]);

@@ -40,15 +40,6 @@ DEV u32 sha_maj(u32 a, u32 b, u32 c) {  // bitwise majority
 #endif
 }
 
-// the low 32 bits of (hi:lo) >> sh, sh in {0, 8, 16, 24}
-DEV u32 funnel32(u32 hi, u32 lo, u32 sh) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_alignbit(hi, lo, sh);
-#else
-  return (u32)((((u64)hi << 32) | lo) >> sh);
-#endif
-}
-
 DEV void sha256_round(u32 (&s)[8], int j, u32 kw) {
   const u32 a = s[(8 - j) & 7], b = s[(9 - j) & 7], c = s[(10 - j) & 7];
   u32& d = s[(11 - j) & 7];

@@ -52,8 +52,7 @@ DEV void sha512_compress(u64 h[8], u64 w[16]) {
   for (int i = 0; i < 8; i++) h[i] += s[i];
 }
 
-// 8 little-endian bytes at byte offset off of a 4-byte-aligned buffer (reads 12 bytes starting at
-// the aligned word below off; the buffer carries >= 16 bytes of slack after the last message).
+// the digest as 16 little-endian u32 words (its 64 bytes in order)
 DEV void sha512_out(const u64 h[8], u32 out[16]) {
 #pragma unroll
   for (int i = 0; i < 8; i++) {
@@ -63,24 +62,29 @@ DEV void sha512_out(const u64 h[8], u32 out[16]) {
   }
 }
 
-template <class Mem>
-DEV u64 load8_le(const Mem& mem, u64 off) {
-  const u64 wa = off >> 2;
-  const u32 sh = (u32)(off & 3u) * 8u;
-  const u32 d0 = mem.word(wa), d1 = mem.word(wa + 1), d2 = mem.word(wa + 2);
-  const u64 lo = (u64)d0 | ((u64)d1 << 32);
-  const u64 hi = (u64)d2;
-  return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
-}
-
 // blocks of the padded preimage of total bytes: the bytes, 0x80, zeros, the 128-bit bit length
 DEV u32 sha512_nblocks(u64 total) { return (u32)((total + 1 + 16 + 127) >> 7); }
 
 // Schedule words 0..15 of block b (of nblk) of the padded preimage hdr[0..HL) || msg[0..len),
-// hdr given as 16 raw u32 words (only the first HL/4 are used), msg at byte offset msg_off.
+// hdr given as 16 raw u32 words (only the first HL/4 are used), msg at byte offset msg_off of a
+// 4-byte-aligned buffer. The block's message bytes come from one window of 33 aligned words,
+// assembled by funnel shifts (the shift is the same for every block: 128 and HL are multiples of
+// 4). Window loads are clamped to the words that hold message bytes (the word at msg_off for an
+// empty message), so no load branches or leaves the message; bytes past its end are masked off.
 template <int HL, class Mem>
 DEV void sha512_block_words(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 len, u32 b, u32 nblk, u64 w[16]) {
   const u64 total = (u64)HL + len;
+  // byte offset of preimage byte b * 128 in the buffer (below msg_off in block 0)
+  const i64 S = (i64)msg_off + (i64)b * 128 - HL;
+  const i64 wa = S >> 2;  // floor
+  const u32 sh = (u32)(msg_off & 3u) * 8u;
+  const i64 lo = (i64)(msg_off >> 2), hi = (i64)((msg_off + len - (len != 0)) >> 2);
+  u32 raw[33];
+#pragma unroll
+  for (int k = 0; k < 33; k++) {
+    const i64 i = wa + k;
+    raw[k] = mem.word((u64)(i < lo ? lo : (i > hi ? hi : i)));
+  }
 #pragma unroll
   for (int q = 0; q < 16; q++) {
     const u64 o = (u64)b * 128 + 8 * q;  // byte offset in the preimage
@@ -90,8 +94,8 @@ DEV void sha512_block_words(const u32 hdr[16], const Mem& mem, u64 msg_off, u32 
       word = bswap64((u64)hdr[2 * hq] | ((u64)hdr[2 * hq + 1] << 32));
     } else {
       const u64 mo = o - HL;
-      u64 le = 0;
-      if (mo < len) le = load8_le(mem, msg_off + mo);
+      u64 le = (u64)funnel32(raw[2 * q + 1], raw[2 * q], sh) |
+               ((u64)funnel32(raw[2 * q + 2], raw[2 * q + 1], sh) << 32);
       const i64 keep = (i64)len - (i64)mo;  // message bytes in this word
       if (keep < 8) {
         const u64 m = keep <= 0 ? 0ull : ((1ull << (8 * keep)) - 1ull);
