@@ -26,7 +26,21 @@
 namespace sigops {
 
 constexpr int INV_BATCHES = 20;  // 600 divsteps >= 590
+// Random inputs take 517 divsteps on average and at most 533 in a sample of 19,200 (for both
+// secp256k1 moduli); every 64-lane wave of that sample was done within 18 batches. Once g = 0 on
+// every lane of the wave, further batches leave f and d unchanged (u = 2^30, v = 0 and no multiple
+// of M added), so from this batch on the loop ends as soon as the whole wave has converged.
+constexpr int INV_CHECK_FROM = 18;
 constexpr i32 L30 = (1 << 30) - 1;
+
+// true when c holds on every active lane of the wavefront (on the host: c)
+DEV bool wave_all(bool c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __all(c);
+#else
+  return c;
+#endif
+}
 
 // 9 limbs of 29 bits (value < 2^261) -> 9 limbs of 30 bits
 DEV void limbs29_to_30(const fe& a, i32 o[9]) {
@@ -151,6 +165,12 @@ DEV fe finv_gcd(const fe& x) {
   i32 eta = 1;
 #pragma unroll 1
   for (int b = 0; b < INV_BATCHES; b++) {
+    if (b >= INV_CHECK_FROM) {
+      u32 gz = 0;
+#pragma unroll
+      for (int i = 0; i < 9; i++) gz |= (u32)g[i];
+      if (wave_all(gz == 0)) break;
+    }
     i32 u, v, q, r;
     divsteps30(eta, (u32)f[0] | ((u32)f[1] << 30), (u32)g[0] | ((u32)g[1] << 30), u, v, q, r);
     apply_fg(f, g, u, v, q, r);
