@@ -400,6 +400,9 @@ def main():
                         "the kernel executes fewer MACs than that, so frac can exceed 1",
                 "valu_issue_frac": prof["valu_issue_frac"],
                 "traffic": prof["traffic"], "traffic_unit": "B per launch", "traffic_source": prof["source"],
+                # the same traffic over this run's kernel time, against MI355X's 8 TB/s HBM peak
+                "hbm_GBps": round(prof["traffic"] / (kern_ms / 1e3) / 1e9, 1) if prof["traffic"] else None,
+                "hbm_frac": round(prof["traffic"] / (kern_ms / 1e3) / 8e12, 4) if prof["traffic"] else None,
                 "kernel": "k_batch_run", "kernel_ms": round(kern_ms, 3), "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
                 "mac_per_sig": round(mac_sig), "peak_source": "sigops_probe_valu (v_mad_u64_u32)"}
 
