@@ -108,17 +108,24 @@ sigops_host::Pool& host_pool(sigops_ctx* c) {
   return *c->pool;
 }
 
+// a failed transfer: SIGOPS_E_NOMEM when the pinned staging could not grow, else SIGOPS_E_DEVICE
+int xfer_fail(sigops_ctx* c, hipError_t e, const char* where) {
+  if (e == hipErrorOutOfMemory)
+    return fail(c, SIGOPS_E_NOMEM, std::string(where) + ": pinned staging: " + hipGetErrorString(e));
+  return hip_fail(c, e, where);
+}
+
 // caller buffers -> device buffers, queued on the context's stream
 int put(sigops_ctx* c, const std::vector<sigops_host::Up>& segs) {
-  HIPCHK(c, sigops_host::upload(host_pool(c), c->h_up, segs, c->stream));
-  return SIGOPS_OK;
+  const hipError_t e = sigops_host::upload(host_pool(c), c->h_up, segs, c->stream);
+  return e == hipSuccess ? SIGOPS_OK : xfer_fail(c, e, "upload");
 }
 
 // device buffers -> caller buffers after the work queued on the context's stream; returns when
 // they are in place
 int get(sigops_ctx* c, const std::vector<sigops_host::Down>& segs) {
-  HIPCHK(c, sigops_host::download(host_pool(c), c->h_down, segs, c->stream, c->xfer_ev));
-  return SIGOPS_OK;
+  const hipError_t e = sigops_host::download(host_pool(c), c->h_down, segs, c->stream, c->xfer_ev);
+  return e == hipSuccess ? SIGOPS_OK : xfer_fail(c, e, "download");
 }
 
 // pred(i) for every i in [0, n), evaluated by the pool; false if any is false
