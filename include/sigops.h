@@ -198,8 +198,9 @@ int sigops_secp256r1_recover_address(sigops_ctx* ctx, const uint8_t* sigs, const
    With timing on, sigops_batch / sigops_batch_dev launch the stages eagerly (no graph) with HIP
    events recorded on the launch stream between them. sigops_batch_stage_ms waits for the last
    batch and writes SIGOPS_STAGES durations in milliseconds: [0] bucketing (clear, histogram,
-   scan, scatter), [1] Ed25519 challenge hashing, [2] the persistent verification kernel,
-   [3] the careful pass for exceptional additions. ---- */
+   scan, scatter), [1] a separate Ed25519 hashing pass, now always empty (the persistent kernel
+   hashes each Ed25519 tile's challenges itself; the slot keeps the indices stable), [2] the
+   persistent verification kernel, [3] the careful pass for exceptional additions. ---- */
 #define SIGOPS_STAGES 4
 int sigops_set_stage_timing(sigops_ctx* ctx, int on);
 int sigops_batch_stage_ms(sigops_ctx* ctx, float* ms);
