@@ -194,14 +194,14 @@ DEV void ed25519_challenge(const u32 sig_raw[16], const u32 pk_raw[8], const Mem
 }
 
 // verify_strict's extra rules (dalek VerifyingKey::verify_strict): R must decompress and neither R
-// nor A may have small order. Returns ED_OK, ED_BAD_R or ED_SMALL_ORDER. They are evaluated in a
-// pass of their own between the hash and the verifier (registers to spare) and travel to the
-// verifier in bits 29..31 of the challenge's top word: k < l < 2^253 leaves those bits zero.
-DEV u32 ed25519_strict_checks(const u32 sig_raw[16], const u32 pk_raw[8]) {
-  fe ax, ay, rx, ry;
-  (void)ed_decompress(pk_raw, ax, ay);  // an undecodable A is rejected by the verifier first
+// nor A may have small order. The rules on R are evaluated in a pass of their own between the hash
+// and the verifier (registers to spare): this returns ED_OK, ED_BAD_R or ED_SMALL_ORDER, which
+// travels to the verifier in bits 29..31 of the challenge's top word (k < l < 2^253 leaves those
+// bits zero). The rule on A runs in the verifier, which decompresses A anyway.
+DEV u32 ed25519_strict_r_checks(const u32 sig_raw[16]) {
+  fe rx, ry;
   if (!ed_decompress(sig_raw, rx, ry)) return ED_BAD_R;
-  return (ed_small_order(rx, ry) || ed_small_order(ax, ay)) ? ED_SMALL_ORDER : ED_OK;
+  return ed_small_order(rx, ry) ? ED_SMALL_ORDER : ED_OK;
 }
 constexpr u32 ED_K_STRICT_SHIFT = 29;
 
@@ -231,6 +231,8 @@ DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u3
   if (!ok && status == ED_OK) status = ED_BAD_KEY;
   const fe one = fe_from_const(Fe::ONE);
   if (STRICT && status == ED_OK) status = strict_status;
+  // verify_strict: A of small order (checked after R's rules, as dalek checks R first)
+  if (STRICT && status == ED_OK && ed_small_order(x, y)) status = ED_SMALL_ORDER;
   // odd representatives: s even -> l - s with B negated; k even -> k + 1 and a final +A
   const bool b_neg = (s[0] & 1u) == 0;
   const bool k_even = (k[0] & 1u) == 0;
@@ -321,7 +323,7 @@ DEV u32 ed25519_verify_item(const u32 sig_raw[16], const u32 pk_raw[8], const Me
                             Tab& tab, const BTab& bt) {
   u32 k[8];
   ed25519_challenge(sig_raw, pk_raw, mem, msg_off, len, k);
-  if constexpr (STRICT) k[7] |= ed25519_strict_checks(sig_raw, pk_raw) << ED_K_STRICT_SHIFT;
+  if constexpr (STRICT) k[7] |= ed25519_strict_r_checks(sig_raw) << ED_K_STRICT_SHIFT;
   return ed25519_verify_core<STRICT>(sig_raw, pk_raw, k, tab, bt);
 }
 

@@ -5,14 +5,12 @@
 namespace sigops {
 
 // verify_strict: the extra rules of every item into the top bits of its k (ed25519_strict_checks)
-static __global__ __launch_bounds__(256) void k_ed25519_strict(const uint4* __restrict__ sigs,
-                                                               const uint4* __restrict__ pks, u32 n,
+static __global__ __launch_bounds__(256) void k_ed25519_strict(const uint4* __restrict__ sigs, u32 n,
                                                                uint4* __restrict__ kbuf) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    u32 sr[16], pr[8];
+    u32 sr[16];
     load_words(sigs + 4 * (u64)i, sr, 4);
-    load_words(pks + 2 * (u64)i, pr, 2);
-    const u32 st = ed25519_strict_checks(sr, pr);
+    const u32 st = ed25519_strict_r_checks(sr);
     uint4 top = kbuf[2 * (u64)i + 1];
     top.w |= st << ED_K_STRICT_SHIFT;
     kbuf[2 * (u64)i + 1] = top;
@@ -26,7 +24,7 @@ hipError_t launch_ed25519_verify(const void* sigs, const void* pks, void* status
   const u32 want = (n + 255) / 256;
   const u32 g = want < grid ? want : grid;
   if (strict) {
-    k_ed25519_strict<<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, n, (uint4*)kbuf);
+    k_ed25519_strict<<<g, 256, 0, s>>>((const uint4*)sigs, n, (uint4*)kbuf);
     k_ed25519_verify<true><<<g, 256, 0, s>>>((const uint4*)sigs, (const uint4*)pks, (const uint4*)kbuf, (u8*)status,
                                             n, (const uint4*)btab, (uint4*)ctab, slots);
   } else {
