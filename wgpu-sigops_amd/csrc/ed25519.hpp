@@ -207,8 +207,9 @@ constexpr u32 ED_K_STRICT_SHIFT = 29;
 
 // verification given the challenge k (from ed25519_challenge).
 // BTab: load(idx, ypx, ymx, t2d) for (2 idx + 1) B. Tab: per-lane store(e, cached) / load(e).
-// STRICT: dalek VerifyingKey::verify_strict; the status of its extra rules arrives in the top bits
-// of k (ed25519_strict_checks). The final comparison stays on the compressed bytes of R.
+// STRICT: dalek VerifyingKey::verify_strict; the status of its rules on R arrives in the top bits
+// of k (ed25519_strict_r_checks) and A's small order is checked here. The final comparison stays
+// on the compressed bytes of R.
 template <bool STRICT = false, class Tab, class BTab>
 DEV u32 ed25519_verify_core(const u32 sig_raw[16], const u32 pk_raw[8], const u32 k_in[8], Tab& tab,
                             const BTab& bt) {

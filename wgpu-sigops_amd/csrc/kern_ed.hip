@@ -4,7 +4,7 @@
 
 namespace sigops {
 
-// verify_strict: the extra rules of every item into the top bits of its k (ed25519_strict_checks)
+// verify_strict: the rules on R of every item into the top bits of its k (ed25519_strict_r_checks)
 static __global__ __launch_bounds__(256) void k_ed25519_strict(const uint4* __restrict__ sigs, u32 n,
                                                                uint4* __restrict__ kbuf) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
