@@ -21,8 +21,9 @@ def _say(*a):
 
 @pytest.fixture(scope="module")
 def mixed():
-    """200,000 mixed items (about 40 MB of inputs: five staging parts) and their device-path results"""
-    n = 200000
+    """300,000 mixed items (about 60 MB of inputs: several staging parts) and their device-path
+    results"""
+    n = 300000
     with sigops.Context() as c:
         mx = W.mixed(c, n, seed=77)
         out = torch.zeros((n, 64), dtype=torch.uint8, device="cuda:0")
