@@ -219,3 +219,41 @@ def test_batch_full_size_mixed_properties(ctx):
     sub_msgs = np.concatenate([msgs_h[offs[i]:offs[i + 1]] for i in ss]) if len(ss) else np.zeros(0, np.uint8)
     v = O.verify_batch(sigs_h[ss], aux_h[ss], sub_msgs, sub_off)
     assert np.array_equal(v.astype(bool), st_h[ss] == 0)
+
+
+def test_batch_max_size_properties(ctx):
+    """a batch of the context's full capacity (max_batch = 2^22 items, messages up to 256 bytes):
+    every uncorrupted secp item recovers the generator's key, every uncorrupted Ed25519 item
+    verifies and no corrupted item passes; 2048 sampled items match the oracle exactly. One item
+    more is refused before anything runs."""
+    n = 1 << 22
+    _say(f"batch max size n={n}")
+    mx = W.mixed(ctx, n, seed=9, max_len=256)
+    out = torch.zeros((n, 64), dtype=torch.uint8, device="cuda:0")
+    st = torch.full((n,), 0xEE, dtype=torch.uint8, device="cuda:0")
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.batch_dev(mx.schemes.data_ptr(), mx.sigs.data_ptr(), mx.aux.data_ptr(), mx.msgs.data_ptr(),
+                  mx.msg_off.data_ptr(), n, out.data_ptr(), st.data_ptr(), s)
+    torch.cuda.synchronize()
+    out_h, st_h, cor = out.cpu().numpy(), st.cpu().numpy(), mx.corrupted.cpu().numpy()
+    assert (st_h != 0xEE).all()
+    for sc in (0, 1):
+        ii = mx.idx[sc]
+        pubs = mx.parts[sc].pubs.cpu().numpy()
+        ok = cor[ii] == 0
+        assert np.array_equal(out_h[ii][ok], pubs[ok]) and not st_h[ii][ok].any()
+        assert not (out_h[ii][~ok] == pubs[~ok]).all(axis=1).any()
+    ii = mx.idx[2]
+    assert np.array_equal(st_h[ii] == 0, cor[ii] == 0)
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(n, 2048, replace=False))
+    sch = mx.schemes_host
+    d_sample = torch.from_numpy(sample).cuda()
+    sigs_h, aux_h = mx.sigs[d_sample].cpu().numpy(), mx.aux[d_sample].cpu().numpy()
+    for sc in (0, 1):
+        sel = sch[sample] == sc
+        o, t = O.recover_batch(sc, sigs_h[sel], aux_h[sel])
+        assert np.array_equal(o, out_h[sample[sel]]) and np.array_equal(t, st_h[sample[sel]])
+    with pytest.raises(sigops.SigopsError):
+        ctx.batch_dev(mx.schemes.data_ptr(), mx.sigs.data_ptr(), mx.aux.data_ptr(), mx.msgs.data_ptr(),
+                      mx.msg_off.data_ptr(), n + 1, out.data_ptr(), st.data_ptr(), s)
