@@ -257,3 +257,38 @@ def test_batch_max_size_properties(ctx):
     with pytest.raises(sigops.SigopsError):
         ctx.batch_dev(mx.schemes.data_ptr(), mx.sigs.data_ptr(), mx.aux.data_ptr(), mx.msgs.data_ptr(),
                       mx.msg_off.data_ptr(), n + 1, out.data_ptr(), st.data_ptr(), s)
+
+
+def test_long_messages_in_batch_and_plain(ctx):
+    """Ed25519 messages from 0 bytes up to 200 KB (the batch's last bucket holds 40 or more SHA-512
+    blocks, so its lanes hash very different lengths), interleaved with secp items, through the
+    mixed batch and through the plain entry point; every result equals the oracle's"""
+    lens = np.array([0, 1, 111, 112, 4990, 5000, 5100, 8192, 20000, 65537, 100000, 200000] * 6
+                    + list(np.random.default_rng(5).integers(0, 300, 60)), np.int64)
+    ne = len(lens)
+    off_e = W.msg_offsets(lens).astype(np.uint64)
+    _say(f"long messages n_ed={ne} bytes={int(off_e[-1])}")
+    esigs, epks, emsgs, ecor = O.gen_ed25519(4040, ne, off_e, corrupt_ppm=200000)
+    ref_e = O.verify_batch(esigs, epks, emsgs, off_e)
+    valid, st = ctx.ed25519_verify(esigs, epks, emsgs, off_e)
+    assert np.array_equal(valid, ref_e.astype(bool))
+    assert np.array_equal(valid, ecor == 0)
+    # the same items in a mixed batch with secp256k1 items
+    ssigs, sdig, spubs, scor = O.gen_secp(0, 4041, 100, corrupt_ppm=100000)
+    n = ne + 100
+    order = np.random.default_rng(6).permutation(n)
+    is_ed = order < ne
+    schemes = np.where(is_ed, 2, 0).astype(np.uint8)
+    sigs = np.where(is_ed[:, None], esigs[np.minimum(order, ne - 1)], ssigs[np.maximum(order - ne, 0)])
+    aux = np.where(is_ed[:, None], epks[np.minimum(order, ne - 1)], sdig[np.maximum(order - ne, 0)])
+    blens = np.where(is_ed, lens[np.minimum(order, ne - 1)], 0)
+    boff = W.msg_offsets(blens).astype(np.uint64)
+    bmsgs = np.concatenate([emsgs[off_e[j]:off_e[j + 1]] for j in order[is_ed]])
+    out, bst = ctx.batch(schemes, sigs, aux, bmsgs, boff)
+    for i in range(n):
+        j = order[i]
+        if is_ed[i]:
+            assert (bst[i] == 0) == bool(ref_e[j]), i
+        else:
+            o, t = O.recover_batch(0, ssigs[j - ne:j - ne + 1], sdig[j - ne:j - ne + 1])
+            assert bst[i] == t[0] and bytes(out[i]) == bytes(o[0]), i
