@@ -121,7 +121,8 @@ struct sigops_server {
   bool stopping = false;
   bool dispatcher_done = false;
   std::thread dispatcher, completer;
-  std::unique_ptr<sigops_host::Pool> pack_pool;  // used by the dispatcher only
+  std::unique_ptr<sigops_host::Pool> pack_pool;    // used by the dispatcher only
+  std::unique_ptr<sigops_host::Pool> unpack_pool;  // used by the completion thread only
   std::atomic<uint64_t> n_batches{0}, n_items{0};
   int fatal = 0;
 
@@ -273,20 +274,38 @@ void sigops_server::complete_loop() {
     for (Slot* s : ready) {
       const hipError_t e = hipEventSynchronize(s->done_ev);
       const uint64_t t = now_ns();
-      size_t i0 = 0;
-      for (auto& r : s->reqs) {
-        if (e != hipSuccess) {
-          r->err = SIGOPS_E_DEVICE;
-        } else {
-          r->out.assign(s->h_out.p + i0 * 64, s->h_out.p + (i0 + r->n) * 64);
-          r->status.assign(s->h_st.p + i0, s->h_st.p + i0 + r->n);
-          for (size_t k = 0; k < r->n; k++) {
-            const uint8_t lim = r->schemes[k] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY;
-            if (r->status[k] > lim) r->err = SIGOPS_E_CANARY;
+      // each request's results out of the slot's staging, spread over the pool by items
+      const size_t nr = s->reqs.size();
+      std::vector<size_t> ri(nr + 1);
+      ri[0] = 0;
+      for (size_t k = 0; k < nr; k++) ri[k + 1] = ri[k] + s->reqs[k]->n;
+      auto scatter = [&](size_t ka, size_t kb) {
+        for (size_t k = ka; k < kb; k++) {
+          Request& r = *s->reqs[k];
+          const size_t i0 = ri[k];
+          if (e != hipSuccess) {
+            r.err = SIGOPS_E_DEVICE;
+          } else {
+            r.out.assign(s->h_out.p + i0 * 64, s->h_out.p + (i0 + r.n) * 64);
+            r.status.assign(s->h_st.p + i0, s->h_st.p + i0 + r.n);
+            for (size_t j = 0; j < r.n; j++) {
+              const uint8_t lim = r.schemes[j] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY;
+              if (r.status[j] > lim) r.err = SIGOPS_E_CANARY;
+            }
           }
+          r.t_done = t;
         }
-        r->t_done = t;
-        i0 += r->n;
+      };
+      const size_t n = ri[nr];
+      const unsigned nt = unpack_pool->size();
+      if (nt > 1 && n >= 8192 && nr > 1) {
+        unpack_pool->run([&](unsigned q) {
+          const size_t ka = std::lower_bound(ri.begin(), ri.begin() + nr, n * q / nt) - ri.begin();
+          const size_t kb = std::lower_bound(ri.begin(), ri.begin() + nr, n * (q + 1) / nt) - ri.begin();
+          scatter(ka, kb);
+        });
+      } else {
+        scatter(0, nr);
       }
     }
     lk.lock();
@@ -314,6 +333,7 @@ sigops_server* sigops_server_create(const sigops_server_config* cfg_in) {
   if (!S) return nullptr;
   S->cfg = cfg;
   S->pack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
+  S->unpack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
   for (uint32_t k = 0; k < cfg.slots; k++) {
     auto slot = std::make_unique<Slot>();
     sigops_config cc{};
