@@ -220,7 +220,7 @@ void dc_mont_raw(int op, int mod, const u32* a, const u32* b, u32* out) {
 }
 
 // SHA-256 of len bytes placed at byte offset `at` of a word buffer (exercises unaligned starts)
-// memory view that records reads outside the words [lo, hi]
+// memory view that records reads outside the words [lo, hi] (quad: the 16-byte chunks [lo, hi])
 struct MemChecked {
   const u32* w;
   u64 lo, hi;
@@ -232,16 +232,29 @@ struct MemChecked {
     }
     return w[i];
   }
+  void quad(u64 c, u32* r) const {
+    if (c < lo || c > hi) {
+      outside = true;
+      r[0] = r[1] = r[2] = r[3] = 0;
+      return;
+    }
+    memcpy(r, w + 4 * c, 16);
+  }
 };
 
 // returns -1 if the hash read a word other than those holding message bytes (for an empty message:
-// the word at its offset)
-int dc_sha256(const unsigned char* msg, unsigned len, unsigned at, unsigned char* out) {
-  u32* buf = new u32[(at + len + 16) / 4 + 4]();
+// the word at its offset). x4: the 16-byte-load reader, allowed the 16-byte chunks that hold
+// message bytes (for an empty message: the chunk at its offset).
+int dc_sha256(const unsigned char* msg, unsigned len, unsigned at, unsigned char* out, int x4) {
+  u32* buf = new u32[(at + len + 16) / 4 + 8]();
   memcpy((unsigned char*)buf + at, msg, len);
-  MemChecked mem{buf, at / 4u, (at + len - (len != 0)) / 4u, false};
+  const unsigned last = at + len - (len != 0);
+  MemChecked mem{buf, x4 ? at / 16u : at / 4u, x4 ? last / 16u : last / 4u, false};
   u32 d[8];
-  sha256_msg(mem, at, len, d);
+  if (x4)
+    sha256_msg<true>(mem, at, len, d);
+  else
+    sha256_msg(mem, at, len, d);
   memcpy(out, d, 32);
   delete[] buf;
   return mem.outside ? -1 : 0;

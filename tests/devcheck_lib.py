@@ -33,7 +33,7 @@ def lib():
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.dc_mont_raw.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p]
         L.dc_glv_split.argtypes = [u32p, u32p, u32p]
-        L.dc_sha256.argtypes = [c_p, ctypes.c_uint, ctypes.c_uint, c_p]
+        L.dc_sha256.argtypes = [c_p, ctypes.c_uint, ctypes.c_uint, c_p, ctypes.c_int]
         L.dc_secp_verify.argtypes = [ctypes.c_int, c_p, c_p, c_p, ctypes.c_int]
         L.dc_table.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
         L.dc_table.restype = ctypes.c_uint
@@ -116,11 +116,13 @@ def resident_table(which):
     return out
 
 
-def sha256(msg, at=0):
+def sha256(msg, at=0, x4=False):
+    """x4: the reader with 16-byte loads (from a 16-byte-aligned buffer)"""
     out = ctypes.create_string_buffer(32)
-    rc = lib().dc_sha256(bytes(msg), len(msg), at, out)
+    rc = lib().dc_sha256(bytes(msg), len(msg), at, out, int(x4))
     if rc != 0:
-        raise AssertionError(f"SHA-256 of {len(msg)} bytes at offset {at} read outside the message's words")
+        unit = "16-byte chunks" if x4 else "words"
+        raise AssertionError(f"SHA-256 of {len(msg)} bytes at offset {at} read outside the message's {unit}")
     return out.raw
 
 

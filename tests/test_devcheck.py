@@ -104,12 +104,18 @@ def test_glv_split_secp256k1():
 
 def test_sha256_message_new_and_address_forms():
     """the device SHA-256 (built for the host) against hashlib on every length 0..300 and a few
-    long ones, at every byte alignment of the start, and the fixed 64-byte form used for addresses"""
+    long ones, at every byte alignment of the start, with the word reader and with the 16-byte
+    reader (every start offset within a 16-byte chunk: every placement of the block's words),
+    each reading nothing outside the message's words or chunks; and the fixed 64-byte form used
+    for addresses"""
     rnd = random.Random(9)
     for L in list(range(0, 301)) + [1000, 4095, 4096, 4097]:
         msg = bytes(rnd.randrange(256) for _ in range(L))
+        want = hashlib.sha256(msg).digest()
         for at in (0, 1, 2, 3) if L < 80 or L > 999 else (L % 4,):
-            assert D.sha256(msg, at) == hashlib.sha256(msg).digest(), (L, at)
+            assert D.sha256(msg, at) == want, (L, at)
+        for at in range(16) if L < 80 or L > 999 else (L % 16, 16 + (L * 7) % 16):
+            assert D.sha256(msg, at, x4=True) == want, (L, at, "x4")
     for _ in range(50):
         pk = bytes(rnd.randrange(256) for _ in range(64))
         assert D.sha256_64(pk) == hashlib.sha256(pk).digest()

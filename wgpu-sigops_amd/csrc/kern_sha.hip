@@ -73,7 +73,8 @@ static __global__ __launch_bounds__(256) void k_msg_scatter(const u64* __restric
 // queue counter (the lanes of a wave that need one in the same trip take consecutive positions with
 // one atomic) and the message index at that position of the longest-first order. A wave stays busy
 // until the queue runs dry instead of idling behind its longest message. The queue counter must be
-// 0 at launch; every lane leaves once a position >= n comes out.
+// 0 at launch; every lane leaves once a position >= n comes out. X4: msgs is 16-byte aligned.
+template <bool X4>
 static __global__ __launch_bounds__(256) void k_message_new(const u32* __restrict__ msgs,
                                                            const u64* __restrict__ off, u32 n,
                                                            uint4* __restrict__ out, u32* __restrict__ work,
@@ -105,7 +106,10 @@ static __global__ __launch_bounds__(256) void k_message_new(const u32* __restric
       }
     }
     u32 w[16];
-    sha256_block_words(mem, o, len, b, nblk, w);
+    if constexpr (X4)
+      sha256_block_words_x4(mem, o, len, b, nblk, w);
+    else
+      sha256_block_words(mem, o, len, b, nblk, w);
     sha256_compress(h, w);
     if (++b == nblk) {
       out[2 * (u64)i] = make_uint4(bswap32(h[0]), bswap32(h[1]), bswap32(h[2]), bswap32(h[3]));
@@ -140,7 +144,10 @@ hipError_t launch_message_new(const void* msgs, const u64* off, u32 n, void* out
   k_msg_hist<<<grid_for(n, grid), 256, 0, s>>>(off, n, work);
   k_msg_scan<<<1, 64, 0, s>>>(work);
   k_msg_scatter<<<grid_for(n, grid), 256, 0, s>>>(off, n, work, order);
-  k_message_new<<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out, work, order);
+  if (((uintptr_t)msgs & 15u) == 0)
+    k_message_new<true><<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out, work, order);
+  else
+    k_message_new<false><<<grid_for(n, grid), 256, 0, s>>>((const u32*)msgs, off, n, (uint4*)out, work, order);
   return hipGetLastError();
 }
 

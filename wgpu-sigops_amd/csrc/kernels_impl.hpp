@@ -145,6 +145,14 @@ struct GTabGlobal {
 struct MemDev {
   const u32* w;
   DEV u32 word(u64 i) const { return w[i]; }
+  // 16-byte chunk c (the buffer must be 16-byte aligned)
+  DEV void quad(u64 c, u32* r) const {
+    const uint4 v = ((const uint4*)w)[c];
+    r[0] = v.x;
+    r[1] = v.y;
+    r[2] = v.z;
+    r[3] = v.w;
+  }
 };
 
 DEV void load_words(const uint4* p, u32* w, int nq) {

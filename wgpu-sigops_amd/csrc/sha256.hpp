@@ -4,10 +4,11 @@
 // recovery path: src/tests/ed25519_eddsa.rs:19, src/benchmarks/secp256k1_ecdsa.rs:162) and derives
 // an address as SHA-256 of the 64-byte public key X || Y (the step after it). Both run here on
 // the device, so data stays resident from raw bytes to recovered addresses.
-// A padded block is generated word by word from memory at any byte offset (17 aligned loads and
-// funnel shifts, then 0x80, zeros and the 64-bit bit length); the 64 rounds run 16 per loop trip
-// over a circular schedule. The 3-input bitwise functions (Ch, Maj and the xor of three rotations)
-// are one v_bitop3_b32 each on gfx950.
+// A padded block is generated word by word from memory at any byte offset (17 aligned word loads,
+// or 5 aligned 16-byte loads from a 16-byte-aligned buffer, and funnel shifts; then 0x80, zeros
+// and the 64-bit bit length); the 64 rounds run 16 per loop trip over a circular schedule. The
+// 3-input bitwise functions (Ch, Maj and the xor of three rotations) are one v_bitop3_b32 each on
+// gfx950.
 #pragma once
 #include "field.hpp"
 
@@ -88,21 +89,13 @@ DEV void sha256_compress_kw(u32 h[8], const u32 (&kw)[64]) {
 // blocks of the padded message of len bytes: the bytes, 0x80, zeros to 56 mod 64, the bit length
 DEV u32 sha256_nblocks(u32 len) { return (u32)(((u64)len + 1 + 8 + 63) >> 6); }
 
-// Schedule words 0..15 of block b (of nblk) of the padded message msg[0 .. len) that starts at
-// byte offset off of a 4-byte-aligned buffer. Loads are clamped to the aligned word holding the
-// message's last byte (the word at off for an empty message), so they never reach more than 3
-// bytes past the message and need no branch; the bytes they bring in past its end are masked off.
-template <class Mem>
-DEV void sha256_block_words(const Mem& mem, u64 off, u32 len, u32 b, u32 nblk, u32 w[16]) {
-  const u64 base = off + (u64)b * 64;
+// Schedule words 0..15 of block b (of nblk) of the padded message of len bytes, from raw[0..16]:
+// the 17 aligned little-endian words from the one holding the block's first byte, whose byte
+// offset within that word gives sh = 8 * offset. Bytes past the message are masked off, whatever
+// was loaded for them.
+DEV void sha256_block_assemble(const u32 raw[17], u32 sh, u32 len, u32 b, u32 nblk, u32 w[16]) {
   const i64 rem64 = (i64)len - (i64)b * 64;  // message bytes from the block start
   const i32 rem = rem64 > 64 ? 64 : (rem64 < -1 ? -1 : (i32)rem64);
-  const u32 sh = (u32)(base & 3u) * 8u;
-  const u64 wa = base >> 2;
-  const u64 lim = (off + len - (len != 0)) >> 2;
-  u32 raw[17];
-#pragma unroll
-  for (int k = 0; k < 17; k++) raw[k] = mem.word(wa + k < lim ? wa + k : lim);
   const u64 bits = (u64)len * 8u;
 #pragma unroll
   for (int q = 0; q < 16; q++) {
@@ -120,8 +113,51 @@ DEV void sha256_block_words(const Mem& mem, u64 off, u32 len, u32 b, u32 nblk, u
   }
 }
 
-// SHA-256(msg[0..len)), msg at byte offset off; out: the 32 digest bytes as 8 little-endian words
+// Block b of the message msg[0 .. len) at byte offset off of a 4-byte-aligned buffer, one word
+// load per aligned word. Loads are clamped to the aligned word holding the message's last byte
+// (the word at off for an empty message), so they never reach more than 3 bytes past the message
+// and need no branch.
 template <class Mem>
+DEV void sha256_block_words(const Mem& mem, u64 off, u32 len, u32 b, u32 nblk, u32 w[16]) {
+  const u64 base = off + (u64)b * 64;
+  const u64 wa = base >> 2;
+  const u64 lim = (off + len - (len != 0)) >> 2;
+  u32 raw[17];
+#pragma unroll
+  for (int k = 0; k < 17; k++) raw[k] = mem.word(wa + k < lim ? wa + k : lim);
+  sha256_block_assemble(raw, (u32)(base & 3u) * 8u, len, b, nblk, w);
+}
+
+// The same from a 16-byte-aligned buffer with five 16-byte loads per lane instead of 17 word loads
+// (a third of the load instructions when each lane hashes its own message). Loads are clamped to
+// the 16-byte chunk holding the message's last byte (the chunk at off for an empty message), so
+// they reach at most 15 bytes past the message. mem.quad(c, r) reads the four words of chunk c.
+// The block's words are placed with lane masks; a version with conditional selects measured
+// twice as slow on messages of mixed lengths.
+template <class Mem>
+DEV void sha256_block_words_x4(const Mem& mem, u64 off, u32 len, u32 b, u32 nblk, u32 w[16]) {
+  const u64 base = off + (u64)b * 64;
+  const u64 c0 = base >> 4;
+  const u64 cl = (off + len - (len != 0)) >> 4;
+  u32 r[20];
+#pragma unroll
+  for (int k = 0; k < 5; k++) mem.quad(c0 + k < cl ? c0 + k : cl, r + 4 * k);
+  // the block starts at word q0 = 0..3 of the first chunk: raw[j] = r[q0 + j], by lane masks
+  const u32 q0 = (u32)(base >> 2) & 3u;
+  const u32 m1 = 0u - (q0 & 1u), m2 = 0u - ((q0 >> 1) & 1u);
+  u32 raw[17];
+#pragma unroll
+  for (int j = 0; j < 17; j++) {
+    const u32 even = (r[j + 2] & m2) | (r[j] & ~m2);
+    const u32 odd = (r[j + 3] & m2) | (r[j + 1] & ~m2);
+    raw[j] = (odd & m1) | (even & ~m1);
+  }
+  sha256_block_assemble(raw, (u32)(base & 3u) * 8u, len, b, nblk, w);
+}
+
+// SHA-256(msg[0..len)), msg at byte offset off; out: the 32 digest bytes as 8 little-endian words
+// (X4: the buffer is 16-byte aligned and read with 16-byte loads)
+template <bool X4 = false, class Mem>
 DEV void sha256_msg(const Mem& mem, u64 off, u32 len, u32 out[8]) {
   u32 h[8];
 #pragma unroll
@@ -129,7 +165,10 @@ DEV void sha256_msg(const Mem& mem, u64 off, u32 len, u32 out[8]) {
   const u32 nblk = sha256_nblocks(len);
   for (u32 b = 0; b < nblk; b++) {
     u32 w[16];
-    sha256_block_words(mem, off, len, b, nblk, w);
+    if constexpr (X4)
+      sha256_block_words_x4(mem, off, len, b, nblk, w);
+    else
+      sha256_block_words(mem, off, len, b, nblk, w);
     sha256_compress(h, w);
   }
 #pragma unroll
