@@ -70,7 +70,7 @@ hipError_t launch_ed_hash_bucketed(const BatchLaunch& L, hipStream_t s);
 // work: MSG_WORK_WORDS device words (cleared by the launch); order: n words, the longest-first
 // processing order of the messages
 constexpr uint32_t MSG_BINS = 66;  // message ordering bins by SHA-256 block count (66 = "66 or more")
-constexpr uint32_t MSG_WORK_WORDS = 2 + 2 * MSG_BINS;
+constexpr uint32_t MSG_WORK_WORDS = 1 + 2 * MSG_BINS;
 hipError_t launch_message_new(const void* msgs, const uint64_t* off, uint32_t n, void* out, uint32_t* work,
                               uint32_t* order, uint32_t grid, hipStream_t s);
 hipError_t launch_fuel_address(const void* pubs, const void* status, uint32_t n, void* out, uint32_t grid,
