@@ -43,27 +43,36 @@ static __global__ __launch_bounds__(64) void k_msg_scan(u32* __restrict__ work) 
   work[MW_FLAG] = used > 1;
 }
 
-// ranks within each workgroup's chunk of 256 messages by LDS atomics, then one global atomic per
-// bin and chunk reserves the chunk's range in that bin
+// ranks within each workgroup's chunk of 1024 messages (4 per thread) by LDS atomics, then one
+// global atomic per bin and chunk reserves the chunk's range in that bin (the chunks are large
+// because those atomics, on 66 counters shared by every workgroup, are what the scatter waits on)
 static __global__ __launch_bounds__(256) void k_msg_scatter(const u64* __restrict__ off, u32 n, u32* __restrict__ work,
                                                            u32* __restrict__ order) {
+  constexpr u32 PER = 4;
   __shared__ u32 cnt[MSG_BINS];
   __shared__ u32 base[MSG_BINS];
   if (work[MW_FLAG] == 0) return;
-  for (u32 c0 = blockIdx.x * blockDim.x; c0 < n; c0 += gridDim.x * blockDim.x) {
+  for (u32 c0 = blockIdx.x * blockDim.x * PER; c0 < n; c0 += gridDim.x * blockDim.x * PER) {
     for (u32 t = threadIdx.x; t < MSG_BINS; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
-    const u32 i = c0 + threadIdx.x;
-    u32 b = MSG_BINS, rank = 0;
-    if (i < n) {
-      b = msg_bin(off, i);
-      rank = atomicAdd(&cnt[b], 1u);
+    u32 bq[PER], rq[PER];
+#pragma unroll
+    for (u32 q = 0; q < PER; q++) {
+      const u32 i = c0 + q * blockDim.x + threadIdx.x;
+      bq[q] = MSG_BINS;
+      rq[q] = 0;
+      if (i < n) {
+        bq[q] = msg_bin(off, i);
+        rq[q] = atomicAdd(&cnt[bq[q]], 1u);
+      }
     }
     __syncthreads();
     for (u32 t = threadIdx.x; t < MSG_BINS; t += blockDim.x)
       if (cnt[t]) base[t] = atomicAdd(&work[MW_CURSOR + t], cnt[t]);
     __syncthreads();
-    if (b < MSG_BINS) order[base[b] + rank] = i;
+#pragma unroll
+    for (u32 q = 0; q < PER; q++)
+      if (bq[q] < MSG_BINS) order[base[bq[q]] + rq[q]] = c0 + q * blockDim.x + threadIdx.x;
     __syncthreads();
   }
 }
