@@ -93,31 +93,40 @@ __global__ __launch_bounds__(64) void k_bucket_scan(BatchArgs a) {
   }
 }
 
-// Each workgroup ranks its chunk of 256 items per bucket with LDS atomics, reserves one range per
-// bucket with a single global atomic, then writes the indices: one global atomic per bucket and
-// chunk instead of one per item on a handful of hot counters.
+// Each workgroup ranks its chunk of 1024 items (4 per thread) per bucket with LDS atomics,
+// reserves one range per bucket with a single global atomic, then writes the indices: one global
+// atomic per bucket and chunk instead of one per item, on a handful of counters shared by every
+// workgroup.
 __global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
+  constexpr u32 PER = 4;
   __shared__ u32 cnt[NBUCKETS];
   __shared__ u32 base[NBUCKETS];
   const u32 n = *a.n_ptr;
-  for (u32 c0 = blockIdx.x * blockDim.x; c0 < n; c0 += gridDim.x * blockDim.x) {
+  for (u32 c0 = blockIdx.x * blockDim.x * PER; c0 < n; c0 += gridDim.x * blockDim.x * PER) {
     for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
-    const u32 i = c0 + threadIdx.x;
-    u32 b = NBUCKETS, rank = 0;
-    if (i < n) {
-      if (item_scheme(a, i) > 2) {
-        a.status[i] = 0xFD;  // unknown scheme: reported as a canary failure by the host
-      } else {
-        b = item_bucket(a, i);
-        rank = atomicAdd(&cnt[b], 1u);
+    u32 bq[PER], rq[PER];
+#pragma unroll
+    for (u32 q = 0; q < PER; q++) {
+      const u32 i = c0 + q * blockDim.x + threadIdx.x;
+      bq[q] = NBUCKETS;
+      rq[q] = 0;
+      if (i < n) {
+        if (item_scheme(a, i) > 2) {
+          a.status[i] = 0xFD;  // unknown scheme: reported as a canary failure by the host
+        } else {
+          bq[q] = item_bucket(a, i);
+          rq[q] = atomicAdd(&cnt[bq[q]], 1u);
+        }
       }
     }
     __syncthreads();
     for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x)
       if (cnt[t]) base[t] = atomicAdd(&a.cursor[t], cnt[t]);
     __syncthreads();
-    if (b < NBUCKETS) a.order[base[b] + rank] = i;
+#pragma unroll
+    for (u32 q = 0; q < PER; q++)
+      if (bq[q] < NBUCKETS) a.order[base[bq[q]] + rq[q]] = c0 + q * blockDim.x + threadIdx.x;
     __syncthreads();
   }
 }
