@@ -51,11 +51,13 @@ struct BTabHost {
 struct STabHost {
   jac t[8];
   fe before[8];
-  fe ax[8], ay[8];
+  fe ax[8], ay[8], abx[8];
   void store_full(int e, const jac& p, const fe& b) { t[e] = p; before[e] = b; }
   void load_full(u32 e, jac& p, fe& b) const { p = t[e]; b = before[e]; }
   void store_affine(int e, const fe& x, const fe& y) { ax[e] = x; ay[e] = y; }
   void load_affine(u32 e, fe& x, fe& y) const { x = ax[e]; y = ay[e]; }
+  void store_affine_phi(int e, const fe& x, const fe& y, const fe& bx) { ax[e] = x; ay[e] = y; abx[e] = bx; }
+  void load_affine_phi(u32 e, fe& bx, fe& y) const { bx = abx[e]; y = ay[e]; }
 };
 struct CTabHost {
   cached t[8];

@@ -72,9 +72,12 @@ struct LaneTab {
       base[((u64)e * CH + c) * slots + slot] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
   }
   template <int N>
-  DEV void get_n(u32 e, u32* w) const {
+  DEV void get_n(u32 e, u32* w) const { get_range<0, N>(e, w); }
+  // chunks A .. A+N-1 of entry e into w[4A ..]
+  template <int A, int N>
+  DEV void get_range(u32 e, u32* w) const {
 #pragma unroll
-    for (int c = 0; c < N; c++) {
+    for (int c = A; c < A + N; c++) {
       const uint4 q = base[((u64)e * CH + c) * slots + slot];
       w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
     }
@@ -83,7 +86,8 @@ struct LaneTab {
 
 // secp R-table (secp_build_table): while building, entry e holds a Jacobian point and the running
 // product of the Z coordinates before it (36 words, 9 chunks); afterwards its first 5 chunks hold
-// the affine (x, y) the ladder reads
+// the affine (x, y) the ladder reads, and for secp256k1 chunks 4..6 also hold beta * x, so that
+// the phi(R) stream reads (beta x, y) from chunks 2..6 (five loads, as for (x, y))
 struct STabDev : LaneTab<9> {
   DEV void store_full(int e, const jac& p, const fe& before) {
     u32 w[36];
@@ -113,6 +117,19 @@ struct STabDev : LaneTab<9> {
     get_n<5>(e, w);
 #pragma unroll
     for (int k = 0; k < NL; k++) { x.v[k] = w[k]; y.v[k] = w[9 + k]; }
+  }
+  DEV void store_affine_phi(int e, const fe& x, const fe& y, const fe& bx) {
+    u32 w[28];
+#pragma unroll
+    for (int k = 0; k < NL; k++) { w[k] = x.v[k]; w[9 + k] = y.v[k]; w[18 + k] = bx.v[k]; }
+    w[27] = 0;
+    put_n<7>(e, w);
+  }
+  DEV void load_affine_phi(u32 e, fe& bx, fe& y) const {
+    u32 w[28];
+    get_range<2, 5>(e, w);
+#pragma unroll
+    for (int k = 0; k < NL; k++) { y.v[k] = w[9 + k]; bx.v[k] = w[18 + k]; }
   }
 };
 

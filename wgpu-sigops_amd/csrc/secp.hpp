@@ -457,7 +457,8 @@ DEV u32 nibble(const u32 c[8], int i) { return (c[i >> 3] >> ((i & 7) * 4)) & 15
 DEV u32 byte8(const u32 c[8], int j) { return (c[j >> 2] >> ((j & 3) * 8)) & 255u; }
 
 // GLV ladder (secp256k1): Q = h1 G + h2 phi(G) + h3 R + h4 phi(R) on one doubling chain of 132
-// bits, with phi(x, y) = (beta x, y) applied to table entries as they are loaded. R streams use 33
+// bits, with phi(x, y) = (beta x, y): the per-lane table stores beta x beside x, the generator
+// table's entries are multiplied by beta as they are loaded. R streams use 33
 // windows of 4 bits over the per-lane table of odd multiples of R; G streams 17 windows of 8 bits
 // over the table of odd multiples of G, added at every second R window.
 template <class C, class Tab, class GTab>
@@ -475,9 +476,9 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
     acc.Z = fe_from_const(F::ONE);
   }
   {
-    fe ex, ey;
-    tab.load_affine(nibble(o.cr2, 32) - 8, ex, ey);
-    acc = jmadd<C>(acc, mont_mul<F>(ex, beta), fneg_if<F>(o.sr2, ey), exc);
+    fe bx, ey;
+    tab.load_affine_phi(nibble(o.cr2, 32) - 8, bx, ey);
+    acc = jmadd<C>(acc, bx, fneg_if<F>(o.sr2, ey), exc);
   }
 #pragma unroll 1
   for (int i = 32; i >= 0; i--) {
@@ -489,9 +490,9 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
       tab.load_affine((c1 < 8 ? ~c1 : c1) & 7u, x1, y1);
 #pragma unroll 1
       for (int d = 0; d < 4; d++) acc = jdbl<C>(acc);
-      tab.load_affine((c2 < 8 ? ~c2 : c2) & 7u, x2, y2);
+      tab.load_affine_phi((c2 < 8 ? ~c2 : c2) & 7u, x2, y2);  // x2 = beta x
       acc = jmadd<C>(acc, x1, fneg_if<F>((c1 < 8) != o.sr1, y1), exc);
-      acc = jmadd<C>(acc, mont_mul<F>(x2, beta), fneg_if<F>((c2 < 8) != o.sr2, y2), exc);
+      acc = jmadd<C>(acc, x2, fneg_if<F>((c2 < 8) != o.sr2, y2), exc);
     }
     if ((i & 1) == 0) {
       fe gx, gy;
@@ -522,7 +523,12 @@ DEV void secp_build_table(const secp_pro& o, Tab& tab, bool& exc) {
   P.Y = o.ry;
   P.Z = fe_from_const(F::ONE);
   const jac P2 = jdbl<C>(P);
-  tab.store_affine(0, P.X, P.Y);
+  // secp256k1 (GLV): beta x beside each entry, for the phi(R) stream
+  const fe beta = C::GLV ? fe_from_const(K1_BETA) : fe_zero();
+  if constexpr (C::GLV)
+    tab.store_affine_phi(0, P.X, P.Y, mont_mul<F>(P.X, beta));
+  else
+    tab.store_affine(0, P.X, P.Y);
   jac T = P;
   fe before = P.Z;  // Z_1 * ... * Z_(e-1)
 #pragma unroll 1
@@ -540,7 +546,11 @@ DEV void secp_build_table(const secp_pro& o, Tab& tab, bool& exc) {
     const fe zi = mont_mul<F>(inv, b);  // 1 / Z_e
     inv = mont_mul<F>(inv, E.Z);        // 1 / (Z_1 * ... * Z_(e-1))
     const fe zi2 = mont_sqr<F>(zi);
-    tab.store_affine(e, mont_mul<F>(E.X, zi2), mont_mul<F>(E.Y, mont_mul<F>(zi2, zi)));
+    const fe ax = mont_mul<F>(E.X, zi2), ay = mont_mul<F>(E.Y, mont_mul<F>(zi2, zi));
+    if constexpr (C::GLV)
+      tab.store_affine_phi(e, ax, ay, mont_mul<F>(ax, beta));
+    else
+      tab.store_affine(e, ax, ay);
   }
 }
 
