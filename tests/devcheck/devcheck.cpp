@@ -34,6 +34,12 @@ struct GTabHost {
     canonical_bounds<F>(x);
     canonical_bounds<F>(y);
   }
+  // the LDS staging's product (stage_gtab): beta times the canonical x
+  void load_phi(u32 idx, fe& bx, fe& y) const {
+    fe x;
+    load(idx, x, y);
+    bx = mont_mul<F>(x, fe_from_const(K1_BETA));
+  }
 };
 struct BTabHost {
   void load(u32 idx, fe& ypx, fe& ymx, fe& t2d) const {

@@ -182,13 +182,11 @@ DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) 
 __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
                                                    const uint4* __restrict__ gr1, const uint4* __restrict__ gb,
                                                    uint4* __restrict__ lane_tab, u32 slots) {
-  __shared__ uint4 s_k1[GTAB_ENTRIES * 5];
-  __shared__ uint4 s_r1[GTAB_ENTRIES * 5];
+  __shared__ uint4 s_k1[GTAB_ENTRIES * gtab_chunks<CurveK1>()];
+  __shared__ uint4 s_r1[GTAB_ENTRIES * gtab_chunks<CurveR1>()];
   __shared__ uint4 s_b[BTAB_ENTRIES * 7];
-  for (int t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) {
-    s_k1[t] = gk1[t];
-    s_r1[t] = gr1[t];
-  }
+  stage_gtab<CurveK1>(s_k1, gk1);
+  stage_gtab<CurveR1>(s_r1, gr1);
   for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
   __syncthreads();
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,9 +215,9 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArg
       const u32 i = a.order[(u64)t * a.tile_items + k + lane];
       if (i == DUMMY) continue;
       if (b == BUCKET_K1) {
-        batch_secp_item<CurveK1>(a, i, jt, GTabLds{s_k1});
+        batch_secp_item<CurveK1>(a, i, jt, GTabOf<CurveK1>{s_k1});
       } else if (b == BUCKET_R1) {
-        batch_secp_item<CurveR1>(a, i, jt, GTabLds{s_r1});
+        batch_secp_item<CurveR1>(a, i, jt, GTabOf<CurveR1>{s_r1});
       } else {
         // the challenge is hashed here, inside the persistent kernel: the tile's items have equal
         // SHA-512 block counts (bucketed), and the hash's memory latency hides behind the other

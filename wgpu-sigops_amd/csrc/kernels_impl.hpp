@@ -25,19 +25,70 @@
 namespace sigops {
 
 // ---- table accessors ----
-struct GTabLds {  // secp fixed-base: entry = 5 x uint4 (x[9], y[9], pad[2])
+// secp fixed-base table in LDS. CH = 5: entry = 5 x uint4 (x[9], y[9], pad[2]), the format of the
+// table in HBM. CH = 7 (secp256k1, staged by stage_gtab): (x[9], y[9], beta x[9], pad), so that
+// load_phi reads (beta x, y) = phi(entry) from chunks 2..6, as many loads as load's chunks 0..4.
+template <int CH>
+struct GTabLdsN {
   const uint4* t;
-  DEV void load(u32 idx, fe& x, fe& y) const {
-    u32 w[20];
+  template <int A>
+  DEV void chunks(u32 idx, u32* w) const {
 #pragma unroll
-    for (int c = 0; c < 5; c++) {
-      const uint4 q = t[idx * 5 + c];
+    for (int c = A; c < A + 5; c++) {
+      const uint4 q = t[idx * CH + c];
       w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
     }
+  }
+  DEV void load(u32 idx, fe& x, fe& y) const {
+    u32 w[20];
+    chunks<0>(idx, w);
 #pragma unroll
     for (int k = 0; k < NL; k++) { x.v[k] = w[k]; y.v[k] = w[9 + k]; }
   }
+  DEV void load_phi(u32 idx, fe& bx, fe& y) const {
+    static_assert(CH == 7, "phi entries need the 7-chunk layout");
+    u32 w[28];
+    chunks<2>(idx, w);
+#pragma unroll
+    for (int k = 0; k < NL; k++) { y.v[k] = w[9 + k]; bx.v[k] = w[18 + k]; }
+  }
 };
+typedef GTabLdsN<5> GTabLds;
+
+// the LDS table type of curve C: secp256k1 (GLV) carries beta x
+template <class C>
+using GTabOf = GTabLdsN<C::GLV ? 7 : 5>;
+template <class C>
+constexpr int gtab_chunks() { return C::GLV ? 7 : 5; }
+
+// copies the HBM table g (5 chunks per entry) into LDS s in curve C's layout, computing beta x
+// for secp256k1 (one product per entry, GTAB_ENTRIES per workgroup); the caller syncs afterwards
+template <class C>
+DEV void stage_gtab(uint4* s, const uint4* g) {
+  if constexpr (C::GLV) {
+    typedef typename C::Fp F;
+    const fe beta = fe_from_const(K1_BETA);
+    for (u32 e = threadIdx.x; e < GTAB_ENTRIES; e += blockDim.x) {
+      u32 w[28];
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        const uint4 q = g[e * 5 + c];
+        w[4 * c] = q.x; w[4 * c + 1] = q.y; w[4 * c + 2] = q.z; w[4 * c + 3] = q.w;
+      }
+      fe x;
+#pragma unroll
+      for (int k = 0; k < NL; k++) x.v[k] = w[k];
+      const fe bx = mont_mul<F>(x, beta);
+#pragma unroll
+      for (int k = 0; k < NL; k++) w[18 + k] = bx.v[k];
+      w[27] = 0;
+#pragma unroll
+      for (int c = 0; c < 7; c++) s[e * 7 + c] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
+    }
+  } else {
+    for (u32 t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) s[t] = g[t];
+  }
+}
 
 struct BTabLds {  // Ed25519 fixed-base: entry = 7 x uint4 (ypx[9], ymx[9], t2d[9], pad)
   const uint4* t;
@@ -187,10 +238,10 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_recover(const
                                                       const uint4* __restrict__ gtab, uint4* __restrict__ rtab,
                                                       u32 slots, u32* __restrict__ pend_list,
                                                       u32* __restrict__ pend_count) {
-  __shared__ uint4 sg[GTAB_ENTRIES * 5];
-  for (int t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) sg[t] = gtab[t];
+  __shared__ uint4 sg[GTAB_ENTRIES * gtab_chunks<C>()];
+  stage_gtab<C>(sg, gtab);
   __syncthreads();
-  const GTabLds gt{sg};
+  const GTabOf<C> gt{sg};
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
   STabDev tab;
   tab.base = rtab;
@@ -262,10 +313,10 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_verify(
     const uint4* __restrict__ sigs, const uint4* __restrict__ digs, const uint4* __restrict__ pubs,
     u8* __restrict__ status, u32 n, const uint4* __restrict__ gtab, uint4* __restrict__ rtab, u32 slots,
     u32* __restrict__ pend_list, u32* __restrict__ pend_count) {
-  __shared__ uint4 sg[GTAB_ENTRIES * 5];
-  for (int t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) sg[t] = gtab[t];
+  __shared__ uint4 sg[GTAB_ENTRIES * gtab_chunks<C>()];
+  stage_gtab<C>(sg, gtab);
   __syncthreads();
-  const GTabLds gt{sg};
+  const GTabOf<C> gt{sg};
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
   STabDev tab;
   tab.base = rtab;

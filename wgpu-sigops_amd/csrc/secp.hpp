@@ -457,14 +457,13 @@ DEV u32 nibble(const u32 c[8], int i) { return (c[i >> 3] >> ((i & 7) * 4)) & 15
 DEV u32 byte8(const u32 c[8], int j) { return (c[j >> 2] >> ((j & 3) * 8)) & 255u; }
 
 // GLV ladder (secp256k1): Q = h1 G + h2 phi(G) + h3 R + h4 phi(R) on one doubling chain of 132
-// bits, with phi(x, y) = (beta x, y): the per-lane table stores beta x beside x, the generator
-// table's entries are multiplied by beta as they are loaded. R streams use 33
+// bits, with phi(x, y) = (beta x, y): both tables store beta x beside x (Tab::load_affine_phi,
+// GTab::load_phi read (beta x, y)). R streams use 33
 // windows of 4 bits over the per-lane table of odd multiples of R; G streams 17 windows of 8 bits
 // over the table of odd multiples of G, added at every second R window.
 template <class C, class Tab, class GTab>
 DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
   typedef typename C::Fp F;
-  const fe beta = fe_from_const(K1_BETA);
   // top windows: every stream's top digit is positive; only the stream signs apply
   // the accumulator enters jmadd as a subtrahend, so its coordinates are folded to < (1 + 2^-19) M
   jac acc;
@@ -500,9 +499,9 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
       gt.load(((b1 < 128) ? ~b1 : b1) & 127u, gx, gy);
       gy = fneg_if<F>((b1 < 128) != o.sg1, gy);
       acc = jmadd<C>(acc, gx, gy, exc);
-      gt.load(((b2 < 128) ? ~b2 : b2) & 127u, gx, gy);
+      gt.load_phi(((b2 < 128) ? ~b2 : b2) & 127u, gx, gy);  // gx = beta x
       gy = fneg_if<F>((b2 < 128) != o.sg2, gy);
-      acc = jmadd<C>(acc, mont_mul<F>(gx, beta), gy, exc);
+      acc = jmadd<C>(acc, gx, gy, exc);
     }
   }
   return acc;
