@@ -514,6 +514,11 @@ DEV jac secp_ladder_glv(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) 
 // product then yields every 1/Z on the way back down, and each entry is rewritten in place as
 // (x, y). Tab: store_full(e, jac, before) / load_full(e, jac&, before&) and store_affine(e, x, y)
 // / load_affine(e, x&, y&).
+// The entries are built on an isomorphic curve where 2P is affine, so the seven additions are
+// mixed ones too: with 2P = (X2, Y2, Z2), the map (x, y) -> (Z2^2 x, Z2^3 y) takes the curve to
+// y^2 = x^3 + a Z2^4 x + b Z2^6, 2P to the affine (X2, Y2) and P to (Z2^2 x, Z2^3 y). The
+// addition formulas do not involve a or b. A Jacobian (X, Y, Z) there is (X, Y, Z Z2) on the
+// curve, so Z2 joins the product that is inverted.
 template <class C, class Tab>
 DEV void secp_build_table(const secp_pro& o, Tab& tab, bool& exc) {
   typedef typename C::Fp F;
@@ -528,22 +533,27 @@ DEV void secp_build_table(const secp_pro& o, Tab& tab, bool& exc) {
     tab.store_affine_phi(0, P.X, P.Y, mont_mul<F>(P.X, beta));
   else
     tab.store_affine(0, P.X, P.Y);
-  jac T = P;
-  fe before = P.Z;  // Z_1 * ... * Z_(e-1)
+  // P on the isomorphic curve; the accumulator enters jmadd as a subtrahend, so it is folded
+  const fe zz = mont_sqr<F>(P2.Z);
+  jac T;
+  T.X = fnorm_red<F>(mont_mul<F>(P.X, zz));
+  T.Y = fnorm_red<F>(mont_mul<F>(P.Y, mont_mul<F>(zz, P2.Z)));
+  T.Z = P.Z;
+  fe before = P.Z;  // Z_1 * ... * Z_(e-1), isomorphic-curve Z coordinates
 #pragma unroll 1
   for (int e = 1; e < 8; e++) {
-    T = jadd<C>(T, P2, exc);
+    T = jmadd<C>(T, P2.X, P2.Y, exc);
     tab.store_full(e, T, before);
     before = mont_mul<F>(before, T.Z);
   }
-  fe inv = finv<F>(before);  // 1 / (Z_1 * ... * Z_7)
+  fe inv = finv<F>(mont_mul<F>(before, P2.Z));  // 1 / (Z2 * Z_1 * ... * Z_7)
 #pragma unroll 1
   for (int e = 7; e >= 1; e--) {
     jac E;
     fe b;
     tab.load_full(e, E, b);
-    const fe zi = mont_mul<F>(inv, b);  // 1 / Z_e
-    inv = mont_mul<F>(inv, E.Z);        // 1 / (Z_1 * ... * Z_(e-1))
+    const fe zi = mont_mul<F>(inv, b);  // 1 / (Z2 * Z_e), the curve's 1 / Z
+    inv = mont_mul<F>(inv, E.Z);        // 1 / (Z2 * Z_1 * ... * Z_(e-1))
     const fe zi2 = mont_sqr<F>(zi);
     const fe ax = mont_mul<F>(E.X, zi2), ay = mont_mul<F>(E.Y, mont_mul<F>(zi2, zi));
     if constexpr (C::GLV)
