@@ -8,6 +8,8 @@ ed25519: s >= l, small-order and mixed-order public keys, non-canonical key enco
 x = 0 / sign-bit key, undecodable keys, and SHA-512 padding-boundary message lengths.
 """
 import hashlib
+import json
+import os
 import random
 
 import pyref
@@ -82,6 +84,16 @@ def secp_cases(curve, seed=7):
         crafted(f"crafted_inf_m{m % 1000}", m, (-7 * m) % n, 7)
     # R = G exactly, u2*R = -(u1)G at the last step
     crafted("crafted_inf_G", 1, n - 9, 9)
+    if curve == 0:
+        # the reference's Strauss-Shamir vector (x G + y B, tests/golden/secp256k1_strauss.json)
+        # as a recovery: R = B (B.x < n), u1 = x, u2 = y
+        v = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                        "secp256k1_strauss.json")))
+        bx, by = int(v["bx_decimal"]), int(v["by_decimal"])
+        u1, u2 = int(v["x"], 16) % n, int(v["y"], 16) % n
+        assert bx < n and (bx**3 + 7 - by * by) % p == 0
+        out.append(("ref_strauss_vector", _enc_sig(curve, bx, u2 * bx % n, by & 1),
+                    ((-u1 * bx) % n).to_bytes(32, "big")))
     return out
 
 

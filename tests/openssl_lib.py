@@ -36,6 +36,8 @@ def crypto():
             ("EC_POINT_oct2point", ctypes.c_int, [vp, vp, ctypes.c_char_p, ctypes.c_size_t, vp]),
             ("EC_POINT_point2oct", ctypes.c_size_t, [vp, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, vp]),
             ("EC_POINT_is_at_infinity", ctypes.c_int, [vp, vp]),
+            ("EC_GROUP_get_order", ctypes.c_int, [vp, vp, vp]),
+            ("BN_new", vp, []),
             ("BN_bin2bn", vp, [ctypes.c_char_p, ctypes.c_int, vp]),
             ("BN_bn2binpad", ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
             ("BN_free", None, [vp]),
@@ -188,3 +190,84 @@ def ed25519_verify(sig: bytes, pub: bytes, msg: bytes) -> bool:
     C.EVP_MD_CTX_free(ctx)
     C.EVP_PKEY_free(pk)
     return rc == 1
+
+
+def group_order(curve: int) -> int:
+    """the order n of the curve's group, as OpenSSL holds it"""
+    C = crypto()
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    bn = C.BN_new()
+    assert C.EC_GROUP_get_order(grp, bn, None) == 1
+    buf = ctypes.create_string_buffer(32)
+    assert C.BN_bn2binpad(bn, buf, 32) == 32
+    C.BN_free(bn)
+    C.EC_KEY_free(key)
+    return int.from_bytes(buf.raw, "big")
+
+
+def ecdsa_recover(curve: int, sig: bytes, digest: bytes):
+    """Public-key recovery of a fuel-format signature (r || s, recovery parity in bit 7 of byte 32)
+    with OpenSSL doing the curve arithmetic: R is lifted from r by OpenSSL's point decompression
+    (prefix 02 / 03 by the parity; it fails when r is not an x-coordinate or r >= p) and
+    Q = u1 G + u2 R by EC_POINT_mul, u1 = -z / r, u2 = s / r mod n. The acceptance rules around
+    them are libsecp256k1's secp256k1_ecdsa_recover (r, s in [1, n-1], z = digest mod n, R.x = r,
+    Q not at infinity), which fuel-crypto 0.49 calls for both curves' recover paths. Returns
+    (status, X||Y) with the oracle's status codes (1 bad r, 2 bad s, 3 no point, 4 infinity) and
+    64 zero bytes on failure."""
+    C = crypto()
+    n = group_order(curve)
+    r = int.from_bytes(sig[:32], "big")
+    odd = sig[32] >> 7
+    s = int.from_bytes(bytes([sig[32] & 0x7F]) + sig[33:], "big")
+    if not 0 < r < n:
+        return 1, bytes(64)
+    if not 0 < s < n:
+        return 2, bytes(64)
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    R = C.EC_POINT_new(grp)
+    Q = C.EC_POINT_new(grp)
+    try:
+        if C.EC_POINT_oct2point(grp, R, bytes([2 + odd]) + r.to_bytes(32, "big"), 33, None) != 1:
+            return 3, bytes(64)
+        z = int.from_bytes(digest, "big") % n
+        rinv = pow(r, -1, n)
+        u1, u2 = (-z * rinv) % n, s * rinv % n
+        b1, b2 = _bn(u1), _bn(u2)
+        assert C.EC_POINT_mul(grp, Q, b1, R, b2, None) == 1
+        C.BN_free(b1)
+        C.BN_free(b2)
+        if C.EC_POINT_is_at_infinity(grp, Q):
+            return 4, bytes(64)
+        buf = ctypes.create_string_buffer(65)
+        assert C.EC_POINT_point2oct(grp, Q, 4, buf, 65, None) == 65
+        return 0, buf.raw[1:]
+    finally:
+        C.EC_POINT_free(R)
+        C.EC_POINT_free(Q)
+        C.EC_KEY_free(key)
+
+
+def point_mul2(curve: int, P: bytes, u1: int, u2: int):
+    """u1 G + u2 P for P = X||Y with OpenSSL's EC_POINT_mul; None for infinity"""
+    C = crypto()
+    key = C.EC_KEY_new_by_curve_name(NID[curve])
+    grp = C.EC_KEY_get0_group(key)
+    pt = C.EC_POINT_new(grp)
+    res = C.EC_POINT_new(grp)
+    try:
+        assert C.EC_POINT_oct2point(grp, pt, b"\x04" + P, 65, None) == 1
+        b1, b2 = _bn(u1), _bn(u2)
+        assert C.EC_POINT_mul(grp, res, b1, pt, b2, None) == 1
+        C.BN_free(b1)
+        C.BN_free(b2)
+        if C.EC_POINT_is_at_infinity(grp, res):
+            return None
+        buf = ctypes.create_string_buffer(65)
+        assert C.EC_POINT_point2oct(grp, res, 4, buf, 65, None) == 65
+        return buf.raw[1:]
+    finally:
+        C.EC_POINT_free(pt)
+        C.EC_POINT_free(res)
+        C.EC_KEY_free(key)
