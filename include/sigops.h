@@ -152,7 +152,8 @@ int sigops_secp256k1_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void*
                                 size_t n, void* d_status, void* stream);
 int sigops_secp256r1_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_digests, const void* d_pubkeys,
                                 size_t n, void* d_status, void* stream);
-/* d_msgs must stay readable for 16 bytes past its last message byte */
+/* d_msgs must be non-NULL and 4-byte aligned (SIGOPS_E_ARG otherwise) and stay readable for 16
+   bytes past its last message byte (also when every message is empty) */
 int sigops_ed25519_verify_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
                               const uint64_t* d_msg_off, size_t n, void* d_status, void* stream);
 int sigops_ed25519_verify_strict_dev(sigops_ctx* ctx, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
@@ -168,7 +169,9 @@ int sigops_ed25519_verify_strict_dev(sigops_ctx* ctx, const void* d_sigs, const 
    call with SIGOPS_E_ARG. ---- */
 int sigops_batch(sigops_ctx* ctx, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
                  const uint8_t* msgs, const uint64_t* msg_off, size_t n, uint8_t* out, uint8_t* status);
-/* device pointers (d_msgs readable 16 bytes past the last message), enqueued on `stream` */
+/* device pointers (d_msgs non-NULL, 4-byte aligned and readable 16 bytes past the last message), enqueued
+   on `stream`; the context's bucketing and lane scratch are shared, so calls on one context must
+   not overlap on the device */
 int sigops_batch_dev(sigops_ctx* ctx, const void* d_schemes, const void* d_sigs, const void* d_aux,
                      const void* d_msgs, const uint64_t* d_msg_off, size_t n, void* d_out, void* d_status,
                      void* stream);
@@ -204,6 +207,15 @@ int sigops_secp256r1_recover_address(sigops_ctx* ctx, const uint8_t* sigs, const
 #define SIGOPS_STAGES 4
 int sigops_set_stage_timing(sigops_ctx* ctx, int on);
 int sigops_batch_stage_ms(sigops_ctx* ctx, float* ms);
+
+/* ---- fault injection, for tests of the error path. The reference's only failure is the
+   `success` canary a shader did not write (ShaderFailureError, src/lib.rs:12-14,
+   src/secp256k1_ecdsa.rs:203-205); with SIGOPS_FAULT_SKIP_COMPUTE set, every recovery,
+   verification and batch call on this context transfers its inputs and outputs but launches no
+   compute kernel, so the host entry points find their status canaries unwritten and return
+   SIGOPS_E_CANARY. flags = 0 turns it off. Test use only. ---- */
+#define SIGOPS_FAULT_SKIP_COMPUTE 1u
+int sigops_set_fault(sigops_ctx* ctx, uint32_t flags);
 
 /* ---- serving: continuous batching of verification requests. No counterpart in the reference.
    Requests (each a small mixed batch in the sigops_batch layout) are queued on submit and packed,

@@ -13,7 +13,8 @@
  *   r in [1, n-1], s in [1, n-1] (recid bit cleared), z = digest mod n,
  *   R.x = r (fuel never sets the x-reduced recid bit), R must exist, Q != infinity.
  * The group law is textbook Jacobian arithmetic with every exceptional case branched out, so it is
- * complete; it is pinned in tests/test_oracle_secp.py by affine Python arithmetic and OpenSSL.
+ * complete; it is pinned in tests/test_oracle.py and tests/test_oracle_pins.py by affine Python
+ * arithmetic and OpenSSL.
  */
 #include <string.h>
 #include "oracle_internal.h"

@@ -271,3 +271,50 @@ def point_mul2(curve: int, P: bytes, u1: int, u2: int):
         C.EC_POINT_free(pt)
         C.EC_POINT_free(res)
         C.EC_KEY_free(key)
+
+
+# ---- CPU baseline timing (oracle/libossl_baseline.so, C over libcrypto, threaded) ----
+_base = None
+
+
+def baseline_lib():
+    """oracle/libossl_baseline.so (built by oracle/Makefile); None when it is not built"""
+    global _base
+    if _base is None:
+        import os
+        so = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "libossl_baseline.so")
+        if not os.path.exists(so):
+            return None
+        L = ctypes.CDLL(so)
+        vp = ctypes.c_void_p
+        L.ossl_ecdsa_verify_time.restype = ctypes.c_double
+        L.ossl_ecdsa_verify_time.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+        L.ossl_ed25519_verify_time.restype = ctypes.c_double
+        L.ossl_ed25519_verify_time.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+        _base = L
+    return _base
+
+
+def time_ecdsa_verify(curve, sigs, digs, pubs, nthreads):
+    """(seconds, valid flags) for OpenSSL ECDSA verification of fuel-format signatures (numpy
+    uint8 arrays n x 64, n x 32, n x 64 X||Y) on nthreads threads"""
+    import numpy as np
+    sigs, digs, pubs = (np.ascontiguousarray(a, dtype=np.uint8) for a in (sigs, digs, pubs))
+    n = sigs.shape[0]
+    valid = np.zeros(n, np.uint8)
+    t = baseline_lib().ossl_ecdsa_verify_time(curve, sigs.ctypes.data, digs.ctypes.data, pubs.ctypes.data, n,
+                                              nthreads, valid.ctypes.data)
+    return t, valid
+
+
+def time_ed25519_verify(sigs, pubs, msgs, off, nthreads):
+    """(seconds, valid flags) for OpenSSL Ed25519 verification on nthreads threads"""
+    import numpy as np
+    sigs, pubs, msgs = (np.ascontiguousarray(a, dtype=np.uint8) for a in (sigs, pubs, msgs))
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    n = sigs.shape[0]
+    valid = np.zeros(n, np.uint8)
+    mp = msgs.ctypes.data if msgs.size else np.zeros(1, np.uint8).ctypes.data
+    t = baseline_lib().ossl_ed25519_verify_time(sigs.ctypes.data, pubs.ctypes.data, mp, off.ctypes.data, n, nthreads,
+                                                valid.ctypes.data)
+    return t, valid

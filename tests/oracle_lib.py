@@ -11,7 +11,9 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(REPO, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# SIGOPS_ORACLE_SO names another build of the same sources (the sanitizer test runs the tests on
+# an AddressSanitizer / UBSan build through it)
+ORACLE_SO = os.environ.get("SIGOPS_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 K1, R1 = 0, 1
 ST_OK, ST_BAD_R, ST_BAD_S, ST_NO_POINT, ST_INFINITY = 0, 1, 2, 3, 4

@@ -185,3 +185,21 @@ def test_secp256k1_strauss_vector_as_recovery():
     want = S.point_mul2(0, B, x % n, y % n)
     assert O.secp_recover(0, sig, dig) == (0, want)
     assert S.ecdsa_recover(0, sig, dig) == (0, want)
+
+
+def test_openssl_cpu_baseline_harness():
+    """the threaded OpenSSL timing harness (bench.py's tuned-library CPU baseline) verifies every
+    uncorrupted item and rejects every corrupted one, for any thread count"""
+    if S.baseline_lib() is None:
+        pytest.skip("oracle/libossl_baseline.so is not built")
+    for curve in (0, 1):
+        sigs, dig, pubs, cor = O.gen_secp(curve, 31, 300, corrupt_ppm=200000)
+        for threads in (1, 3):
+            t, valid = S.time_ecdsa_verify(curve, sigs, dig, pubs, threads)
+            assert t > 0 and np.array_equal(valid, (cor == 0).astype(np.uint8))
+    lens = np.arange(300) % 200
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    sigs, pks, msgs, cor = O.gen_ed25519(32, 300, off, corrupt_ppm=200000)
+    for threads in (1, 4):
+        t, valid = S.time_ed25519_verify(sigs, pks, msgs, off, threads)
+        assert t > 0 and np.array_equal(valid, (cor == 0).astype(np.uint8))

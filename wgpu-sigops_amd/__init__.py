@@ -97,6 +97,7 @@ def lib():
             for name in ("sigops_secp256k1_recover_address", "sigops_secp256r1_recover_address"):
                 getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
             L.sigops_set_stage_timing.argtypes = [vp, ctypes.c_int]
+            L.sigops_set_fault.argtypes = [vp, ctypes.c_uint32]
             L.sigops_batch_stage_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
             L.sigops_gen_secp_dev.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp,
                                               vp]
@@ -114,6 +115,21 @@ def lib():
             L.sigops_server_replay.argtypes = [vp, c_p, c_p, c_p, c_p, c_p, c_p, c_p, sz, c_p, c_p, c_p]
             _lib = L
         return _lib
+
+
+def source_digest():
+    """SHA-256 over the library's sources (csrc/ and the Makefile): ties a committed counter profile
+    to the build it measured"""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(PKG_DIR, "csrc")
+    for name in sorted(os.listdir(csrc)) + ["../Makefile"]:
+        path = os.path.join(csrc, name)
+        if name.endswith((".hip", ".hpp", ".cpp", ".h", ".py", "Makefile")) and os.path.isfile(path):
+            h.update(name.encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
 
 
 def probe_valu(device=0):
@@ -146,10 +162,22 @@ def _tuned(device):
         return None
 
 
+FAULT_SKIP_COMPUTE = 1
+DEFAULT_MAX_BATCH = 1 << 22
+
+
 class Context:
-    """One GPU context: resident generator tables, per-lane scratch and reusable I/O buffers."""
+    """One GPU context: resident generator tables, per-lane scratch and reusable I/O buffers.
+
+    The native context serves one caller at a time (its staging buffers, stream and scratch are
+    shared), so every native call on a Context holds the Context's lock: threads that share one
+    Context (the mirrors share the default context) are serialized, as the reference's per-call
+    devices are independent. The host-array methods split inputs larger than max_batch into
+    consecutive calls of at most max_batch items."""
 
     def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True, tuned=True):
+        self._lock = threading.RLock()
+        self.max_batch = max_batch or DEFAULT_MAX_BATCH
         if tuned and not blocks_per_cu and not tile_rounds:
             # the launch shape the autotuner measured fastest on this device (autotune.py), if any
             t = _tuned(device)
@@ -182,13 +210,27 @@ class Context:
     def handle(self):
         return self._h
 
-    def _check(self, rc):
+    def _call(self, fn, *args):
+        """fn(handle, *args) under the context's lock; raises on a failure status"""
+        with self._lock:
+            rc = fn(self._h, *args)
+            msg = lib().sigops_last_error(self._h).decode(errors="replace") if rc != OK else ""
         if rc == OK:
             return
-        msg = lib().sigops_last_error(self._h).decode(errors="replace")
         if rc == E_CANARY:
             raise ShaderFailureError(msg)
         raise SigopsError(f"libsigops error {rc}: {msg}")
+
+    def _chunks(self, n):
+        """[a, b) ranges of at most max_batch items covering n"""
+        step = max(1, self.max_batch)
+        return [(a, min(n, a + step)) for a in range(0, n, step)] or [(0, 0)]
+
+    def set_fault(self, flags):
+        """test switch (sigops_set_fault): FAULT_SKIP_COMPUTE makes every recovery, verification
+        and batch call skip its compute kernels, so the host entry points report the unwritten
+        status canaries as ShaderFailureError; 0 turns it off"""
+        self._call(lib().sigops_set_fault, int(flags))
 
     def _secp(self, fn, sigs, digests):
         sigs = _u8(sigs, 64, "sigs")
@@ -198,7 +240,8 @@ class Context:
             raise ValueError("sigs and digests differ in length")
         out = np.zeros((n, 64), np.uint8)
         st = np.zeros(n, np.uint8)
-        self._check(fn(self._h, _ptr(sigs), _ptr(digests), n, _ptr(out), _ptr(st)))
+        for a, b in self._chunks(n):
+            self._call(fn, _ptr(sigs[a:b]), _ptr(digests[a:b]), b - a, _ptr(out[a:b]), _ptr(st[a:b]))
         return out, st
 
     def secp256k1_recover(self, sigs, digests):
@@ -230,10 +273,10 @@ class Context:
         tab = np.ascontiguousarray(table_limbs, dtype=np.uint32).reshape(-1)
         valid = np.zeros(n, np.uint8)
         st = np.zeros(n, np.uint8)
-        self._check(lib().sigops_ed25519_verify_table(self._h, _ptr(sigs), _ptr(pubkeys),
-                                                      _ptr(msgs) if msgs.size else None, _ptr(msg_off), n,
-                                                      tab.ctypes.data, tab.size, int(log_limb_size), _ptr(valid),
-                                                      _ptr(st)))
+        for a, b in self._chunks(n):
+            self._call(lib().sigops_ed25519_verify_table, _ptr(sigs[a:b]), _ptr(pubkeys[a:b]),
+                       _ptr(msgs) if msgs.size else None, _ptr(msg_off[a:b + 1]), b - a, tab.ctypes.data, tab.size,
+                       int(log_limb_size), _ptr(valid[a:b]), _ptr(st[a:b]))
         return valid.astype(bool), st
 
     def _secp_verify(self, fn, sigs, digests, pubkeys):
@@ -245,7 +288,9 @@ class Context:
             raise ValueError("sigs, digests and pubkeys differ in length")
         valid = np.zeros(n, np.uint8)
         st = np.zeros(n, np.uint8)
-        self._check(fn(self._h, _ptr(sigs), _ptr(digests), _ptr(pubkeys), n, _ptr(valid), _ptr(st)))
+        for a, b in self._chunks(n):
+            self._call(fn, _ptr(sigs[a:b]), _ptr(digests[a:b]), _ptr(pubkeys[a:b]), b - a, _ptr(valid[a:b]),
+                       _ptr(st[a:b]))
         return valid.astype(bool), st
 
     def secp256k1_verify(self, sigs, digests, pubkeys):
@@ -257,12 +302,10 @@ class Context:
         return self._secp_verify(lib().sigops_secp256r1_verify, sigs, digests, pubkeys)
 
     def secp256k1_verify_dev(self, d_sigs, d_digests, d_pubkeys, n, d_status, stream=0):
-        self._check(lib().sigops_secp256k1_verify_dev(self._h, d_sigs, d_digests, d_pubkeys, n, d_status,
-                                                      stream or None))
+        self._call(lib().sigops_secp256k1_verify_dev, d_sigs, d_digests, d_pubkeys, n, d_status, stream or None)
 
     def secp256r1_verify_dev(self, d_sigs, d_digests, d_pubkeys, n, d_status, stream=0):
-        self._check(lib().sigops_secp256r1_verify_dev(self._h, d_sigs, d_digests, d_pubkeys, n, d_status,
-                                                      stream or None))
+        self._call(lib().sigops_secp256r1_verify_dev, d_sigs, d_digests, d_pubkeys, n, d_status, stream or None)
 
     def ed25519_verify(self, sigs, pubkeys, msgs, msg_off, strict=False):
         """(n,64) R||s, (n,32) keys, concatenated messages + (n+1) offsets -> ((n,) bool, (n,) status);
@@ -279,16 +322,17 @@ class Context:
         valid = np.zeros(n, np.uint8)
         st = np.zeros(n, np.uint8)
         fn = lib().sigops_ed25519_verify_strict if strict else lib().sigops_ed25519_verify
-        self._check(fn(self._h, _ptr(sigs), _ptr(pubkeys), _ptr(msgs) if msgs.size else None, _ptr(msg_off), n,
-                       _ptr(valid), _ptr(st)))
+        for a, b in self._chunks(n):
+            self._call(fn, _ptr(sigs[a:b]), _ptr(pubkeys[a:b]), _ptr(msgs) if msgs.size else None,
+                       _ptr(msg_off[a:b + 1]), b - a, _ptr(valid[a:b]), _ptr(st[a:b]))
         return valid.astype(bool), st
 
     def ed25519_verify_strict(self, sigs, pubkeys, msgs, msg_off):
         return self.ed25519_verify(sigs, pubkeys, msgs, msg_off, strict=True)
 
     def ed25519_verify_strict_dev(self, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream=0):
-        self._check(lib().sigops_ed25519_verify_strict_dev(self._h, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status,
-                                                           stream or None))
+        self._call(lib().sigops_ed25519_verify_strict_dev, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status,
+                   stream or None)
 
     def batch(self, schemes, sigs, aux, msgs, msg_off):
         """mixed batch: (n,) scheme bytes, (n,64) sigs, (n,32) digest-or-key, messages + (n+1) offsets
@@ -303,13 +347,15 @@ class Context:
         msgs = np.ascontiguousarray(msgs, dtype=np.uint8).reshape(-1)
         out = np.zeros((n, 64), np.uint8)
         st = np.zeros(n, np.uint8)
-        self._check(lib().sigops_batch(self._h, _ptr(schemes), _ptr(sigs), _ptr(aux), _ptr(msgs) if msgs.size else None,
-                                       _ptr(msg_off), n, _ptr(out), _ptr(st)))
+        for a, b in self._chunks(n):
+            self._call(lib().sigops_batch, _ptr(schemes[a:b]), _ptr(sigs[a:b]), _ptr(aux[a:b]),
+                       _ptr(msgs) if msgs.size else None, _ptr(msg_off[a:b + 1]), b - a, _ptr(out[a:b]),
+                       _ptr(st[a:b]))
         return out, st
 
     def batch_dev(self, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status, stream=0):
-        self._check(lib().sigops_batch_dev(self._h, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status,
-                                           stream or None))
+        self._call(lib().sigops_batch_dev, d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, n, d_out, d_status,
+                   stream or None)
 
     # ---- Fuel hashing (SHA-256) either side of the path ----
     def message_new(self, data, off=None):
@@ -323,14 +369,16 @@ class Context:
         off = np.ascontiguousarray(off, dtype=np.uint64)
         n = len(off) - 1
         out = np.zeros((n, 32), np.uint8)
-        self._check(lib().sigops_message_new(self._h, _ptr(data) if data.size else None, _ptr(off), n, _ptr(out)))
+        for a, b in self._chunks(n):
+            self._call(lib().sigops_message_new, _ptr(data) if data.size else None, _ptr(off[a:b + 1]), b - a,
+                       _ptr(out[a:b]))
         return out
 
     def message_new_dev(self, d_data, d_off, n, d_digests, stream=0):
-        self._check(lib().sigops_message_new_dev(self._h, d_data, d_off, n, d_digests, stream or None))
+        self._call(lib().sigops_message_new_dev, d_data, d_off, n, d_digests, stream or None)
 
     def address_from_pubkeys_dev(self, d_pubs, d_status, n, d_addr, stream=0):
-        self._check(lib().sigops_address_from_pubkeys_dev(self._h, d_pubs, d_status, n, d_addr, stream or None))
+        self._call(lib().sigops_address_from_pubkeys_dev, d_pubs, d_status, n, d_addr, stream or None)
 
     def recover_address(self, curve, sigs, digests):
         """recovery then Fuel address SHA-256(X || Y) on the device: (n, 32) addresses (zeros for
@@ -343,37 +391,37 @@ class Context:
         out = np.zeros((n, 32), np.uint8)
         st = np.zeros(n, np.uint8)
         fn = lib().sigops_secp256k1_recover_address if curve == 0 else lib().sigops_secp256r1_recover_address
-        self._check(fn(self._h, _ptr(sigs), _ptr(digests), n, _ptr(out), _ptr(st)))
+        for a, b in self._chunks(n):
+            self._call(fn, _ptr(sigs[a:b]), _ptr(digests[a:b]), b - a, _ptr(out[a:b]), _ptr(st[a:b]))
         return out, st
 
     STAGES = ("bucketing", "ed25519_hash", "verify_kernel", "careful")
 
     def set_stage_timing(self, on):
         """Eager batch launches with HIP events between the stages (see batch_stage_ms)."""
-        self._check(lib().sigops_set_stage_timing(self._h, 1 if on else 0))
+        self._call(lib().sigops_set_stage_timing, 1 if on else 0)
 
     def batch_stage_ms(self):
         """Stage durations (ms) of the last batch launched with stage timing on, keyed by STAGES."""
         ms = (ctypes.c_float * len(self.STAGES))()
-        self._check(lib().sigops_batch_stage_ms(self._h, ms))
+        self._call(lib().sigops_batch_stage_ms, ms)
         return dict(zip(self.STAGES, (float(x) for x in ms)))
 
     def gen_secp_dev(self, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs, d_corrupted=None, stream=0):
-        self._check(lib().sigops_gen_secp_dev(self._h, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs,
-                                              d_corrupted, stream or None))
+        self._call(lib().sigops_gen_secp_dev, curve, seed, n, corrupt_ppm, d_sigs, d_digests, d_pubs, d_corrupted,
+                   stream or None)
 
     def gen_ed25519_dev(self, seed, n, corrupt_ppm, d_msg_off, d_msgs, d_sigs, d_pubkeys, d_corrupted=None, stream=0):
-        self._check(lib().sigops_gen_ed25519_dev(self._h, seed, n, corrupt_ppm, d_msg_off, d_msgs, d_sigs, d_pubkeys,
-                                                 d_corrupted, stream or None))
+        self._call(lib().sigops_gen_ed25519_dev, seed, n, corrupt_ppm, d_msg_off, d_msgs, d_sigs, d_pubkeys,
+                   d_corrupted, stream or None)
 
     # device-resident variants (torch tensors or raw ints as device pointers)
     def secp_recover_dev(self, curve, d_sigs, d_digests, n, d_out, d_status, stream=0):
         fn = lib().sigops_secp256k1_recover_dev if curve == 0 else lib().sigops_secp256r1_recover_dev
-        self._check(fn(self._h, d_sigs, d_digests, n, d_out, d_status, stream or None))
+        self._call(fn, d_sigs, d_digests, n, d_out, d_status, stream or None)
 
     def ed25519_verify_dev(self, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream=0):
-        self._check(lib().sigops_ed25519_verify_dev(self._h, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status,
-                                                    stream or None))
+        self._call(lib().sigops_ed25519_verify_dev, d_sigs, d_pubkeys, d_msgs, d_msg_off, n, d_status, stream or None)
 
 
 class Server:
@@ -456,13 +504,16 @@ class Server:
 
 
 _default_ctx = None
+_default_lock = threading.Lock()
 
 
 def default_context():
+    """the Context the reference-API mirrors use when none is given (created once, thread-safe)"""
     global _default_ctx
-    if _default_ctx is None:
-        _default_ctx = Context()
-    return _default_ctx
+    with _default_lock:
+        if _default_ctx is None:
+            _default_ctx = Context()
+        return _default_ctx
 
 
 from . import ed25519_eddsa, fuel, precompute, secp256k1_ecdsa, secp256r1_ecdsa  # noqa: E402,F401

@@ -91,8 +91,9 @@ DEV void divsteps30(i32& eta, u32 f, u32 g, i32& u, i32& v, i32& q, i32& r) {
     g = (ng + (f & m2)) >> 1;
     qq = nq + (uu & (i32)m2);
     rr = nr + (vv & (i32)m2);
-    uu <<= 1;
-    vv <<= 1;
+    // doubled as unsigned words: a left shift of a negative signed value is undefined in C++17
+    uu = (i32)((u32)uu << 1);
+    vv = (i32)((u32)vv << 1);
     eta += 2;
   }
   u = uu;

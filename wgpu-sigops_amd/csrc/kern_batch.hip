@@ -10,8 +10,8 @@
 //      one bucket) from an atomic counter, so the scheme is uniform across the wavefront and load
 //      balances dynamically; it runs the same per-item code as the plain kernels;
 //   3. k_batch_careful finishes secp items whose fast ladder met an exceptional addition.
-// The n of the batch is read from device memory, so the captured graph of these launches replays
-// for any n up to the context capacity.
+// The batch's buffers and n are read from device memory (BatchIO), so the captured graph of these
+// launches replays for any buffers and any n up to the context capacity.
 #include "kernels_impl.hpp"
 
 namespace sigops {
@@ -26,7 +26,8 @@ struct BatchArgs {
   const u64* msg_off;
   uint4* out;
   u8* status;
-  const u32* n_ptr;
+  u32 n;
+  const BatchIO* io;  // the call's buffers and n in device memory; bind_io copies them into the fields above
   u32* counts;      // [NB]
   u32* cursor;      // [NB]
   u32* tile_start;  // [NB + 1] in processing order
@@ -36,8 +37,22 @@ struct BatchArgs {
   u32* pend_count;
   const u8* rank;   // bucket -> processing rank
   u32 tile_items;   // 64 * tile_rounds
-  const uint4* kbuf;  // Ed25519 challenges, written by the hash pass
 };
+
+// Every kernel starts with bind_io: the per-call buffers come from device memory (written on the
+// stream before the launches), the rest of BatchArgs is fixed per context and passed by value, so a
+// captured graph of these launches is the same for every call.
+DEV void bind_io(BatchArgs& a) {
+  const BatchIO& v = *a.io;
+  a.scheme = (const u8*)v.scheme;
+  a.sigs = (const uint4*)v.sigs;
+  a.aux = (const uint4*)v.aux;
+  a.msgs = (const u32*)v.msgs;
+  a.msg_off = v.msg_off;
+  a.out = (uint4*)v.out;
+  a.status = (u8*)v.status;
+  a.n = v.n;
+}
 
 // scheme == NULL: every item is an Ed25519 verification (the plain Ed25519 path hashes through
 // the same bucketing, so that its wavefronts hash equally long messages)
@@ -54,10 +69,11 @@ DEV u32 item_bucket(const BatchArgs& a, u32 i) {
 }
 
 __global__ __launch_bounds__(256) void k_bucket_hist(BatchArgs a) {
+  bind_io(a);
   __shared__ u32 h[NBUCKETS];
   for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x) h[t] = 0;
   __syncthreads();
-  const u32 n = *a.n_ptr;
+  const u32 n = a.n;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (item_scheme(a, i) > 2) continue;
     atomicAdd(&h[item_bucket(a, i)], 1u);
@@ -69,6 +85,7 @@ __global__ __launch_bounds__(256) void k_bucket_hist(BatchArgs a) {
 
 // one block: tile offsets in processing order, cursors, and DUMMY padding of each bucket's tail
 __global__ __launch_bounds__(64) void k_bucket_scan(BatchArgs a) {
+  bind_io(a);
   __shared__ u32 start[NBUCKETS];
   if (threadIdx.x == 0) {
     u32 pos_tiles = 0;
@@ -98,10 +115,11 @@ __global__ __launch_bounds__(64) void k_bucket_scan(BatchArgs a) {
 // atomic per bucket and chunk instead of one per item, on a handful of counters shared by every
 // workgroup.
 __global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
+  bind_io(a);
   constexpr u32 PER = 4;
   __shared__ u32 cnt[NBUCKETS];
   __shared__ u32 base[NBUCKETS];
-  const u32 n = *a.n_ptr;
+  const u32 n = a.n;
   for (u32 c0 = blockIdx.x * blockDim.x * PER; c0 < n; c0 += gridDim.x * blockDim.x * PER) {
     for (u32 t = threadIdx.x; t < NBUCKETS; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
@@ -128,23 +146,6 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(BatchArgs a) {
     for (u32 q = 0; q < PER; q++)
       if (bq[q] < NBUCKETS) a.order[base[bq[q]] + rq[q]] = c0 + q * blockDim.x + threadIdx.x;
     __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void k_batch_hash(BatchArgs a) {
-  const MemDev mem{a.msgs};
-  const u32 total = a.tile_start[NBUCKETS] * a.tile_items;
-  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
-    const u32 i = a.order[j];
-    if (i == DUMMY || item_scheme(a, i) != 2) continue;
-    u32 sr[16], pr[8], k[8];
-    load_words(a.sigs + 4 * (u64)i, sr, 4);
-    load_words(a.aux + 2 * (u64)i, pr, 2);
-    const u64 off = a.msg_off[i];
-    ed25519_challenge(sr, pr, mem, off, (u32)(a.msg_off[i + 1] - off), k);
-    uint4* kb = (uint4*)a.kbuf;
-    kb[2 * (u64)i] = make_uint4(k[0], k[1], k[2], k[3]);
-    kb[2 * (u64)i + 1] = make_uint4(k[4], k[5], k[6], k[7]);
   }
 }
 
@@ -179,15 +180,26 @@ DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) 
   a.status[i] = (u8)st;
 }
 
+// The schemes a persistent launch serves (template mask): the mixed batch serves all three; the
+// plain Ed25519 entry points serve only Ed25519 (dalek `verify`, or `verify_strict` rules), so
+// they run the same tile loop, with the challenge hashed inside it, and stage only B's table.
+constexpr u32 RUN_K1 = 1, RUN_R1 = 2, RUN_ED = 4, RUN_ED_STRICT = 8;
+constexpr u32 RUN_MIXED = RUN_K1 | RUN_R1 | RUN_ED;
+
+template <u32 MASK>
 __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
                                                    const uint4* __restrict__ gr1, const uint4* __restrict__ gb,
                                                    uint4* __restrict__ lane_tab, u32 slots) {
-  __shared__ uint4 s_k1[GTAB_ENTRIES * gtab_chunks<CurveK1>()];
-  __shared__ uint4 s_r1[GTAB_ENTRIES * gtab_chunks<CurveR1>()];
-  __shared__ uint4 s_b[BTAB_ENTRIES * 7];
-  stage_gtab<CurveK1>(s_k1, gk1);
-  stage_gtab<CurveR1>(s_r1, gr1);
-  for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
+  static_assert(!((MASK & RUN_ED) && (MASK & RUN_ED_STRICT)), "one Ed25519 rule set per launch");
+  constexpr bool K1 = MASK & RUN_K1, R1 = MASK & RUN_R1, ED = MASK & (RUN_ED | RUN_ED_STRICT);
+  bind_io(a);
+  __shared__ uint4 s_k1[K1 ? GTAB_ENTRIES * gtab_chunks<CurveK1>() : 1];
+  __shared__ uint4 s_r1[R1 ? GTAB_ENTRIES * gtab_chunks<CurveR1>() : 1];
+  __shared__ uint4 s_b[ED ? BTAB_ENTRIES * 7 : 1];
+  if constexpr (K1) stage_gtab<CurveK1>(s_k1, gk1);
+  if constexpr (R1) stage_gtab<CurveR1>(s_r1, gr1);
+  if constexpr (ED)
+    for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
   __syncthreads();
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 lane = threadIdx.x & 63u;
@@ -214,22 +226,24 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArg
     for (u32 k = 0; k < a.tile_items; k += 64) {
       const u32 i = a.order[(u64)t * a.tile_items + k + lane];
       if (i == DUMMY) continue;
-      if (b == BUCKET_K1) {
-        batch_secp_item<CurveK1>(a, i, jt, GTabOf<CurveK1>{s_k1});
-      } else if (b == BUCKET_R1) {
-        batch_secp_item<CurveR1>(a, i, jt, GTabOf<CurveR1>{s_r1});
-      } else {
+      if (K1 && b == BUCKET_K1) {
+        if constexpr (K1) batch_secp_item<CurveK1>(a, i, jt, GTabOf<CurveK1>{s_k1});
+      } else if (R1 && b == BUCKET_R1) {
+        if constexpr (R1) batch_secp_item<CurveR1>(a, i, jt, GTabOf<CurveR1>{s_r1});
+      } else if constexpr (ED) {
         // the challenge is hashed here, inside the persistent kernel: the tile's items have equal
         // SHA-512 block counts (bucketed), and the hash's memory latency hides behind the other
         // waves' field arithmetic instead of idling the GPU in a pass of its own
-        u32 sr[16], pr[8], kk[8];
+        u32 sr[16], pr[8];
         load_words(a.sigs + 4 * (u64)i, sr, 4);
         load_words(a.aux + 2 * (u64)i, pr, 2);
         const u64 off = a.msg_off[i];
-        ed25519_challenge(sr, pr, MemDev{a.msgs}, off, (u32)(a.msg_off[i + 1] - off), kk);
-        a.status[i] = (u8)ed25519_verify_core(sr, pr, kk, ct, BTabLds{s_b});
-        const u32 zero[16] = {0};
-        store16(a.out + 4 * (u64)i, zero);
+        a.status[i] = (u8)ed25519_verify_item<(MASK & RUN_ED_STRICT) != 0>(
+            sr, pr, MemDev{a.msgs}, off, (u32)(a.msg_off[i + 1] - off), ct, BTabLds{s_b});
+        if (a.out) {
+          const u32 zero[16] = {0};
+          store16(a.out + 4 * (u64)i, zero);
+        }
       }
     }
   }
@@ -237,6 +251,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArg
 
 __global__ __launch_bounds__(64) void k_batch_careful(BatchArgs a, const uint4* __restrict__ gk1,
                                                       const uint4* __restrict__ gr1) {
+  bind_io(a);
   const u32 cnt = *a.pend_count;
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
     const u32 i = a.pend_list[j];
@@ -263,20 +278,20 @@ __global__ __launch_bounds__(64) void k_batch_careful(BatchArgs a, const uint4* 
   }
 }
 
+static __global__ void k_set_io(BatchIO* d, BatchIO v) { *d = v; }
+
+hipError_t launch_set_batch_io(BatchIO* d_io, const BatchIO& v, hipStream_t s) {
+  k_set_io<<<1, 1, 0, s>>>(d_io, v);
+  return hipGetLastError();
+}
+
 static __global__ void k_zero_u32(u32* p, u32 n) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 static BatchArgs batch_args(const BatchLaunch& L) {
-  BatchArgs a;
-  a.scheme = (const u8*)L.scheme;
-  a.sigs = (const uint4*)L.sigs;
-  a.aux = (const uint4*)L.aux;
-  a.msgs = (const u32*)L.msgs;
-  a.msg_off = L.msg_off;
-  a.out = (uint4*)L.out;
-  a.status = (u8*)L.status;
-  a.n_ptr = L.n_ptr;
+  BatchArgs a{};
+  a.io = L.io;
   a.counts = L.work;                    // NBUCKETS
   a.cursor = L.work + NBUCKETS;         // NBUCKETS
   a.tile_start = L.work + 2 * NBUCKETS;  // NBUCKETS + 1
@@ -286,19 +301,23 @@ static BatchArgs batch_args(const BatchLaunch& L) {
   a.order = L.order;
   a.rank = L.rank;
   a.tile_items = 64u * L.tile_rounds;
-  a.kbuf = (const uint4*)L.kbuf;
   return a;
 }
 
-// Ed25519 challenges of a plain (all-Ed25519, L.scheme == NULL) batch in bucket order: the
-// bucketing and hash passes of launch_batch without the persistent kernel
-hipError_t launch_ed_hash_bucketed(const BatchLaunch& L, hipStream_t s) {
+// plain Ed25519 verification (io->scheme == NULL, every item Ed25519): the bucketing by SHA-512
+// block count, then the persistent kernel serving only Ed25519 tiles, each of which hashes its
+// items' challenges and verifies them (dalek `verify`, or `verify_strict` rules)
+hipError_t launch_ed_verify(const BatchLaunch& L, bool strict, hipStream_t s) {
   const BatchArgs a = batch_args(L);
   k_zero_u32<<<1, 256, 0, s>>>(L.work, BATCH_WORK_WORDS);
   k_bucket_hist<<<L.aux_grid, 256, 0, s>>>(a);
   k_bucket_scan<<<1, 64, 0, s>>>(a);
   k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
-  k_batch_hash<<<L.grid, 256, 0, s>>>(a);
+  if (strict)
+    k_batch_run<RUN_ED_STRICT><<<L.grid, 256, 0, s>>>(a, nullptr, nullptr, (const uint4*)L.gb, (uint4*)L.lane_tab,
+                                                      L.slots);
+  else
+    k_batch_run<RUN_ED><<<L.grid, 256, 0, s>>>(a, nullptr, nullptr, (const uint4*)L.gb, (uint4*)L.lane_tab, L.slots);
   return hipGetLastError();
 }
 
@@ -314,9 +333,10 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   k_bucket_scan<<<1, 64, 0, s>>>(a);
   k_bucket_scatter<<<L.aux_grid, 256, 0, s>>>(a);
   if ((e = mark(1)) != hipSuccess) return e;
-  // (the Ed25519 challenges are hashed inside k_batch_run; the stage stays empty)
+  // (the Ed25519 challenges are hashed inside k_batch_run; the stage stays empty, so that the
+  // stage indices of earlier measurements keep their meaning)
   if ((e = mark(2)) != hipSuccess) return e;
-  k_batch_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+  k_batch_run<RUN_MIXED><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                      (uint4*)L.lane_tab, L.slots);
   if ((e = mark(3)) != hipSuccess) return e;
   k_batch_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);

@@ -27,12 +27,6 @@ hipError_t launch_secp_verify_CurveR1(const void* sigs, const void* digs, const 
                                      const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
                                      uint32_t* pend_count, uint32_t grid, hipStream_t s);
 
-// Ed25519 verification given the challenges in kbuf (launch_ed_hash_bucketed writes them);
-// strict: ed25519-dalek verify_strict instead of verify
-hipError_t launch_ed25519_verify(const void* sigs, const void* pks, void* status, uint32_t n, const void* btab,
-                                 void* ctab, uint32_t slots, void* kbuf, uint32_t grid, hipStream_t s,
-                                 bool strict = false);
-
 // ---- mixed batch ----
 constexpr uint32_t BUCKET_K1 = 0, BUCKET_R1 = 1, BUCKET_ED0 = 2;
 constexpr uint32_t ED_BINS = 40;  // Ed25519 buckets by SHA-512 block count 1..40 (40 = "40 or more")
@@ -40,15 +34,23 @@ constexpr uint32_t NBUCKETS = 2 + ED_BINS;
 constexpr uint32_t BATCH_WORK_WORDS = 3 * NBUCKETS + 3;  // counts, cursors, tile starts, tile ctr, pending ctr
 constexpr uint32_t MAX_TILE_ROUNDS = 4;
 
-struct BatchLaunch {
-  const void* scheme;
+// The buffers of one batch call and its size. The host writes them into device memory
+// (launch_set_batch_io, stream-ordered) before the launches, and the kernels read them from
+// there, so one captured graph per context serves every call whatever buffers it passes.
+struct BatchIO {
+  const void* scheme;  // NULL: every item is an Ed25519 verification
   const void* sigs;
   const void* aux;
   const void* msgs;
   const uint64_t* msg_off;
-  void* out;
+  void* out;  // NULL for the plain Ed25519 path
   void* status;
-  const uint32_t* n_ptr;
+  uint32_t n;
+};
+hipError_t launch_set_batch_io(BatchIO* d_io, const BatchIO& v, hipStream_t s);
+
+struct BatchLaunch {
+  const BatchIO* io;    // device copy of the call's buffers
   uint32_t* work;       // BATCH_WORK_WORDS
   uint32_t* order;      // capacity max_batch + NBUCKETS * 64 * tile_rounds
   uint32_t* pend_list;  // capacity max_batch
@@ -57,14 +59,15 @@ struct BatchLaunch {
   const void* gr1;
   const void* gb;
   void* lane_tab;
-  void* kbuf;  // n x 32 bytes: Ed25519 challenges
   uint32_t slots, grid, aux_grid, tile_rounds;
   hipEvent_t* stage_ev;  // null, or 5 events recorded before, between and after the 4 stages
 };
 
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
-// bucketing and hash passes only, for an all-Ed25519 batch (L.scheme == NULL): challenges into L.kbuf
-hipError_t launch_ed_hash_bucketed(const BatchLaunch& L, hipStream_t s);
+// plain Ed25519 verification of an all-Ed25519 batch (io->scheme == NULL, io->out == NULL): the
+// bucketing, then the persistent kernel on Ed25519 tiles with the challenges hashed inside it;
+// strict: ed25519-dalek verify_strict rules; L.gb is the fixed-base table
+hipError_t launch_ed_verify(const BatchLaunch& L, bool strict, hipStream_t s);
 
 // SHA-256: digests of messages (Fuel Message::new) and addresses of 64-byte public keys.
 // work: MSG_WORK_WORDS device words (cleared by the launch); order: n words, the longest-first

@@ -368,29 +368,6 @@ __global__ __launch_bounds__(64) void k_secp_verify_careful(const uint4* __restr
   }
 }
 
-template <bool STRICT>
-__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_ed25519_verify(
-    const uint4* __restrict__ sigs, const uint4* __restrict__ pks, const uint4* __restrict__ kbuf,
-    u8* __restrict__ status, u32 n, const uint4* __restrict__ btab, uint4* __restrict__ ctab, u32 slots) {
-  __shared__ uint4 sb[BTAB_ENTRIES * 7];
-  for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) sb[t] = btab[t];
-  __syncthreads();
-  const BTabLds bt{sb};
-  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
-  CTabDev tab;
-  tab.base = ctab;
-  tab.slots = slots;
-  tab.slot = slot;
-  const u32 stride = gridDim.x * blockDim.x;
-  for (u32 i = slot; i < n; i += stride) {
-    u32 sr[16], pr[8], k[8];
-    load_words(sigs + 4 * (u64)i, sr, 4);
-    load_words(pks + 2 * (u64)i, pr, 2);
-    load_words(kbuf + 2 * (u64)i, k, 2);
-    status[i] = (u8)ed25519_verify_core<STRICT>(sr, pr, k, tab, bt);
-  }
-}
-
 static __global__ void k_fill_u32(u32* p, u32 v, u32 n) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }

@@ -9,9 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
-#include <map>
 #include <memory>
-#include <tuple>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -51,18 +49,18 @@ struct sigops_ctx {
   uint8_t* d_rank = nullptr;
   uint32_t* d_work = nullptr;
   uint32_t* d_order = nullptr;
-  uint32_t* d_n = nullptr;
+  sigops::BatchIO* d_io = nullptr;  // the current batch call's buffers, read by its kernels
   uint32_t* d_hq = nullptr;      // work words of the message-hashing launch (MSG_WORK_WORDS)
   uint32_t* d_horder = nullptr;  // max_batch words: longest-first order of the messages
   // caller-supplied tables in the reference's layout: limbs in, converted table out, check flags
   uint32_t* d_tlimbs = nullptr;
   void* d_ctab = nullptr;
   uint32_t* d_tflags = nullptr;
-  std::map<std::tuple<const void*, const void*, const void*, const void*, const void*, void*, void*>,
-           hipGraphExec_t>
-      graphs;
+  // the captured mixed-batch launches; they read the call's buffers from d_io, so one graph serves
+  // every call on this context
+  hipGraphExec_t graph = nullptr;
   DevBuf in, aux, msg, off, out, st, sch;
-  void* d_kbuf = nullptr;  // max_batch x 32 bytes: Ed25519 challenges
+  uint32_t fault = 0;  // SIGOPS_FAULT_* test switch
   bool stage_timing = false;
   bool stage_recorded = false;
   hipEvent_t stage_ev[SIGOPS_STAGES + 1] = {};
@@ -152,13 +150,12 @@ __global__ void k_fill_bytes(uint8_t* p, uint8_t v, size_t n) {
     p[i] = v;
 }
 
-__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
-
 // gtab: the fixed-base table to use (NULL: the resident table of G)
 int secp_recover_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_digs, size_t n, void* d_out,
                      void* d_status, hipStream_t s, const void* gtab = nullptr) {
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
+  if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return SIGOPS_OK;
   if (curve == 0)
     HIPCHK(c, sigops::launch_secp_recover_CurveK1(d_sigs, d_digs, d_out, d_status, (uint32_t)n,
                                                  gtab ? gtab : c->d_gtab_k1, c->d_lane_tab, c->slots, c->d_pend + 1,
@@ -239,6 +236,7 @@ int secp_verify_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
   if (!d_sigs || !d_digs || !d_pubs || !d_status) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return SIGOPS_OK;
   if (curve == 0)
     HIPCHK(c, sigops::launch_secp_verify_CurveK1(d_sigs, d_digs, d_pubs, d_status, (uint32_t)n, c->d_gtab_k1,
                                                 c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
@@ -345,7 +343,7 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
                                                                       sigops::MAX_TILE_ROUNDS) * sizeof(uint32_t))) !=
       hipSuccess)
     return bad(e, "hipMalloc batch order");
-  if ((e = hipMalloc((void**)&c->d_n, sizeof(uint32_t))) != hipSuccess) return bad(e, "hipMalloc n");
+  if ((e = hipMalloc((void**)&c->d_io, sizeof(sigops::BatchIO))) != hipSuccess) return bad(e, "hipMalloc batch io");
   if ((e = hipMalloc((void**)&c->d_hq, sigops::MSG_WORK_WORDS * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc((void**)&c->d_horder, (size_t)c->max_batch * sizeof(uint32_t))) != hipSuccess)
     return bad(e, "hipMalloc message hashing");
@@ -353,7 +351,6 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
       (e = hipMalloc(&c->d_ctab, sizeof(sigops::ED_BTABLE))) != hipSuccess ||
       (e = hipMalloc((void**)&c->d_tflags, sizeof(uint32_t))) != hipSuccess)
     return bad(e, "hipMalloc caller tables");
-  if ((e = hipMalloc(&c->d_kbuf, (size_t)c->max_batch * 32)) != hipSuccess) return bad(e, "hipMalloc challenges");
   return c;
 }
 
@@ -361,13 +358,13 @@ void sigops_destroy(sigops_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (hipEvent_t e : c->stage_ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
                   c->msg.p, c->off.p, c->out.p, c->st.p, c->sch.p, (void*)c->d_rank, (void*)c->d_work,
-                  (void*)c->d_order, (void*)c->d_n, (void*)c->d_hq, (void*)c->d_horder, (void*)c->d_tlimbs, c->d_ctab,
-                  (void*)c->d_tflags, c->d_kbuf})
+                  (void*)c->d_order, (void*)c->d_io, (void*)c->d_hq, (void*)c->d_horder, (void*)c->d_tlimbs,
+                  c->d_ctab, (void*)c->d_tflags})
     if (p) (void)hipFree(p);
   c->pool.reset();
   sigops_host::pinned_free(c->h_up);
@@ -424,36 +421,36 @@ int sigops_secp256r1_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d
 }
 
 // Ed25519 verification, dalek `verify` rules or (strict) `verify_strict` rules; btab: the
-// fixed-base table (NULL: the resident table of B)
+// fixed-base table (NULL: the resident table of B). The items are bucketed by SHA-512 block count
+// and a persistent kernel pops tiles of one bucket, hashes their challenges and verifies them, so
+// a wavefront's lanes hash equally long messages.
 static int ed_verify_dev(sigops_ctx* c, const void* d_sigs, const void* d_pubkeys, const void* d_msgs,
                          const uint64_t* d_msg_off, size_t n, void* d_status, void* stream, bool strict,
                          const void* btab = nullptr) {
   if (!c) return SIGOPS_E_ARG;
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
+  // the SHA-512 reader loads the word at each message's offset even for an empty message
+  if (!d_sigs || !d_pubkeys || !d_msgs || !d_msg_off || !d_status) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (((uintptr_t)d_msgs & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_msgs must be 4-byte aligned");
   hipStream_t s = (hipStream_t)stream;
-  // challenges through the batch's bucketing (all items Ed25519): each wavefront hashes messages
-  // of one SHA-512 block count, instead of waiting for the longest message among its 64 lanes
+  HIPCHK(c, sigops::launch_set_batch_io(c->d_io, {nullptr, d_sigs, d_pubkeys, d_msgs, d_msg_off, nullptr, d_status,
+                                                  (uint32_t)n}, s));
   sigops::BatchLaunch L{};
-  L.scheme = nullptr;
-  L.sigs = d_sigs;
-  L.aux = d_pubkeys;
-  L.msgs = d_msgs;
-  L.msg_off = d_msg_off;
-  L.status = d_status;
-  L.n_ptr = c->d_n;
+  L.io = c->d_io;
   L.work = c->d_work;
   L.order = c->d_order;
+  L.pend_list = c->d_pend + 1;
   L.rank = c->d_rank;
-  L.kbuf = c->d_kbuf;
-  L.grid = c->grid;
+  L.gb = btab ? btab : c->d_btab;
+  L.lane_tab = c->d_lane_tab;
+  L.slots = c->slots;
+  // every bucket may end in a partial tile: at most NBUCKETS tiles beyond n / 64, 4 tiles per block
+  L.grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256 + (sigops::NBUCKETS + 3) / 4));
   L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
   L.tile_rounds = 1;
-  k_set_u32<<<1, 1, 0, s>>>(c->d_n, (uint32_t)n);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, sigops::launch_ed_hash_bucketed(L, s));
-  HIPCHK(c, sigops::launch_ed25519_verify(d_sigs, d_pubkeys, d_status, (uint32_t)n, btab ? btab : c->d_btab,
-                                          c->d_lane_tab, c->slots, c->d_kbuf, c->grid, s, strict));
+  if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return SIGOPS_OK;
+  HIPCHK(c, sigops::launch_ed_verify(L, strict, s));
   return SIGOPS_OK;
 }
 
@@ -579,17 +576,17 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
   if (!c) return SIGOPS_E_ARG;
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
-  if (!d_schemes || !d_sigs || !d_aux || !d_msg_off || !d_out || !d_status) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (!d_schemes || !d_sigs || !d_aux || !d_msgs || !d_msg_off || !d_out || !d_status)
+    return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (((uintptr_t)d_msgs & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_msgs must be 4-byte aligned");
   hipStream_t s = (hipStream_t)stream;
-  sigops::BatchLaunch L;
-  L.scheme = d_schemes;
-  L.sigs = d_sigs;
-  L.aux = d_aux;
-  L.msgs = d_msgs;
-  L.msg_off = d_msg_off;
-  L.out = d_out;
-  L.status = d_status;
-  L.n_ptr = c->d_n;
+  // the call's buffers and n go to device memory first (stream-ordered, by value at enqueue time),
+  // so one captured graph serves every call and back-to-back enqueues do not race
+  HIPCHK(c, sigops::launch_set_batch_io(c->d_io, {d_schemes, d_sigs, d_aux, d_msgs, d_msg_off, d_out, d_status,
+                                                  (uint32_t)n}, s));
+  if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return SIGOPS_OK;
+  sigops::BatchLaunch L{};
+  L.io = c->d_io;
   L.work = c->d_work;
   L.order = c->d_order;
   L.pend_list = c->d_pend + 1;
@@ -598,16 +595,11 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
   L.gr1 = c->d_gtab_r1;
   L.gb = c->d_btab;
   L.lane_tab = c->d_lane_tab;
-  L.kbuf = c->d_kbuf;
   L.slots = c->slots;
   L.grid = c->grid;
   L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
   L.tile_rounds = c->tile_rounds;
   L.stage_ev = nullptr;
-  // n goes to device memory first (stream-ordered, by value at enqueue time), so one captured graph
-  // serves every n and back-to-back enqueues do not race
-  k_set_u32<<<1, 1, 0, s>>>(c->d_n, (uint32_t)n);
-  HIPCHK(c, hipGetLastError());
   if (c->stage_timing) {
     L.stage_ev = c->stage_ev;
     HIPCHK(c, sigops::launch_batch(L, s));
@@ -618,23 +610,22 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
     HIPCHK(c, sigops::launch_batch(L, s));
     return SIGOPS_OK;
   }
-  L.aux_grid = c->grid;  // the graph is shape-independent of n
-  auto key = std::make_tuple(d_schemes, d_sigs, d_aux, d_msgs, (const void*)d_msg_off, d_out, d_status);
-  auto it = c->graphs.find(key);
-  if (it == c->graphs.end()) {
+  if (!c->graph) {
+    L.aux_grid = c->grid;  // the graph does not depend on n
     hipGraph_t g;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t le = sigops::launch_batch(L, c->stream);
     hipError_t ce = hipStreamEndCapture(c->stream, &g);
     if (le != hipSuccess) return hip_fail(c, le, "launch_batch under capture");
     if (ce != hipSuccess) return hip_fail(c, ce, "hipStreamEndCapture");
-    hipGraphExec_t ex;
-    hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) return hip_fail(c, ie, "hipGraphInstantiate");
-    it = c->graphs.emplace(key, ex).first;
+    if (ie != hipSuccess) {
+      c->graph = nullptr;
+      return hip_fail(c, ie, "hipGraphInstantiate");
+    }
   }
-  HIPCHK(c, hipGraphLaunch(it->second, s));
+  HIPCHK(c, hipGraphLaunch(c->graph, s));
   return SIGOPS_OK;
 }
 
@@ -691,6 +682,13 @@ int sigops_address_from_pubkeys_dev(sigops_ctx* c, const void* d_pubs, const voi
   if (n > 0xFFFFFFFFull) return fail(c, SIGOPS_E_ARG, "n too large");
   if (!d_pubs || !d_addr) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   HIPCHK(c, sigops::launch_fuel_address(d_pubs, d_status, (uint32_t)n, d_addr, c->grid, (hipStream_t)stream));
+  return SIGOPS_OK;
+}
+
+int sigops_set_fault(sigops_ctx* c, uint32_t flags) {
+  if (!c) return SIGOPS_E_ARG;
+  if (flags & ~(uint32_t)SIGOPS_FAULT_SKIP_COMPUTE) return fail(c, SIGOPS_E_ARG, "unknown fault flag");
+  c->fault = flags;
   return SIGOPS_OK;
 }
 
