@@ -188,6 +188,17 @@ int sigops_message_new(sigops_ctx* ctx, const uint8_t* data, const uint64_t* off
    hashing work words and order are shared: calls on one context must not overlap on the device. */
 int sigops_message_new_dev(sigops_ctx* ctx, const void* d_data, const uint64_t* d_off, size_t n, void* d_digests,
                            void* stream);
+/* Message::new with no allocation. No counterpart in the reference: it is for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc). Same work and digests as sigops_message_new_dev, but its kernels' scratch (the
+   counting sort's work words and the longest-first order) is a fixed array in the library's device
+   memory, sized for SIGOPS_NOALLOC_MAX_MESSAGES messages (4 MiB + 1 KiB), which every GPU the
+   library runs on holds until the process ends; the call allocates nothing. n above the maximum
+   fails with SIGOPS_E_ARG. Every call on one GPU, from any context, uses that GPU's one array, so
+   calls that overlap on one GPU race. */
+#define SIGOPS_NOALLOC_MAX_MESSAGES (1u << 20)
+int sigops_message_new_noalloc(sigops_ctx* ctx, const void* d_data, const uint64_t* d_off, size_t n,
+                               void* d_digests, void* stream);
 /* d_status may be NULL (every key hashed) */
 int sigops_address_from_pubkeys_dev(sigops_ctx* ctx, const void* d_pubs, const void* d_status, size_t n,
                                     void* d_addr, void* stream);

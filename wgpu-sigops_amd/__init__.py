@@ -93,6 +93,7 @@ def lib():
             L.sigops_batch_dev.argtypes = [vp, vp, vp, vp, vp, vp, sz, vp, vp, vp]
             L.sigops_message_new.argtypes = [vp, c_p, c_p, sz, c_p]
             L.sigops_message_new_dev.argtypes = [vp, vp, vp, sz, vp, vp]
+            L.sigops_message_new_noalloc.argtypes = [vp, vp, vp, sz, vp, vp]
             L.sigops_address_from_pubkeys_dev.argtypes = [vp, vp, vp, sz, vp, vp]
             for name in ("sigops_secp256k1_recover_address", "sigops_secp256r1_recover_address"):
                 getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
@@ -163,6 +164,7 @@ def _tuned(device):
 
 
 FAULT_SKIP_COMPUTE = 1
+NOALLOC_MAX_MESSAGES = 1 << 20
 DEFAULT_MAX_BATCH = 1 << 22
 
 
@@ -376,6 +378,11 @@ class Context:
 
     def message_new_dev(self, d_data, d_off, n, d_digests, stream=0):
         self._call(lib().sigops_message_new_dev, d_data, d_off, n, d_digests, stream or None)
+
+    def message_new_noalloc_dev(self, d_data, d_off, n, d_digests, stream=0):
+        """message_new_dev with the kernels' scratch in the library's fixed device array: the call
+        allocates nothing (at most NOALLOC_MAX_MESSAGES messages; calls on one GPU must not overlap)"""
+        self._call(lib().sigops_message_new_noalloc, d_data, d_off, n, d_digests, stream or None)
 
     def address_from_pubkeys_dev(self, d_pubs, d_status, n, d_addr, stream=0):
         self._call(lib().sigops_address_from_pubkeys_dev, d_pubs, d_status, n, d_addr, stream or None)

@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,7 +20,22 @@
 #include "host_xfer.hpp"
 #include "kernels.hpp"
 
+// ---- scratch of sigops_message_new_noalloc ----
+// The hashing kernels' work words (the first NOALLOC_WORK_BYTES) and the longest-first order of up
+// to SIGOPS_NOALLOC_MAX_MESSAGES messages, in a fixed array of the library's device memory instead
+// of context buffers; every GPU the library runs on holds its own copy until the process ends. On
+// the host the array's name is only a stand-in, never a device address: sigops_create looks up the
+// array's address on its device with hipGetSymbolAddress (once per device, that device current),
+// and the entry point passes that address to the kernels.
+constexpr size_t NOALLOC_WORK_BYTES = 1024;
+static_assert(sigops::MSG_WORK_WORDS * sizeof(uint32_t) <= NOALLOC_WORK_BYTES, "work words fit");
+static __device__ __align__(256) uint8_t scratch[NOALLOC_WORK_BYTES + (size_t)SIGOPS_NOALLOC_MAX_MESSAGES * 4];
+
 namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[NOALLOC_MAX_DEVICES];  // device address of `scratch` per device, 0 until looked up
 
 constexpr uint8_t CANARY = 0xEE;
 // largest caller table: 16 entries x 3 coordinates x 24 limbs (l = 11)
@@ -61,6 +77,7 @@ struct sigops_ctx {
   hipGraphExec_t graph = nullptr;
   DevBuf in, aux, msg, off, out, st, sch;
   uint32_t fault = 0;  // SIGOPS_FAULT_* test switch
+  void* noalloc_scratch = nullptr;  // device address of `scratch` on this context's device
   bool stage_timing = false;
   bool stage_recorded = false;
   hipEvent_t stage_ev[SIGOPS_STAGES + 1] = {};
@@ -294,6 +311,15 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
   };
   hipError_t e;
   if ((e = hipSetDevice(c->device)) != hipSuccess) return bad(e, "hipSetDevice");
+  if (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[c->device]) {
+      void* p = nullptr;
+      if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return bad(e, "hipGetSymbolAddress");
+      g_scratch[c->device] = p;
+    }
+    c->noalloc_scratch = g_scratch[c->device];
+  }
   int cus = 0;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)) != hipSuccess)
     return bad(e, "hipDeviceGetAttribute");
@@ -649,6 +675,25 @@ int sigops_message_new_dev(sigops_ctx* c, const void* d_data, const uint64_t* d_
   if (!d_data || !d_off || !d_digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
   if (((uintptr_t)d_data & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_data must be 4-byte aligned");
   HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, c->d_hq, c->d_horder, 2 * c->grid,
+                                       (hipStream_t)stream));
+  return SIGOPS_OK;
+}
+
+int sigops_message_new_noalloc(sigops_ctx* c, const void* d_data, const uint64_t* d_off, size_t n, void* d_digests,
+                               void* stream) {
+  if (!c) return SIGOPS_E_ARG;
+  if (n == 0) return SIGOPS_OK;
+  if (n > SIGOPS_NOALLOC_MAX_MESSAGES)
+    return fail(c, SIGOPS_E_ARG, "n exceeds SIGOPS_NOALLOC_MAX_MESSAGES (" + std::to_string(SIGOPS_NOALLOC_MAX_MESSAGES) +
+                                     ")");
+  if (!d_data || !d_off || !d_digests) return fail(c, SIGOPS_E_ARG, "NULL buffer");
+  if (((uintptr_t)d_data & 3u) != 0) return fail(c, SIGOPS_E_ARG, "d_data must be 4-byte aligned");
+  if (!c->noalloc_scratch) return fail(c, SIGOPS_E_DEVICE, "no fixed scratch on this device");
+  uint32_t* work = (uint32_t*)c->noalloc_scratch;
+  uint32_t* order = (uint32_t*)((uint8_t*)c->noalloc_scratch + NOALLOC_WORK_BYTES);
+  // the launch clears the work words with a kernel and writes the order before reading it, so
+  // what an earlier call left in the array does not matter
+  HIPCHK(c, sigops::launch_message_new(d_data, d_off, (uint32_t)n, d_digests, work, order, 2 * c->grid,
                                        (hipStream_t)stream));
   return SIGOPS_OK;
 }
