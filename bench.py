@@ -7,17 +7,24 @@ log-uniform on 0..4096 bytes, schemes randomly interleaved, 5 % corrupted -- gen
 by the engine's seeded signer and resident in HBM when the timed region starts.
 
 One step = one batched, graph-captured pass over the whole batch (device-side bucketing + one
-persistent kernel + careful pass). `value` is signatures per second over the K timed steps.
-Also reported, each on the same mixed workload:
-  * plain: the three per-scheme kernels (gather by scheme, one launch per scheme, scatter back),
-    and the batched/plain ratio;
-  * transfers: the batched path through the host API (H2D + graph + D2H), the reference's
-    benchmark method (src/benchmarks/secp256k1_ecdsa.rs:126-138);
-  * roofline: algorithmic u32 multiply-accumulates per signature (the canonical textbook schedule:
-    fixed-window Fermat inversions, 252 doublings, dense 2 x 8^2 MACs per 256-bit product) divided
-    by the launch time, against the measured v_mad_u64_u32 issue rate;
-  * cpu_baseline: the CPU oracle timed on a bounded sample of the same items on this host's cores,
-    also used to check the GPU outputs of that sample bit for bit.
+persistent kernel + careful pass). `value` is signatures per second over the K timed steps, inputs
+resident in HBM. Also reported, each checked for exactness:
+  * incl_transfers: the batch through the host API (host arrays in, host arrays out), the
+    reference's benchmark method (src/benchmarks/secp256k1_ecdsa.rs:126-138);
+  * plain: the three per-scheme kernels on the same items (device-resident, and through the host
+    API with transfers: three calls against the batch's one), and the batched/plain ratios;
+  * configs: the plain per-scheme configurations -- secp256k1 recovery at N = 1024, secp256r1
+    recovery at N = 65536, Ed25519 verification at N = 65536 (messages 0..1024 bytes, 10 %
+    corrupted) -- through the host API with transfers and device-resident, each with its CPU
+    baselines on the same items;
+  * roofline: the persistent kernel's VALU use per launch, from the committed counter profile of
+    this build at this launch shape (busy SIMD cycles, instructions), over its duration measured
+    here with HIP events, against MI355X's VALU issue peak; the canonical textbook work per
+    signature over the same peak is `vs_textbook`;
+  * cpu_baseline: the CPU oracle (a plain C restatement) on a bounded sample of the same items on
+    this host's cores, with OpenSSL libcrypto (a tuned library; ECDSA verification stands in for
+    recovery, which it does not offer) timed on the same sample beside it;
+  * served: Poisson arrivals of 1..4096-signature requests at three offered loads.
 Prints ONE JSON line.
 """
 import argparse
@@ -31,10 +38,17 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
+# the CPU baseline leg times the oracle and OpenSSL through the tests' bindings
+sys.path.insert(1, os.path.join(REPO, "tests"))
 
-# canonical multiply-accumulates per signature (u32 x u32 -> u64) of the textbook schedule above
+# canonical multiply-accumulates per signature (u32 x u32 -> u64) of a textbook schedule: fixed-window
+# Fermat inversions, 252 doublings, dense 2 x 8^2 MACs per 256-bit product
 MAC_PER_SIG = {"secp256k1": 595072, "secp256r1": 627456, "ed25519_per_block": 24 * 128, "ed25519_base": 3850 * 128}
-LIMBS_MAC = 128  # MACs per canonical 256-bit modular multiplication
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, each issuing one wave64 VALU instruction every 4 cycles
+# (16 lanes per cycle) at the 2.4 GHz peak engine clock: 39.3 T lane-operation slots per second
+CUS, SIMDS_PER_CU, CLOCK_HZ = 256, 4, 2.4e9
+VALU_PEAK = CUS * SIMDS_PER_CU * 16 * CLOCK_HZ
+COUNTER_PROFILE = os.path.join(REPO, "profiles", "counters_k_batch_run_mixed_2p20.md")
 
 
 def log(*a):
@@ -146,24 +160,197 @@ def mixed_mac_per_sig(schemes, lens):
     return float(total) / len(schemes)
 
 
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hbm_traffic_and_valu_k_batch_run_mixed_2p20.md")
-
-
-def read_profile():
-    """From the committed summary of the counter passes over this workload: HBM bytes per launch of
-    the persistent kernel (FETCH_SIZE / WRITE_SIZE, each scaled by a copy of known size) and the
-    fraction of its active cycles that issue VALU instructions (SQ_INSTS_VALU x 4 / GRBM_GUI_ACTIVE)."""
-    out = {"traffic": None, "valu_issue_frac": None, "source": None}
+def read_counter_profile(digest, shape):
+    """Counters of one launch of the persistent kernel from the committed counter profile
+    (rocprofv3 --pmc passes of this benchmark): used only when the profile was taken on this build
+    (same source digest) at this launch shape; otherwise the status says why not."""
+    out = {"source": os.path.relpath(COUNTER_PROFILE, REPO), "status": None, "values": {}}
     try:
-        for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                out["traffic"] = int(line.split(":")[1])
-            if line.startswith("valu_issue_frac:"):
-                out["valu_issue_frac"] = float(line.split(":")[1])
-        out["source"] = os.path.relpath(TRAFFIC_PROFILE, REPO)
+        kv = {}
+        for line in open(COUNTER_PROFILE):
+            if ":" in line and not line.startswith(("#", "|", " ", "-", "`")):
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
     except OSError:
-        pass
+        out["status"] = "missing"
+        return out
+    if kv.get("sources_sha256") != digest:
+        out["status"] = "stale: the library sources changed after the profile was taken"
+    elif kv.get("launch_shape") != shape:
+        out["status"] = f"taken at another launch shape ({kv.get('launch_shape')})"
+    else:
+        out["status"] = "matches this build and launch shape"
+        for k in ("valu_insts_per_launch", "valu_busy_quads_per_launch", "valu_int64_per_launch",
+                  "valu_int32_per_launch", "traffic_bytes_per_launch", "kernel_ms_profiled", "valu_busy"):
+            if k in kv:
+                out["values"][k] = float(kv[k])
     return out
+
+
+def roofline(n, mac_sig, kern_ms, stage_ms, prof):
+    """VALU roofline of the persistent kernel: busy SIMD quad-cycles per launch (each one wave64
+    VALU instruction, 64 lane-operations) over the launch's duration, against the issue peak.
+    frac is at most 1 by construction: a SIMD is busy at most every cycle."""
+    t = kern_ms / 1e3
+    v = prof["values"]
+    r = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK / 1e12, 3),
+         "unit": "T VALU lane-op slots/s (busy SIMD quad-cycles x 64 lanes)", "frac": None, "traffic": None,
+         "peak_basis": "256 CU x 4 SIMD x one wave64 VALU instruction per 4 cycles at 2.4 GHz",
+         "kernel": "k_batch_run", "kernel_ms": round(kern_ms, 3), "stage_ms": {k: round(x, 3) for k, x in stage_ms.items()},
+         "vs_textbook": round(n * mac_sig / t / VALU_PEAK, 4), "textbook_mac_per_sig": round(mac_sig),
+         "vs_textbook_note": "canonical textbook MACs (fixed-window Fermat, 252 doublings, dense products) per "
+                             "second over the peak; this kernel executes far fewer (GLV, sparse reduction, chains, "
+                             "divsteps), so this ratio is not a utilisation",
+         "counter_profile": {"source": prof["source"], "status": prof["status"]}}
+    if v:
+        busy = v.get("valu_busy_quads_per_launch")
+        if busy:
+            r["achieved"] = round(busy * 64 / t / 1e12, 3)
+            r["frac"] = round(busy * 64 / t / VALU_PEAK, 4)
+        if v.get("valu_insts_per_launch"):
+            r["valu_insts_per_launch"] = v["valu_insts_per_launch"]
+            r["issued_T_lane_ops_per_s"] = round(v["valu_insts_per_launch"] * 64 / t / 1e12, 3)
+        if v.get("valu_int64_per_launch"):
+            r["int64_insts_per_launch"] = v["valu_int64_per_launch"]
+            r["int64_frac_of_peak"] = round(v["valu_int64_per_launch"] * 64 / t / VALU_PEAK, 4)
+        if v.get("traffic_bytes_per_launch"):
+            r["traffic"] = int(v["traffic_bytes_per_launch"])
+            r["traffic_unit"] = "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+            r["hbm_GBps"] = round(r["traffic"] / t / 1e9, 1)
+            r["hbm_frac"] = round(r["traffic"] / t / 8e12, 4)
+        if v.get("kernel_ms_profiled"):
+            r["counter_profile"]["kernel_ms_profiled"] = v["kernel_ms_profiled"]
+        if v.get("valu_busy"):
+            r["counter_profile"]["valu_busy_of_profiled_run"] = v["valu_busy"]
+    return r
+
+
+def cpu_time(fn, *args, **kw):
+    t0 = time.perf_counter()
+    res = fn(*args, **kw)
+    return time.perf_counter() - t0, res
+
+
+def cpu_secp_sample(curve, sigs, digs, pubs, threads):
+    """oracle recovery (threads, and one core) and OpenSSL verification (threads) of a sample"""
+    import oracle_lib as O
+    import openssl_lib as S
+    t, (o, st) = cpu_time(O.recover_batch, curve, sigs, digs, nthreads=threads)
+    m1 = min(len(sigs), 1024)
+    t1, _ = cpu_time(O.recover_batch, curve, sigs[:m1], digs[:m1], nthreads=1)
+    res = {"oracle": {"sigs_per_s": round(len(sigs) / t, 1), "threads": threads,
+                      "serial_1_core": round(m1 / t1, 1)}, "outputs": (o, st)}
+    if S.available() and S.baseline_lib() is not None:
+        to, valid = S.time_ecdsa_verify(curve, sigs, digs, pubs, threads)
+        t1o, _ = S.time_ecdsa_verify(curve, sigs[:m1], digs[:m1], pubs[:m1], 1)
+        res["openssl"] = {"sigs_per_s": round(len(sigs) / to, 1), "threads": threads,
+                          "serial_1_core": round(m1 / t1o, 1), "op": "ECDSA_do_verify", "valid": valid}
+    return res
+
+
+def cpu_ed_sample(sigs, pks, msgs, off, threads):
+    import oracle_lib as O
+    import openssl_lib as S
+    t, v = cpu_time(O.verify_batch, sigs, pks, msgs, off, nthreads=threads)
+    m1 = min(len(sigs), 1024)
+    t1, _ = cpu_time(O.verify_batch, sigs[:m1], pks[:m1], msgs, off[:m1 + 1], nthreads=1)
+    res = {"oracle": {"sigs_per_s": round(len(sigs) / t, 1), "threads": threads,
+                      "serial_1_core": round(m1 / t1, 1)}, "outputs": v}
+    if S.available() and S.baseline_lib() is not None:
+        to, valid = S.time_ed25519_verify(sigs, pks, msgs, off, threads)
+        t1o, _ = S.time_ed25519_verify(sigs[:m1], pks[:m1], msgs, off[:m1 + 1], 1)
+        res["openssl"] = {"sigs_per_s": round(len(sigs) / to, 1), "threads": threads,
+                          "serial_1_core": round(m1 / t1o, 1), "op": "EVP_DigestVerify (Ed25519)", "valid": valid}
+    return res
+
+
+def plain_configs(ctx, W, dev, stream, steps, threads, with_cpu, cpu_sample):
+    """The plain per-scheme configurations through the host API (host arrays in and out, transfers
+    included, the reference's method) and device-resident, each checked on every item and, on a
+    bounded sample, against the oracle, with the CPU baselines of that sample."""
+    import torch
+    res = []
+    specs = (("config 1: secp256k1 recovery, N=1024, all valid", "secp", 0, 1024),
+             ("config 2: secp256r1 recovery, N=65536, all valid", "secp", 1, 65536),
+             ("config 3: ed25519 verification, N=65536, messages 0..1024 B, 10% corrupted", "ed", 2, 65536))
+    for name, kind, curve, n in specs:
+        if kind == "secp":
+            it = W.secp(ctx, curve, n, seed=41 + curve, corrupt_ppm=0)
+            torch.cuda.synchronize()
+            h_sigs, h_dig, h_pub = it.sigs.cpu().numpy(), it.aux.cpu().numpy(), it.pubs.cpu().numpy()
+            host_fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
+            out = torch.zeros((n, 64), dtype=torch.uint8, device=dev)
+            st = torch.zeros(n, dtype=torch.uint8, device=dev)
+
+            def host_call():
+                return host_fn(h_sigs, h_dig)
+
+            def dev_call():
+                ctx.secp_recover_dev(curve, it.sigs.data_ptr(), it.aux.data_ptr(), n, out.data_ptr(), st.data_ptr(),
+                                     stream.cuda_stream)
+
+            def exact(r):
+                return bool(np.array_equal(r[0], h_pub) and not r[1].any())
+        else:
+            lens = W.uniform_lengths(n, 1024, 43)
+            it = W.ed25519(ctx, n, lens, seed=44, corrupt_ppm=100000)
+            torch.cuda.synchronize()
+            h_sigs, h_pks, h_off = it.sigs.cpu().numpy(), it.aux.cpu().numpy(), it.msg_off_host.astype(np.uint64)
+            h_msgs = it.msgs.cpu().numpy()[: int(h_off[-1])]
+            cor = it.corrupted.cpu().numpy()
+            st = torch.zeros(n, dtype=torch.uint8, device=dev)
+
+            def host_call():
+                return ctx.ed25519_verify(h_sigs, h_pks, h_msgs, h_off)
+
+            def dev_call():
+                ctx.ed25519_verify_dev(it.sigs.data_ptr(), it.aux.data_ptr(), it.msgs.data_ptr(),
+                                       it.msg_off.data_ptr(), n, st.data_ptr(), stream.cuda_stream)
+
+            def exact(r):
+                return bool(np.array_equal(r[0], cor == 0))
+        first = host_call()
+        reps = max(3, steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = host_call()
+        incl_ms = (time.perf_counter() - t0) / reps * 1e3
+        dev_call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            dev_call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        dev_ms = e0.elapsed_time(e1) / reps
+        entry = {"config": name, "n": n,
+                 "incl_transfers": {"ms_per_call": round(incl_ms, 3), "sigs_per_s": round(n / incl_ms * 1e3, 1)},
+                 "device_resident": {"ms_per_call": round(dev_ms, 3), "sigs_per_s": round(n / dev_ms * 1e3, 1)},
+                 "all_items_as_generated": exact(first) and exact(r)}
+        log(f"{name}: {incl_ms:.3f} ms incl. transfers ({n / incl_ms / 1e3:.2f} M sigs/s), {dev_ms:.3f} ms "
+            f"device-resident; every item as generated: {entry['all_items_as_generated']}")
+        if with_cpu:
+            m = min(n, cpu_sample)
+            if kind == "secp":
+                c = cpu_secp_sample(curve, h_sigs[:m], h_dig[:m], h_pub[:m], threads)
+                o, t = c.pop("outputs")
+                entry["sample_bit_exact_vs_oracle"] = bool(np.array_equal(o, r[0][:m]) and np.array_equal(t, r[1][:m]))
+            else:
+                off_m = h_off[: m + 1]
+                c = cpu_ed_sample(h_sigs[:m], h_pks[:m], h_msgs, off_m, threads)
+                v = c.pop("outputs")
+                entry["sample_bit_exact_vs_oracle"] = bool(np.array_equal(v.astype(bool), r[0][:m]))
+            if "openssl" in c:
+                valid = c["openssl"].pop("valid")
+                want = np.ones(m, bool) if kind == "secp" else (cor[:m] == 0)
+                c["openssl"]["agrees_with_gpu"] = bool(np.array_equal(valid.astype(bool), want))
+            entry["cpu_baseline"] = dict(c, sample=f"the first {m} items", unit="sigs/s")
+            log(f"  cpu: oracle {c['oracle']['sigs_per_s']:.0f} sigs/s on {threads} threads"
+                + (f", OpenSSL {c['openssl']['sigs_per_s']:.0f}" if "openssl" in c else ""))
+        res.append(entry)
+        del it
+    return res
 
 
 def main():
@@ -181,6 +368,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--tile-rounds", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the plain per-scheme configurations (secp256k1 1024, secp256r1 / Ed25519 65536)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (calibrates HBM byte counters)")
@@ -327,14 +516,58 @@ def main():
     h_sch, h_sigs, h_aux = mx.schemes_host, mx.sigs.cpu().numpy(), mx.aux.cpu().numpy()
     h_msgs = mx.msgs.cpu().numpy()[: int(mx.msg_off_host[-1])]
     h_off = mx.msg_off_host.astype(np.uint64)
-    ctx.batch(h_sch, h_sigs, h_aux, h_msgs, h_off)
-    t0 = time.perf_counter()
-    reps = max(1, min(3, args.steps))
-    for _ in range(reps):
-        h_out, h_st = ctx.batch(h_sch, h_sigs, h_aux, h_msgs, h_off)
-    incl_ms = (time.perf_counter() - t0) / reps * 1e3
-    incl_ok = bool(np.array_equal(h_out, out_h) and np.array_equal(h_st, st_h))
-    log(f"with transfers: {incl_ms:.2f} ms per batch, {n / incl_ms * 1e3 / 1e6:.2f} M sigs/s")
+    reps = max(3, min(10, args.steps))
+
+    def host_batch(streamed):
+        if hasattr(ctx, "set_streaming"):
+            ctx.set_streaming(streamed)
+        ctx.batch(h_sch, h_sigs, h_aux, h_msgs, h_off)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            o, t = ctx.batch(h_sch, h_sigs, h_aux, h_msgs, h_off)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        return ms, bool(np.array_equal(o, out_h) and np.array_equal(t, st_h))
+
+    # copy-then-launch (transfers, then the graph, then transfers back), then streamed (the
+    # default: the host packs chunks into pinned memory while the persistent kernel drains them)
+    copy_ms, copy_ok = host_batch(False)
+    incl_ms, incl_ok = host_batch(True)
+    log(f"with transfers: streamed {incl_ms:.2f} ms per batch ({n / incl_ms * 1e3 / 1e6:.2f} M sigs/s), "
+        f"copy-then-launch {copy_ms:.2f} ms; identical to the device path: {incl_ok}, {copy_ok}")
+
+    # the same items through the three plain host entry points (one call per scheme, inputs split
+    # by scheme beforehand): what a caller of the per-scheme API pays, transfers included
+    plain_incl = None
+    if not args.no_plain:
+        ii = [np.nonzero(h_sch == s)[0] for s in (0, 1, 2)]
+        ed_off = np.concatenate([h_off[ii[2]], h_off[ii[2][-1] + 1: ii[2][-1] + 2]]) if len(ii[2]) else \
+            np.zeros(1, np.uint64)
+        split = [(h_sigs[ii[0]], h_aux[ii[0]]), (h_sigs[ii[1]], h_aux[ii[1]]),
+                 (h_sigs[ii[2]], h_aux[ii[2]], h_msgs, ed_off)]
+
+        def plain_host():
+            a = ctx.secp256k1_recover(*split[0])
+            b = ctx.secp256r1_recover(*split[1])
+            c = ctx.ed25519_verify(*split[2])
+            return a, b, c
+
+        plain_host()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            pa, pb, pc = plain_host()
+        p_ms = (time.perf_counter() - t0) / reps * 1e3
+        p_ok = bool(np.array_equal(pa[0], out_h[ii[0]]) and np.array_equal(pa[1], st_h[ii[0]]) and
+                    np.array_equal(pb[0], out_h[ii[1]]) and np.array_equal(pb[1], st_h[ii[1]]) and
+                    np.array_equal(pc[1], st_h[ii[2]]))
+        plain_incl = {"ms_per_batch": round(p_ms, 3), "sigs_per_s": round(n / p_ms * 1e3, 1),
+                      "batched_over_plain": round(p_ms / incl_ms, 4), "identical_to_batch": p_ok,
+                      "calls": "secp256k1_recover, secp256r1_recover, ed25519_verify on the items of each scheme"}
+        log(f"plain with transfers: {p_ms:.2f} ms (3 host calls); batched/plain with transfers = {p_ms / incl_ms:.2f}x")
+
+    configs = None
+    if not args.no_configs:
+        configs = plain_configs(ctx, W, dev, stream, args.steps, min(16, os.cpu_count() or 1), not args.no_cpu,
+                                args.cpu_sample)
 
     # served: Poisson arrivals of mixed requests (1..4096 items, log-uniform) at three offered loads,
     # replayed by the library's native client against the continuous-batching server
@@ -387,24 +620,15 @@ def main():
                     f"{served[-1]['sustained_sigs_per_s'] / 1e6:.2f} M sigs/s sustained, identical: {same}")
             served_batches = srv.stats()
 
-    # roofline of the dominant kernel (the persistent verification kernel): algorithmic MACs of the
-    # launch over its own event-timed duration; traffic from the calibrated counter passes
+    # roofline of the dominant kernel (the persistent verification kernel): its VALU use per launch
+    # from the counter profile of this build and launch shape, over its duration measured here
     mac_sig = mixed_mac_per_sig(sch, mx.lens)
     kern_ms = stage_ms["verify_kernel"]
-    achieved = n * mac_sig / (kern_ms / 1e3) / 1e12
-    peak = mad_per_s / 1e12
-    prof = read_profile()
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
-                "unit": "TMAC/s (u32 x u32 -> u64 multiply-accumulate)", "frac": round(achieved / peak, 4),
-                "work": "canonical textbook schedule (fixed-window Fermat, 252 doublings, dense Montgomery); "
-                        "the kernel executes fewer MACs than that, so frac can exceed 1",
-                "valu_issue_frac": prof["valu_issue_frac"],
-                "traffic": prof["traffic"], "traffic_unit": "B per launch", "traffic_source": prof["source"],
-                # the same traffic over this run's kernel time, against MI355X's 8 TB/s HBM peak
-                "hbm_GBps": round(prof["traffic"] / (kern_ms / 1e3) / 1e9, 1) if prof["traffic"] else None,
-                "hbm_frac": round(prof["traffic"] / (kern_ms / 1e3) / 8e12, 4) if prof["traffic"] else None,
-                "kernel": "k_batch_run", "kernel_ms": round(kern_ms, 3), "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
-                "mac_per_sig": round(mac_sig), "peak_source": "sigops_probe_valu (v_mad_u64_u32)"}
+    shape = (f"blocks_per_cu={ctx.blocks_per_cu or 2} tile_rounds={ctx.tile_rounds or 1} n={n} seed={args.seed}")
+    prof = read_counter_profile(sigops.source_digest(), shape)
+    roof = roofline(n, mac_sig, kern_ms, stage_ms, prof)
+    log(f"roofline: frac {roof['frac']}, achieved {roof['achieved']} of {roof['peak']} T lane-op slots/s; "
+        f"counter profile: {prof['status']}; vs_textbook {roof['vs_textbook']}")
 
     # CPU baseline: the oracle on a bounded sample of the same items (rank 0, N=1)
     cpu = None
@@ -428,6 +652,27 @@ def main():
         v = O.verify_batch(h_sigs[ss], h_aux[ss], sub_msgs, sub_off, nthreads=threads)
         exact &= bool(np.array_equal(v.astype(bool), st_h[ss] == 0))
         cpu_s = time.perf_counter() - t0
+        # OpenSSL on the same sample: ECDSA verification under the generator's keys for the secp
+        # items, Ed25519 verification for the rest
+        import openssl_lib as S
+        ossl = None
+        if S.available() and S.baseline_lib() is not None:
+            h_pubs = np.zeros((n, 64), np.uint8)
+            for s_ in (0, 1):
+                h_pubs[mx.idx[s_]] = mx.parts[s_].pubs.cpu().numpy()
+            t_o, agree = 0.0, True
+            for s_ in (0, 1):
+                ss_ = sample[sch[sample] == s_]
+                t_, valid = S.time_ecdsa_verify(s_, h_sigs[ss_], h_aux[ss_], h_pubs[ss_], threads)
+                t_o += t_
+                agree &= bool(np.array_equal(valid.astype(bool), st_h[ss_] == 0))
+            t_, valid = S.time_ed25519_verify(h_sigs[ss], h_aux[ss], sub_msgs, sub_off, threads)
+            t_o += t_
+            agree &= bool(np.array_equal(valid.astype(bool), st_h[ss] == 0))
+            ossl = {"value": round(len(sample) / t_o, 1), "unit": "sigs/s", "cores": threads,
+                    "kind": "tuned library (OpenSSL libcrypto 3, C, threaded)",
+                    "op": "ECDSA_do_verify under the signer's key (secp items; OpenSSL has no recovery) and "
+                          "EVP_DigestVerify (Ed25519 items)", "agrees_with_gpu_status": agree}
         # serial, one core, on the first 2048 items of the sample (the reference's CPU method)
         s1 = sample[:2048]
         t1 = time.perf_counter()
@@ -444,7 +689,7 @@ def main():
                "sample": f"{len(sample)} items drawn uniformly from the same 2^20 mixed batch, C oracle "
                          f"(oracle/), {threads} threads; outputs also compared with the GPU's",
                "bit_exact_vs_gpu": exact, "serial_1_core": round(serial, 1),
-               "host_cpus": os.cpu_count()}
+               "host_cpus": os.cpu_count(), "openssl_same_sample": ossl}
         log(f"cpu oracle: {cpu['value']:.0f} sigs/s on {threads} threads, {serial:.0f} sigs/s on 1 core; "
             f"GPU sample bit-exact: {exact}")
 
@@ -467,11 +712,16 @@ def main():
                    "n": n, "seed": args.seed, "blocks_per_cu": ctx.blocks_per_cu or 2,
                    "tile_rounds": ctx.tile_rounds or 1,
                    "launch_shape_from": "autotune (tuned.json)" if tuned_shape else "arguments or library defaults"},
-        "roofline": roofline,
+        "roofline": roof,
         "cpu_baseline": cpu,
         "plain": plain,
         "incl_transfers": {"ms_per_batch": round(incl_ms, 3), "sigs_per_s": round(n / incl_ms * 1e3, 1),
-                           "identical_to_device_path": incl_ok},
+                           "identical_to_device_path": incl_ok, "method": "streamed host batch (sigops_batch)",
+                           "copy_then_launch": {"ms_per_batch": round(copy_ms, 3),
+                                                "sigs_per_s": round(n / copy_ms * 1e3, 1),
+                                                "identical_to_device_path": copy_ok}},
+        "plain_incl_transfers": plain_incl,
+        "configs": configs,
         "roundtrip_all_items_ok": roundtrip_ok,
         "served": served,
         "fuel_hashing": fuel,
