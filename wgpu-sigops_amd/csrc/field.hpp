@@ -24,6 +24,7 @@
 //     precondition, so one run of a formula verifies its bounds for all inputs.
 #pragma once
 #include <cstdint>
+#include <type_traits>
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
@@ -202,6 +203,34 @@ __constant__ u64 g_col_offsets[3] = {COL_E, COL_E - (COL_E >> SIGOPS_LIMB_BITS),
                                      (u64)0 - (COL_E >> SIGOPS_LIMB_BITS)};
 #endif
 
+// The sparse coefficients of the three field primes in one table (secp256k1 p's, then P-256 p's,
+// then 2^255-19's). On the device every reduction term reads its coefficient from the extern
+// constant copy, for the same reason as the column offsets: folded into an immediate, a
+// power-of-two coefficient becomes a 64-bit shift plus add (or a 2-instruction subtract) and
+// 2^24-1 a shift and subtract; read from memory, each term stays one multiply-add with a 32-bit
+// scalar operand, and the values cost no registers between products.
+template <class F>
+DEV constexpr int sp_base() {
+  if constexpr (std::is_same<F, ModPk1>::value) return 0;
+  if constexpr (std::is_same<F, ModPr1>::value) return ModPk1::SP_N;
+  if constexpr (std::is_same<F, ModPed>::value) return ModPk1::SP_N + ModPr1::SP_N;
+  return -1;
+}
+constexpr int SP_TABLE_N = ModPk1::SP_N + ModPr1::SP_N + ModPed::SP_N;
+struct SpTable {
+  i32 v[SP_TABLE_N];
+};
+constexpr SpTable sp_table() {
+  SpTable t{};
+  for (int k = 0; k < ModPk1::SP_N; k++) t.v[sp_base<ModPk1>() + k] = ModPk1::SP_COEF[k];
+  for (int k = 0; k < ModPr1::SP_N; k++) t.v[sp_base<ModPr1>() + k] = ModPr1::SP_COEF[k];
+  for (int k = 0; k < ModPed::SP_N; k++) t.v[sp_base<ModPed>() + k] = ModPed::SP_COEF[k];
+  return t;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__constant__ SpTable g_sp_coef = sp_table();
+#endif
+
 template <class F>
 DEV void mont_acc_init(u64 (&acc)[2 * NL - 1]) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -228,9 +257,8 @@ DEV void mont_step(u64 (&acc)[2 * NL - 1], int i) {
       // the carry shift discards anyway
       if (t == 0 && coef == -1) continue;
 #if defined(__HIP_DEVICE_COMPILE__)
-      // a power-of-two coefficient would otherwise become a 64-bit shift plus a 64-bit add or a
-      // 2-instruction subtract; an opaque scalar keeps every term one multiply-add
-      asm("" : "+s"(coef));
+      static_assert(sp_base<F>() >= 0, "a sparse modulus outside the coefficient table");
+      coef = g_sp_coef.v[sp_base<F>() + t];  // the same value, kept out of the compiler's folding
 #endif
       if (F::SP_COEF[t] < 0)
         acc[i + F::SP_OFF[t]] += (u64)((i64)coef * (i64)(i32)m);
