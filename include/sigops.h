@@ -66,6 +66,12 @@ typedef struct {
   uint32_t tile_rounds;     /* mixed batch: a wavefront takes 64 * tile_rounds items of one bucket per
                                queue pop (1..4, 0: default 1) */
   uint32_t no_graph;        /* mixed batch: 1 = plain stream launches instead of the captured graph */
+  uint32_t waves_per_simd;  /* mixed batch: the persistent kernel built for 2 waves per SIMD (256 registers
+                               per lane; 0: default 2) or 1 (512: no spills to scratch memory, half the
+                               waves); blocks_per_cu defaults to this */
+  uint32_t bucket_order;    /* mixed batch: processing order of the buckets, 0 (default) secp256r1,
+                               secp256k1, Ed25519 by decreasing SHA-512 block count; 1 secp256r1, Ed25519,
+                               secp256k1; 2 secp256k1, secp256r1, Ed25519 */
 } sigops_config;
 
 /* Creates a context on one GPU: uploads the fixed-base generator tables to HBM once (the
@@ -169,6 +175,21 @@ int sigops_ed25519_verify_strict_dev(sigops_ctx* ctx, const void* d_sigs, const 
    call with SIGOPS_E_ARG. ---- */
 int sigops_batch(sigops_ctx* ctx, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
                  const uint8_t* msgs, const uint64_t* msg_off, size_t n, uint8_t* out, uint8_t* status);
+/* How sigops_batch moves a host batch (no counterpart in the reference). Streamed (the default,
+   for n >= 4096): the host packs the batch into pinned, device-mapped memory chunk by chunk while
+   one persistent launch already processes the published chunks, reading them over PCIe, and the
+   results come back the same way as tiles finish, so packing, transfers and compute overlap.
+   Off: the whole batch is copied to the device, processed, and copied back. Both give the same
+   results; a streamed launch that cannot take every chunk (its waits are bounded) is redone the
+   other way. */
+int sigops_set_streaming(sigops_ctx* ctx, int on);
+/* phases of the last streamed batch on this context, in milliseconds from the call's start (for
+   profiling): [0] first chunks packed and the launch enqueued, [1] every chunk packed and
+   published, [2] every chunk's results unpacked, [3] the launch (and its careful pass) complete,
+   [4] the call done; [5] the launch's own duration on the GPU (HIP events). */
+#define SIGOPS_HOST_PHASES 6
+int sigops_batch_host_ms(sigops_ctx* ctx, float* ms);
+
 /* device pointers (d_msgs non-NULL, 4-byte aligned and readable 16 bytes past the last message), enqueued
    on `stream`; the context's bucketing and lane scratch are shared, so calls on one context must
    not overlap on the device */

@@ -137,12 +137,14 @@ def test_batch_stage_timing_path(ctx):
     assert np.array_equal(out2, ref_out) and np.array_equal(st2, ref_st)
 
 
-def test_autotune_launch_shapes_agree():
-    """every launch shape the autotuner tries gives the same outputs (it raises otherwise)"""
+def test_autotune_variants_agree():
+    """every variant the autotuner tries (waves per SIMD, tile rounds, bucket order) gives the same
+    outputs (it raises otherwise), and a decision is recorded"""
     at = __import__("importlib").import_module("wgpu-sigops_amd.autotune")
-    entry = at.tune(n=8192, steps=1, candidates=((2, 1), (1, 2), (3, 4)), write=False, log=_say)
-    assert {(r["blocks_per_cu"], r["tile_rounds"]) for r in entry["results"]} == {(2, 1), (1, 2), (3, 4)}
-    assert all(r["ms_per_batch"] > 0 for r in entry["results"])
+    entry = at.tune(n=8192, steps=1, rounds=2, write=False, log=_say)
+    assert len(entry["results"]) == len(at.CANDIDATES)
+    assert all(r["ms_median"] > 0 for r in entry["results"])
+    assert {k: entry[k] for k in at.DEFAULT} in list(at.CANDIDATES)
 
 
 def test_batch_edge_cases(ctx):

@@ -39,7 +39,8 @@ class SigopsError(RuntimeError):
 
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("blocks_per_cu", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
-                ("tile_rounds", ctypes.c_uint32), ("no_graph", ctypes.c_uint32)]
+                ("tile_rounds", ctypes.c_uint32), ("no_graph", ctypes.c_uint32), ("waves_per_simd", ctypes.c_uint32),
+                ("bucket_order", ctypes.c_uint32)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -99,6 +100,8 @@ def lib():
                 getattr(L, name).argtypes = [vp, c_p, c_p, sz, c_p, c_p]
             L.sigops_set_stage_timing.argtypes = [vp, ctypes.c_int]
             L.sigops_set_fault.argtypes = [vp, ctypes.c_uint32]
+            L.sigops_set_streaming.argtypes = [vp, ctypes.c_int]
+            L.sigops_batch_host_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
             L.sigops_batch_stage_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
             L.sigops_gen_secp_dev.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_uint32, vp, vp, vp, vp,
                                               vp]
@@ -177,16 +180,20 @@ class Context:
     devices are independent. The host-array methods split inputs larger than max_batch into
     consecutive calls of at most max_batch items."""
 
-    def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True, tuned=True):
+    def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True, tuned=True, waves=0,
+                 bucket_order=0):
         self._lock = threading.RLock()
         self.max_batch = max_batch or DEFAULT_MAX_BATCH
-        if tuned and not blocks_per_cu and not tile_rounds:
-            # the launch shape the autotuner measured fastest on this device (autotune.py), if any
+        if tuned and not (blocks_per_cu or tile_rounds or waves or bucket_order):
+            # the variant the autotuner measured fastest on this device (autotune.py), if any
             t = _tuned(device)
             if t:
-                blocks_per_cu, tile_rounds = t["blocks_per_cu"], t["tile_rounds"]
-        self.blocks_per_cu, self.tile_rounds = blocks_per_cu, tile_rounds
-        cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1)
+                waves, tile_rounds, bucket_order = t.get("waves", 0), t.get("tile_rounds", 0), t.get("bucket_order", 0)
+        self.waves = waves or 2
+        self.blocks_per_cu = blocks_per_cu or self.waves
+        self.tile_rounds = tile_rounds or 1
+        self.bucket_order = bucket_order
+        cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1, waves, bucket_order)
         self._h = lib().sigops_create(ctypes.byref(cfg))
         if not self._h:
             raise SigopsError("sigops_create failed (no usable GPU?)")
@@ -227,6 +234,20 @@ class Context:
         """[a, b) ranges of at most max_batch items covering n"""
         step = max(1, self.max_batch)
         return [(a, min(n, a + step)) for a in range(0, n, step)] or [(0, 0)]
+
+    def set_streaming(self, on):
+        """host batches streamed through pinned memory while the kernel runs (default), or copied
+        to the device first (sigops_set_streaming); the results are the same"""
+        self._call(lib().sigops_set_streaming, 1 if on else 0)
+
+    HOST_PHASES = ("head_packed_launched", "all_packed", "all_unpacked", "launch_done", "call_done", "launch_gpu_ms")
+
+    def batch_host_ms(self):
+        """phases of the last streamed host batch (ms from the call's start; the last is the launch's
+        own GPU time), keyed by HOST_PHASES"""
+        ms = (ctypes.c_float * len(self.HOST_PHASES))()
+        self._call(lib().sigops_batch_host_ms, ms)
+        return dict(zip(self.HOST_PHASES, (round(float(x), 3) for x in ms)))
 
     def set_fault(self, flags):
         """test switch (sigops_set_fault): FAULT_SKIP_COMPUTE makes every recovery, verification

@@ -154,28 +154,35 @@ DEV void store16(uint4* p, const u32 o[16]) {
   for (int c = 0; c < 4; c++) p[c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
 }
 
+// one recovery on the fast ladder: the status (ST_PENDING when the ladder met an exceptional
+// addition and the careful pass must redo the item) and the key X || Y (zeros unless ST_OK)
 template <class C, class GT>
-DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) {
-  u32 sr[16], dr[8], o[16];
-  load_words(a.sigs + 4 * (u64)i, sr, 4);
-  load_words(a.aux + 2 * (u64)i, dr, 2);
+DEV u32 secp_item(const u32 sr[16], const u32 dr[8], STabDev& tab, const GT& gt, u32 o[16]) {
   secp_pro pro;
   secp_prologue<C>(sr, dr, pro);
   u32 st = pro.status;
   if (st == ST_OK) {
     bool exc = false;
     const jac Q = secp_ladder<C>(pro, tab, gt, exc);
-    if (exc) {
+    if (exc)
       st = ST_PENDING;
-      a.pend_list[atomicAdd(a.pend_count, 1u)] = i;
-    } else {
+    else
       secp_output<C>(Q, o);
-    }
   }
   if (st != ST_OK) {
 #pragma unroll
     for (int k = 0; k < 16; k++) o[k] = 0;
   }
+  return st;
+}
+
+template <class C, class GT>
+DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) {
+  u32 sr[16], dr[8], o[16];
+  load_words(a.sigs + 4 * (u64)i, sr, 4);
+  load_words(a.aux + 2 * (u64)i, dr, 2);
+  const u32 st = secp_item<C>(sr, dr, tab, gt, o);
+  if (st == ST_PENDING) a.pend_list[atomicAdd(a.pend_count, 1u)] = i;
   store16(a.out + 4 * (u64)i, o);
   a.status[i] = (u8)st;
 }
@@ -186,8 +193,12 @@ DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) 
 constexpr u32 RUN_K1 = 1, RUN_R1 = 2, RUN_ED = 4, RUN_ED_STRICT = 8;
 constexpr u32 RUN_MIXED = RUN_K1 | RUN_R1 | RUN_ED;
 
-template <u32 MASK>
-__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
+// WAVES: waves per SIMD the kernel is built for, which sets its register budget per lane (512 /
+// WAVES): 2 by default (256 registers, some values spilled to scratch memory); 1 lets the
+// compiler keep them in registers (accumulation registers included) with half the waves to hide
+// latencies. The autotuner measures both.
+template <u32 MASK, int WAVES = SIGOPS_WAVES_PER_EU>
+__global__ __launch_bounds__(256, WAVES) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
                                                    const uint4* __restrict__ gr1, const uint4* __restrict__ gb,
                                                    uint4* __restrict__ lane_tab, u32 slots) {
   static_assert(!((MASK & RUN_ED) && (MASK & RUN_ED_STRICT)), "one Ed25519 rule set per launch");
@@ -321,6 +332,218 @@ hipError_t launch_ed_verify(const BatchLaunch& L, bool strict, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---- streamed batch (layout and host side in stream_host.hpp) ----
+struct StreamArgs {
+  const uint4* desc;
+  const uint4* in;
+  const u32* mrel;
+  const u32* mlen;
+  const uint4* hmsgs;
+  uint4* dmsgs;
+  uint4* out;
+  u8* st;
+  u32* tile_done;
+  const u64* ready;
+  const u32* stop;
+  u32* err;
+  u32* ctr;
+  u32* pend_list;
+  u32 tpc, slots, seq;
+};
+
+// Polls between sleeps of 4 x 127 x 64 cycles (about 13 us); 2^20 polls are about 15 s, far beyond
+// any packing delay, so a wave gives up only when the host stopped feeding the launch
+constexpr u32 STREAM_SPIN_LIMIT = 1u << 20;
+
+// Waits until the host has published chunk c of this batch (seq); false when it gave up (the
+// host's stop flag, or the spin limit), after marking the launch failed. `known` is the count of
+// published chunks this wavefront last read: the pinned word is read only for a chunk beyond it
+// (the host usually packs well ahead of the kernel, so most tiles need no PCIe round trip). One
+// lane polls; the wavefront follows its answer.
+DEV bool stream_wait(const StreamArgs& a, u32 c, u32& known) {
+  if (c < known) return true;
+  u32 ok = 1, seen = known;
+  if ((threadIdx.x & 63u) == 0) {
+    for (u32 spin = 0;; spin++) {
+      const u64 r = __hip_atomic_load(a.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if ((u32)(r >> 32) == a.seq && (u32)r > c) {
+        seen = (u32)r;
+        break;
+      }
+      if (spin >= STREAM_SPIN_LIMIT || __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+        __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(127);
+      __builtin_amdgcn_s_sleep(127);
+      __builtin_amdgcn_s_sleep(127);
+      __builtin_amdgcn_s_sleep(127);
+    }
+  }
+  ok = __shfl(ok, 0);
+  known = __shfl(seen, 0);
+  // the tile's reads come after the published count (the pinned buffers are not cached)
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return ok != 0;
+}
+
+// inputs of lane l of slot t from the tile layout: 16 signature words, 8 digest / key words
+DEV void stream_load(const StreamArgs& a, u32 t, u32 l, u32 sr[16], u32 ar[8]) {
+  const uint4* in = a.in + (u64)t * (6 * 64) + l;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const uint4 q = in[c * 64];
+    sr[4 * c] = q.x; sr[4 * c + 1] = q.y; sr[4 * c + 2] = q.z; sr[4 * c + 3] = q.w;
+  }
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const uint4 q = in[(4 + c) * 64];
+    ar[4 * c] = q.x; ar[4 * c + 1] = q.y; ar[4 * c + 2] = q.z; ar[4 * c + 3] = q.w;
+  }
+}
+
+DEV void stream_store_out(const StreamArgs& a, u32 t, u32 l, const u32 o[16]) {
+  uint4* out = a.out + (u64)t * (4 * 64) + l;
+#pragma unroll
+  for (int c = 0; c < 4; c++) out[c * 64] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
+}
+
+// Persistent: each wavefront takes the next tile slot, waits until its chunk is published, and
+// processes the tile's items (one per lane) straight from pinned memory; an Ed25519 tile first
+// copies its messages into HBM with 1 KB-wide loads (the hash reads them word by word), and the
+// results go back to pinned memory, after which the tile's done word is set. Secp items whose fast
+// ladder meets an exceptional addition go to the pending list for k_stream_careful.
+__global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamArgs a, const uint4* __restrict__ gk1,
+                                                                    const uint4* __restrict__ gr1,
+                                                                    const uint4* __restrict__ gb,
+                                                                    uint4* __restrict__ lane_tab, u32 lane_slots) {
+  __shared__ uint4 s_k1[GTAB_ENTRIES * gtab_chunks<CurveK1>()];
+  __shared__ uint4 s_r1[GTAB_ENTRIES * gtab_chunks<CurveR1>()];
+  __shared__ uint4 s_b[BTAB_ENTRIES * 7];
+  stage_gtab<CurveK1>(s_k1, gk1);
+  stage_gtab<CurveR1>(s_r1, gr1);
+  for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
+  __syncthreads();
+  const u32 lane = threadIdx.x & 63u;
+  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+  STabDev jt;
+  jt.base = lane_tab;
+  jt.slots = lane_slots;
+  jt.slot = slot;
+  CTabDev ct;
+  ct.base = lane_tab;
+  ct.slots = lane_slots;
+  ct.slot = slot;
+  u32 known = 0;  // chunks this wavefront knows to be published
+  for (;;) {
+    u32 t = 0;
+    if (lane == 0) t = atomicAdd(&a.ctr[0], 1u);
+    t = __shfl(t, 0);
+    if (t >= a.slots) break;
+    if (!stream_wait(a, t / a.tpc, known)) break;
+    const uint4 d = a.desc[t];
+    const u32 info = __builtin_amdgcn_readfirstlane(d.x);
+    const u32 cnt = info & 0xFFu, b = info >> 8;
+    if (cnt == 0) continue;  // an unused slot of its chunk
+    if (b >= BUCKET_ED0) {
+      // the tile's messages, [m0, m0 + mbytes) rounded up to 16 bytes, pinned -> HBM
+      const u64 m0 = ((u64)d.w << 32) | d.z;
+      const u64 q0 = m0 >> 4, nq = ((u64)d.y + 15) >> 4;
+      const uint4* src = a.hmsgs + q0;
+      uint4* dst = a.dmsgs + q0;
+      u64 q = lane;
+      for (; q + 192 < nq; q += 256) {  // four PCIe loads in flight per lane
+        const uint4 v0 = src[q], v1 = src[q + 64], v2 = src[q + 128], v3 = src[q + 192];
+        dst[q] = v0;
+        dst[q + 64] = v1;
+        dst[q + 128] = v2;
+        dst[q + 192] = v3;
+      }
+      for (; q < nq; q += 64) dst[q] = src[q];
+      // the wave's copy is complete and visible before its lanes hash the messages
+      __threadfence();
+      if (lane < cnt) {
+        u32 sr[16], ar[8];
+        stream_load(a, t, lane, sr, ar);
+        const u32 st = ed25519_verify_item<false>(sr, ar, MemDev{(const u32*)a.dmsgs},
+                                                  m0 + a.mrel[(u64)t * 64 + lane], a.mlen[(u64)t * 64 + lane], ct,
+                                                  BTabLds{s_b});
+        a.st[(u64)t * 64 + lane] = (u8)st;
+      }
+    } else if (lane < cnt) {
+      u32 sr[16], ar[8], o[16];
+      stream_load(a, t, lane, sr, ar);
+      const u32 st = b == BUCKET_K1 ? secp_item<CurveK1>(sr, ar, jt, GTabOf<CurveK1>{s_k1}, o)
+                                    : secp_item<CurveR1>(sr, ar, jt, GTabOf<CurveR1>{s_r1}, o);
+      if (st == ST_PENDING) a.pend_list[atomicAdd(&a.ctr[1], 1u)] = t * 64 + lane;
+      stream_store_out(a, t, lane, o);
+      a.st[(u64)t * 64 + lane] = (u8)st;
+    }
+    // every lane's results reach host memory before the tile is marked done: the stores into the
+    // pinned buffers may sit in L2, so the release is at system scope (waiting for the stores
+    // alone left results the host could not yet see)
+    __threadfence_system();
+    if (lane == 0) __hip_atomic_store(&a.tile_done[t], a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// the careful pass of the streamed batch: pending slot lanes, inputs and results in the tile layout
+__global__ __launch_bounds__(64) void k_stream_careful(StreamArgs a, const uint4* __restrict__ gk1,
+                                                       const uint4* __restrict__ gr1) {
+  const u32 cnt = a.ctr[1];
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const u32 p = a.pend_list[j];
+    const u32 t = p >> 6, l = p & 63u;
+    const u32 b = a.desc[t].x >> 8;
+    u32 sr[16], ar[8], o[16];
+    stream_load(a, t, l, sr, ar);
+    secp_pro pro;
+    jacf Q;
+    if (b == BUCKET_K1) {
+      secp_prologue<CurveK1>(sr, ar, pro);
+      Q = secp_ladder_careful<CurveK1>(pro, GTabGlobal{gk1});
+      if (!Q.inf) secp_output<CurveK1>(Q.P, o);
+    } else {
+      secp_prologue<CurveR1>(sr, ar, pro);
+      Q = secp_ladder_careful<CurveR1>(pro, GTabGlobal{gr1});
+      if (!Q.inf) secp_output<CurveR1>(Q.P, o);
+    }
+    if (Q.inf) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) o[k] = 0;
+    }
+    stream_store_out(a, t, l, o);
+    a.st[p] = (u8)(Q.inf ? ST_INFINITY : ST_OK);
+  }
+}
+
+hipError_t launch_stream(const StreamLaunch& L, hipStream_t s) {
+  StreamArgs a;
+  a.desc = (const uint4*)L.desc;
+  a.in = (const uint4*)L.in;
+  a.mrel = L.mrel;
+  a.mlen = L.mlen;
+  a.hmsgs = (const uint4*)L.hmsgs;
+  a.dmsgs = (uint4*)L.dmsgs;
+  a.out = (uint4*)L.out;
+  a.st = L.st;
+  a.tile_done = L.tile_done;
+  a.ready = L.ready;
+  a.stop = L.stop;
+  a.err = L.err;
+  a.ctr = L.ctr;
+  a.pend_list = L.pend_list;
+  a.tpc = L.tpc;
+  a.slots = L.slots;
+  a.seq = L.seq;
+  k_zero_u32<<<1, 64, 0, s>>>(L.ctr, 2);
+  k_stream_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+                                      (uint4*)L.lane_tab, L.lane_slots);
+  k_stream_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   const BatchArgs a = batch_args(L);
   // stage events exist only on the eager (timed) path, never under graph capture
@@ -336,8 +559,12 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   // (the Ed25519 challenges are hashed inside k_batch_run; the stage stays empty, so that the
   // stage indices of earlier measurements keep their meaning)
   if ((e = mark(2)) != hipSuccess) return e;
-  k_batch_run<RUN_MIXED><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
-                                     (uint4*)L.lane_tab, L.slots);
+  if (L.waves == 1)
+    k_batch_run<RUN_MIXED, 1><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+                                                    (uint4*)L.lane_tab, L.slots);
+  else
+    k_batch_run<RUN_MIXED><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+                                                 (uint4*)L.lane_tab, L.slots);
   if ((e = mark(3)) != hipSuccess) return e;
   k_batch_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);
   if ((e = mark(4)) != hipSuccess) return e;

@@ -60,6 +60,7 @@ struct BatchLaunch {
   const void* gb;
   void* lane_tab;
   uint32_t slots, grid, aux_grid, tile_rounds;
+  uint32_t waves;        // mixed batch: waves per SIMD the persistent kernel is built for (1 or 2)
   hipEvent_t* stage_ev;  // null, or 5 events recorded before, between and after the 4 stages
 };
 
@@ -68,6 +69,31 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
 // bucketing, then the persistent kernel on Ed25519 tiles with the challenges hashed inside it;
 // strict: ed25519-dalek verify_strict rules; L.gb is the fixed-base table
 hipError_t launch_ed_verify(const BatchLaunch& L, bool strict, hipStream_t s);
+
+// ---- streamed batch: the host packs chunks of the batch into pinned, device-mapped memory while
+// one persistent launch drains them (layout in stream_host.hpp) ----
+struct StreamLaunch {
+  const void* desc;      // pinned: 16-byte tile descriptors, tpc per chunk
+  const void* in;        // pinned: inputs, [slot][6][64] 16-byte chunks
+  const uint32_t* mrel;  // pinned: per slot lane, message start relative to the tile's
+  const uint32_t* mlen;  // pinned: per slot lane, message length
+  const void* hmsgs;     // pinned: the tiles' messages
+  void* dmsgs;           // device: the messages copied by their tiles (same offsets), 16 bytes of slack
+  void* out;             // pinned: results, [slot][4][64] 16-byte chunks
+  uint8_t* st;           // pinned: statuses, [slot][64]
+  uint32_t* tile_done;   // pinned: per slot, set to seq once the tile's results are written
+  const uint64_t* ready; // pinned: seq << 32 | chunks published so far
+  const uint32_t* stop;  // pinned: nonzero makes waiting waves give up
+  uint32_t* err;         // pinned: set to 1 by a wave that gave up waiting
+  uint32_t* ctr;         // device: [0] next tile slot, [1] pending count (cleared by the launch)
+  uint32_t* pend_list;   // device: slot lanes (slot * 64 + lane) the careful pass redoes
+  const void* gk1;
+  const void* gr1;
+  const void* gb;
+  void* lane_tab;
+  uint32_t tpc, slots, seq, lane_slots, grid;
+};
+hipError_t launch_stream(const StreamLaunch& L, hipStream_t s);
 
 // SHA-256: digests of messages (Fuel Message::new) and addresses of 64-byte public keys.
 // work: MSG_WORK_WORDS device words (cleared by the launch); order: n words, the longest-first

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <memory>
 #include <mutex>
@@ -19,6 +20,7 @@
 #include "consts.hpp"
 #include "host_xfer.hpp"
 #include "kernels.hpp"
+#include "stream_host.hpp"
 
 // ---- scratch of sigops_message_new_noalloc ----
 // The hashing kernels' work words (the first NOALLOC_WORK_BYTES) and the longest-first order of up
@@ -61,6 +63,7 @@ struct sigops_ctx {
   uint32_t* d_pend = nullptr;  // [0] = count, [1 ..] = list
   // mixed batch
   uint32_t tile_rounds = 1;
+  uint32_t waves = 2;  // waves per SIMD the mixed batch's persistent kernel is built for
   bool use_graph = true;
   uint8_t* d_rank = nullptr;
   uint32_t* d_work = nullptr;
@@ -77,6 +80,21 @@ struct sigops_ctx {
   hipGraphExec_t graph = nullptr;
   DevBuf in, aux, msg, off, out, st, sch;
   uint32_t fault = 0;  // SIGOPS_FAULT_* test switch
+  // streamed host batches (stream_host.hpp): one pinned, device-mapped block carved into the
+  // layout's regions, a pinned control block, the device counters, the batch tag
+  bool streaming = true;
+  uint32_t bucket_order[sigops::NBUCKETS];  // processing rank -> bucket (the rank table inverted)
+  uint8_t* s_pin = nullptr;
+  size_t s_cap = 0;
+  uint8_t* s_ctl = nullptr;  // u64 ready, u32 stop, u32 err
+  uint32_t* d_sctr = nullptr;
+  uint32_t s_seq = 0;
+  hipEvent_t s_ev = nullptr;
+  hipEvent_t s_tev[2] = {};  // timing events around the streamed launch
+  float s_ms[SIGOPS_HOST_PHASES] = {};  // phases of the last streamed batch
+  std::vector<uint32_t> s_perm;
+  std::vector<uint32_t> s_sorted;  // the batch in processing order (caller indices)
+  std::vector<uint8_t> s_bkt;      // bucket of every item
   void* noalloc_scratch = nullptr;  // device address of `scratch` on this context's device
   bool stage_timing = false;
   bool stage_recorded = false;
@@ -323,7 +341,13 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
   int cus = 0;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)) != hipSuccess)
     return bad(e, "hipDeviceGetAttribute");
-  const uint32_t bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : 2;
+  if (cfg->waves_per_simd > 2 || cfg->bucket_order > 2) {
+    fprintf(stderr, "sigops_create: waves_per_simd must be 0..2 and bucket_order 0..2\n");
+    sigops_destroy(c);
+    return nullptr;
+  }
+  c->waves = cfg->waves_per_simd ? cfg->waves_per_simd : 2;
+  const uint32_t bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : c->waves;
   c->grid = (uint32_t)cus * bpc;
   c->slots = c->grid * 256;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
@@ -346,20 +370,23 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
   if (cfg->tile_rounds) c->tile_rounds = std::min<uint32_t>(cfg->tile_rounds, sigops::MAX_TILE_ROUNDS);
   c->use_graph = cfg->no_graph == 0;
   {
-    // processing order of the buckets (higher key first): secp256r1, then secp256k1, then Ed25519
-    // by decreasing SHA-512 block count. Measured on MI355X, alternating builds on one box, for the
-    // 2^20 mixed batch: 16.30 ms in this order; 16.55 ms for an estimated-cost order that put the
-    // few longest Ed25519 buckets first and the short ones after secp256k1; 16.67 ms with
-    // secp256k1 before secp256r1; about 17 ms with the Ed25519 buckets first. Ending on the
-    // cheapest tiles (secp256r1, Ed25519, then secp256k1) shortens the final tiles but took
-    // 16.15-16.30 ms against 15.87-15.88 ms for this order (4 alternations on one box).
+    // processing order of the buckets (higher key first). Default: secp256r1, then secp256k1, then
+    // Ed25519 by decreasing SHA-512 block count. Measured on MI355X, alternating builds on one box,
+    // for the 2^20 mixed batch: 16.30 ms in this order; 16.55 ms for an estimated-cost order that
+    // put the few longest Ed25519 buckets first and the short ones after secp256k1; 16.67 ms with
+    // secp256k1 before secp256r1 (order 2); about 17 ms with the Ed25519 buckets first. Ending on
+    // the cheapest tiles (secp256r1, Ed25519, then secp256k1: order 1) shortens the final tiles
+    // but took 16.15-16.30 ms against 15.87-15.88 ms for the default (4 alternations on one box).
+    // The autotuner measures the three again per device.
+    static const double KEYS[3][2] = {{2.0, 3.0}, {0.5, 3.0}, {3.0, 2.0}};  // (secp256k1, secp256r1)
     std::vector<std::pair<double, uint32_t>> key;
-    key.push_back({2.0, sigops::BUCKET_K1});
-    key.push_back({3.0, sigops::BUCKET_R1});
+    key.push_back({KEYS[cfg->bucket_order][0], sigops::BUCKET_K1});
+    key.push_back({KEYS[cfg->bucket_order][1], sigops::BUCKET_R1});
     for (uint32_t b = 0; b < sigops::ED_BINS; b++) key.push_back({1.0 + 0.001 * (b + 1), sigops::BUCKET_ED0 + b});
     std::stable_sort(key.begin(), key.end(), [](auto& x, auto& y) { return x.first > y.first; });
     uint8_t rank[sigops::NBUCKETS];
     for (uint32_t r = 0; r < sigops::NBUCKETS; r++) rank[key[r].second] = (uint8_t)r;
+    for (uint32_t r = 0; r < sigops::NBUCKETS; r++) c->bucket_order[r] = key[r].second;
     if ((e = hipMalloc((void**)&c->d_rank, sizeof rank)) != hipSuccess) return bad(e, "hipMalloc rank");
     if ((e = hipMemcpy(c->d_rank, rank, sizeof rank, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "upload");
   }
@@ -385,6 +412,12 @@ void sigops_destroy(sigops_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  if (c->s_ev) (void)hipEventDestroy(c->s_ev);
+  for (hipEvent_t e : c->s_tev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->s_pin) (void)hipHostFree(c->s_pin);
+  if (c->s_ctl) (void)hipHostFree(c->s_ctl);
+  if (c->d_sctr) (void)hipFree(c->d_sctr);
   for (hipEvent_t e : c->stage_ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : {c->d_gtab_k1, c->d_gtab_r1, c->d_btab, c->d_lane_tab, (void*)c->d_pend, c->in.p, c->aux.p,
@@ -625,6 +658,7 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
   L.grid = c->grid;
   L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
   L.tile_rounds = c->tile_rounds;
+  L.waves = c->waves;
   L.stage_ev = nullptr;
   if (c->stage_timing) {
     L.stage_ev = c->stage_ev;
@@ -756,6 +790,235 @@ int sigops_batch_stage_ms(sigops_ctx* c, float* ms) {
   return SIGOPS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// regions of the streamed layout in the pinned block, each 256-byte aligned
+struct StreamRegions {
+  size_t desc, in, mrel, mlen, out, st, done, msgs, total;
+};
+
+StreamRegions stream_regions(const sigops_host::StreamGeom& g) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  StreamRegions r;
+  size_t at = 0;
+  r.desc = at;
+  at = al(at + g.slots * 16);
+  r.in = at;
+  at = al(at + g.slots * sigops_host::ST_IN_CHUNKS * 64 * 16);
+  r.mrel = at;
+  at = al(at + g.slots * 64 * 4);
+  r.mlen = at;
+  at = al(at + g.slots * 64 * 4);
+  r.out = at;
+  at = al(at + g.slots * sigops_host::ST_OUT_CHUNKS * 64 * 16);
+  r.st = at;
+  at = al(at + g.slots * 64);
+  r.done = at;
+  at = al(at + g.slots * 4);
+  r.msgs = at;
+  at = al(at + g.msg_base[g.nchunks] + 16);
+  r.total = at;
+  return r;
+}
+
+// device address of a pinned, device-mapped host address
+void* dev_addr(void* host) {
+  void* d = nullptr;
+  return hipHostGetDevicePointer(&d, host, 0) == hipSuccess ? d : nullptr;
+}
+
+// Streamed mixed batch (caller buffers validated). Returns SIGOPS_OK with every result written;
+// 1 when the launch could not take every chunk (the caller falls back to the copy-then-launch
+// path); or an error code.
+int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux,
+                   const uint8_t* msgs, const uint64_t* msg_off, size_t n, uint8_t* out, uint8_t* status) {
+  using namespace sigops_host;
+  typedef std::chrono::steady_clock clk;
+  const auto T0 = clk::now();
+  auto since = [&](clk::time_point t) { return std::chrono::duration<float, std::milli>(t - T0).count(); };
+  std::atomic<int64_t> t_packed{0};  // when the last chunk was published (ns after T0)
+  // chunks: about 1/16 of the batch, 1024..16384 items, cut from the batch sorted by bucket
+  const size_t chunk = std::min<size_t>(16384, std::max<size_t>(1024, ((n / 16) + 63) & ~(size_t)63));
+  StreamGeom g = stream_geometry(n, chunk);
+  Pool& pool = host_pool(c);
+  static_assert(ST_BUCKETS == sigops::NBUCKETS && ST_ED0 == sigops::BUCKET_ED0, "one bucket numbering");
+  c->s_bkt.resize(n);
+  c->s_sorted.resize(n);
+  const uint32_t* sorted = c->s_sorted.data();
+  const uint8_t* bkt = c->s_bkt.data();
+  stream_sort([&](const std::function<void(unsigned)>& fn) { pool.run(fn); }, pool.size(), n, schemes, msg_off,
+              c->bucket_order, c->s_bkt.data(), c->s_sorted.data());
+  {
+    std::atomic<size_t> next_k{0};
+    pool.run([&](unsigned) {
+      for (size_t k; (k = next_k.fetch_add(1)) < g.nchunks;) stream_message_bytes(g, k, sorted, msg_off);
+    });
+  }
+  for (size_t k = 0; k < g.nchunks; k++)
+    if (g.msg_base[k + 1] >= (1ull << 32)) return 1;  // descriptors hold 32-bit tile sizes
+  stream_message_layout(g);
+  const StreamRegions R = stream_regions(g);
+  hipError_t e;
+  if (!c->s_ctl) {
+    if ((e = hipHostMalloc((void**)&c->s_ctl, 256, hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess)
+      return xfer_fail(c, e, "streamed batch control");
+    memset(c->s_ctl, 0, 256);
+    if ((e = hipMalloc((void**)&c->d_sctr, 2 * sizeof(uint32_t))) != hipSuccess) return hip_fail(c, e, "hipMalloc");
+    if ((e = hipEventCreateWithFlags(&c->s_ev, hipEventDisableTiming)) != hipSuccess) return hip_fail(c, e, "event");
+    for (hipEvent_t& te : c->s_tev)
+      if ((e = hipEventCreate(&te)) != hipSuccess) return hip_fail(c, e, "event");
+  }
+  if (c->s_cap < R.total) {
+    if (c->s_pin) (void)hipHostFree(c->s_pin);
+    c->s_pin = nullptr;
+    c->s_cap = 0;
+    const size_t want = (R.total + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+    if ((e = hipHostMalloc((void**)&c->s_pin, want, hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess) {
+      c->s_pin = nullptr;
+      return xfer_fail(c, e, "streamed batch staging");
+    }
+    memset(c->s_pin, 0, want);  // the tile done words start below any batch tag
+    c->s_cap = want;
+  }
+  int rc;
+  if ((rc = ensure(c, c->msg, g.msg_base[g.nchunks] + 16))) return rc;  // HBM copies of the tiles' messages
+  c->s_perm.resize(g.slots * 64);
+  uint8_t* P = c->s_pin;
+  StreamBufs B{(StreamDesc*)(P + R.desc), P + R.in, (uint32_t*)(P + R.mrel), (uint32_t*)(P + R.mlen), P + R.msgs,
+               P + R.out, P + R.st, c->s_perm.data()};
+  uint32_t* done = (uint32_t*)(P + R.done);
+  uint64_t* ready = (uint64_t*)c->s_ctl;
+  uint32_t* stop = (uint32_t*)(c->s_ctl + 8);
+  uint32_t* err = (uint32_t*)(c->s_ctl + 12);
+  const uint32_t seq = ++c->s_seq ? c->s_seq : ++c->s_seq;  // never 0, the cleared done words
+  __atomic_store_n(ready, (uint64_t)seq << 32, __ATOMIC_RELEASE);
+  __atomic_store_n(stop, 0u, __ATOMIC_RELEASE);
+  __atomic_store_n(err, 0u, __ATOMIC_RELEASE);
+  std::unique_ptr<std::atomic<uint32_t>[]> packed(new std::atomic<uint32_t>[g.nchunks]());
+  std::vector<uint32_t> tiles(g.nchunks, 0);
+  std::mutex pub_mu;
+  size_t published = 0;
+  auto pack = [&](size_t k) {
+    tiles[k] = stream_pack_chunk(g, k, sorted, bkt, sigs, aux, msgs, msg_off, B);
+    packed[k].store(1, std::memory_order_release);
+    std::lock_guard<std::mutex> lk(pub_mu);
+    while (published < g.nchunks && packed[published].load(std::memory_order_acquire)) published++;
+    __atomic_store_n(ready, ((uint64_t)seq << 32) | published, __ATOMIC_RELEASE);
+    if (published == g.nchunks)
+      t_packed.store(std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - T0).count());
+  };
+  // chunks packed before the launch: enough tiles for half the resident wavefronts, so that the
+  // waves that start first find work instead of polling host memory
+  const size_t waves = (size_t)c->grid * 4;
+  const size_t head = std::min(g.nchunks, (waves / 2 + chunk / 64 - 1) / (chunk / 64) + 1);
+  std::atomic<size_t> next{0};
+  pool.run([&](unsigned) {
+    for (size_t k; (k = next.fetch_add(1)) < head;) pack(k);
+  });
+  hipStream_t s = c->stream;
+  sigops::StreamLaunch L{};
+  // device addresses of the two pinned blocks; the regions sit at the same offsets
+  uint8_t* DP = (uint8_t*)dev_addr(P);
+  uint8_t* DC = (uint8_t*)dev_addr(c->s_ctl);
+  if (!DP || !DC) return fail(c, SIGOPS_E_DEVICE, "hipHostGetDevicePointer failed for the streamed batch");
+  L.desc = DP + R.desc;
+  L.in = DP + R.in;
+  L.mrel = (const uint32_t*)(DP + R.mrel);
+  L.mlen = (const uint32_t*)(DP + R.mlen);
+  L.hmsgs = DP + R.msgs;
+  L.dmsgs = c->msg.p;
+  L.out = DP + R.out;
+  L.st = DP + R.st;
+  L.tile_done = (uint32_t*)(DP + R.done);
+  L.ready = (const uint64_t*)DC;
+  L.stop = (const uint32_t*)(DC + 8);
+  L.err = (uint32_t*)(DC + 12);
+  L.ctr = c->d_sctr;
+  L.pend_list = c->d_pend + 1;
+  L.gk1 = c->d_gtab_k1;
+  L.gr1 = c->d_gtab_r1;
+  L.gb = c->d_btab;
+  L.lane_tab = c->d_lane_tab;
+  L.tpc = (uint32_t)g.tpc;
+  L.slots = (uint32_t)g.slots;
+  L.seq = seq;
+  L.lane_slots = c->slots;
+  L.grid = std::min<uint32_t>(c->grid, (uint32_t)((g.slots + 3) / 4));
+  HIPCHK(c, hipEventRecord(c->s_tev[0], s));
+  if (!(c->fault & SIGOPS_FAULT_SKIP_COMPUTE)) HIPCHK(c, sigops::launch_stream(L, s));
+  HIPCHK(c, hipEventRecord(c->s_tev[1], s));
+  HIPCHK(c, hipEventRecord(c->s_ev, s));
+  const auto T1 = clk::now();
+  // the rest of the chunks, then the results of every chunk as its tiles finish
+  std::atomic<size_t> next_unpack{0};
+  std::atomic<bool> lost{false};
+  next.store(head);
+  pool.run([&](unsigned) {
+    for (size_t k; (k = next.fetch_add(1)) < g.nchunks;) pack(k);
+    for (size_t k; (k = next_unpack.fetch_add(1)) < g.nchunks;) {
+      while (!packed[k].load(std::memory_order_acquire)) std::this_thread::yield();
+      const size_t t0 = k * g.tpc;
+      bool ok = true;
+      for (uint32_t j = 0; j < tiles[k] && ok; j++) {
+        for (unsigned spin = 1; __atomic_load_n(&done[t0 + j], __ATOMIC_ACQUIRE) != seq; spin++) {
+          if (spin % 256 == 0) {
+            // the launch has ended: a tile still not done will not be
+            if (hipEventQuery(c->s_ev) == hipSuccess && __atomic_load_n(&done[t0 + j], __ATOMIC_ACQUIRE) != seq) {
+              ok = false;
+              break;
+            }
+            std::this_thread::yield();
+          }
+        }
+      }
+      if (!ok) {
+        lost.store(true);
+        __atomic_store_n(stop, 1u, __ATOMIC_RELEASE);
+        continue;
+      }
+      stream_unpack_tiles(t0, t0 + tiles[k], B, out, status);
+    }
+  });
+  const auto T3 = clk::now();
+  HIPCHK(c, hipEventSynchronize(c->s_ev));
+  const auto T4 = clk::now();
+  c->s_ms[0] = since(T1);
+  c->s_ms[1] = (float)(t_packed.load() / 1e6);
+  c->s_ms[2] = since(T3);
+  c->s_ms[3] = since(T4);
+  (void)hipEventElapsedTime(&c->s_ms[5], c->s_tev[0], c->s_tev[1]);
+  if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return fail(c, SIGOPS_E_CANARY, "a batch result was not written");
+  if (lost.load() || __atomic_load_n(err, __ATOMIC_ACQUIRE)) return 1;
+  // items the careful pass redid
+  uint32_t ctr[2];
+  HIPCHK(c, hipMemcpy(ctr, c->d_sctr, sizeof ctr, hipMemcpyDeviceToHost));
+  if (ctr[1]) {
+    std::vector<uint32_t> pend(ctr[1]);
+    HIPCHK(c, hipMemcpy(pend.data(), c->d_pend + 1, pend.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t p : pend) stream_unpack_lane(p, B, out, status);
+  }
+  c->s_ms[4] = since(clk::now());
+  return SIGOPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sigops_batch_host_ms(sigops_ctx* c, float* ms) {
+  if (!c || !ms) return SIGOPS_E_ARG;
+  for (int i = 0; i < SIGOPS_HOST_PHASES; i++) ms[i] = c->s_ms[i];
+  return SIGOPS_OK;
+}
+
+int sigops_set_streaming(sigops_ctx* c, int on) {
+  if (!c) return SIGOPS_E_ARG;
+  c->streaming = on != 0;
+  return SIGOPS_OK;
+}
+
 int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, const uint8_t* aux, const uint8_t* msgs,
                  const uint64_t* msg_off, size_t n, uint8_t* out, uint8_t* status) {
   if (!c) return SIGOPS_E_ARG;
@@ -770,7 +1033,24 @@ int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, con
   const uint64_t base = msg_off[0];
   const uint64_t total = msg_off[n] - base;
   if (total && !msgs) return fail(c, SIGOPS_E_ARG, "NULL msgs");
+  uint8_t* st = status;
+  if (!st) {
+    c->host_status.resize(n);
+    st = c->host_status.data();
+  }
   int rc;
+  if (c->streaming && n >= 4096) {
+    rc = batch_streamed(c, schemes, sigs, aux, msgs, msg_off, n, out, st);
+    if (rc < 0) return rc;
+    if (rc == SIGOPS_OK) {
+      if (!all_of(c, n, [&](size_t i) {
+            return st[i] <= (schemes[i] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY);
+          }))
+        return fail(c, SIGOPS_E_CANARY, "a batch result was not written");
+      return SIGOPS_OK;
+    }
+    // rc == 1: the streamed launch did not take every chunk; redo the batch the copy-then-launch way
+  }
   if ((rc = ensure(c, c->in, n * 64)) || (rc = ensure(c, c->aux, n * 32)) || (rc = ensure(c, c->msg, total + 16)) ||
       (rc = ensure(c, c->off, (n + 1) * 8)) || (rc = ensure(c, c->out, n * 64)) || (rc = ensure(c, c->st, n)) ||
       (rc = ensure(c, c->sch, n)))
@@ -788,11 +1068,6 @@ int sigops_batch(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, con
   if ((rc = sigops_batch_dev(c, c->sch.p, c->in.p, c->aux.p, c->msg.p, (const uint64_t*)c->off.p, n, c->out.p,
                              c->st.p, s)))
     return rc;
-  uint8_t* st = status;
-  if (!st) {
-    c->host_status.resize(n);
-    st = c->host_status.data();
-  }
   if ((rc = get(c, {{out, c->out.p, n * 64}, {st, c->st.p, n}}))) return rc;
   if (!all_of(c, n, [&](size_t i) {
         return st[i] <= (schemes[i] == SIGOPS_SCHEME_ED25519 ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY);
