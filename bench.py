@@ -571,55 +571,59 @@ def main():
                                 args.cpu_sample)
 
     # served: Poisson arrivals of mixed requests (1..4096 items, log-uniform) at three offered loads,
-    # replayed by the library's native client against the continuous-batching server
+    # replayed by the library's native client against both servers: slot batches (requests packed
+    # into per-slot batches) and the device-drained ring (requests packed into tiles of one running
+    # persistent launch)
     served = None
     if not args.no_serve:
         served = []
         rng = np.random.default_rng(7)
         # serving moves every request over PCIe, so its capacity is the transfer-inclusive batch rate
         capacity = n / (incl_ms / 1e3)
-        with sigops.Server(slots=3) as srv:
-            warm = True
-            # the first pass warms up and is dropped: at the highest load, so that every slot's buffers
-            # reach their largest size (and their graphs are captured) before anything is measured
-            # the last pass submits every request at once: its sustained rate is the server's capacity
-            for load in (0.8, 0.25, 0.5, 0.8, "saturated"):
-                rate_items = capacity if load == "saturated" else load * capacity
-                mean_req = (4096 - 1) / np.log(4096)
-                nreq = int(max(2000, (0.5 if warm else args.serve_seconds) * rate_items / mean_req))
-                sizes = np.floor(np.exp(rng.random(nreq) * np.log(4096))).astype(np.int64).clip(1, 4096)
-                starts = rng.integers(0, n - 4096, nreq)
-                # requests are contiguous slices of the pool; the replay takes one offsets array, so
-                # lay the requests out back to back in a gathered copy
-                idx = np.concatenate([np.arange(a, a + k) for a, k in zip(starts, sizes)])
-                req_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
-                lens = (h_off[idx + 1] - h_off[idx]).astype(np.int64)
-                g_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
-                g_msgs = np.concatenate([h_msgs[h_off[i]:h_off[i + 1]] for i in idx[h_sch[idx] == 2]]) \
-                    if (h_sch[idx] == 2).any() else np.zeros(0, np.uint8)
-                # messages of non-Ed25519 items are empty, so the Ed25519 messages concatenate in order
-                arrivals = np.cumsum(rng.exponential(1e9 * mean_req / rate_items, nreq)).astype(np.uint64)
-                if load == "saturated":
-                    arrivals[:] = 0
-                r_sch, r_sigs, r_aux = h_sch[idx], h_sigs[idx], h_aux[idx]
-                lat, s_out, s_st = srv.replay(r_sch, r_sigs, r_aux, g_msgs, g_off, req_off, arrivals)
-                # on the replay's own clock: from the first arrival to the last completion
-                span_s = float((arrivals.astype(np.float64) + lat.astype(np.float64)).max()) / 1e9
-                same = bool(np.array_equal(s_out, out_h[idx]) and np.array_equal(s_st, st_h[idx]))
-                lat_ms = lat.astype(np.float64) / 1e6
-                if warm:
-                    warm = False
-                    continue
-                served.append({"load": load, "capacity_basis": "batched incl. transfers",
-                               "offered_sigs_per_s": None if load == "saturated" else round(rate_items, 1),
-                               "latency_from": "scheduled arrival", "requests": nreq,
-                               "items": int(req_off[-1]), "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-                               "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-                               "sustained_sigs_per_s": round(float(req_off[-1]) / span_s, 1),
-                               "results_identical_to_batch": same})
-                log(f"served load {load}: {nreq} requests, p50 {served[-1]['p50_ms']} ms, p99 {served[-1]['p99_ms']} ms, "
-                    f"{served[-1]['sustained_sigs_per_s'] / 1e6:.2f} M sigs/s sustained, identical: {same}")
-            served_batches = srv.stats()
+        mean_req = (4096 - 1) / np.log(4096)
+        for server, mk in (("ring", lambda: sigops.Server(mode=sigops.SERVE_RING)),
+                           ("slots", lambda: sigops.Server(slots=3))):
+            with mk() as srv:
+                warm = True
+                # the first pass warms up and is dropped: at the highest load, so that every buffer
+                # reaches its largest size before anything is measured; the last pass submits every
+                # request at once, so its sustained rate is the server's capacity
+                for load in (0.8, 0.25, 0.5, 0.8, "saturated"):
+                    rate_items = capacity if load == "saturated" else load * capacity
+                    nreq = int(max(2000, (0.5 if warm else args.serve_seconds) * rate_items / mean_req))
+                    sizes = np.floor(np.exp(rng.random(nreq) * np.log(4096))).astype(np.int64).clip(1, 4096)
+                    starts = rng.integers(0, n - 4096, nreq)
+                    # requests are contiguous slices of the pool; the replay takes one offsets array, so
+                    # lay the requests out back to back in a gathered copy
+                    idx = np.concatenate([np.arange(a, a + k) for a, k in zip(starts, sizes)])
+                    req_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+                    lens = (h_off[idx + 1] - h_off[idx]).astype(np.int64)
+                    g_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+                    g_msgs = np.concatenate([h_msgs[h_off[i]:h_off[i + 1]] for i in idx[h_sch[idx] == 2]]) \
+                        if (h_sch[idx] == 2).any() else np.zeros(0, np.uint8)
+                    # messages of non-Ed25519 items are empty, so the Ed25519 messages concatenate in order
+                    arrivals = np.cumsum(rng.exponential(1e9 * mean_req / rate_items, nreq)).astype(np.uint64)
+                    if load == "saturated":
+                        arrivals[:] = 0
+                    r_sch, r_sigs, r_aux = h_sch[idx], h_sigs[idx], h_aux[idx]
+                    lat, s_out, s_st = srv.replay(r_sch, r_sigs, r_aux, g_msgs, g_off, req_off, arrivals)
+                    # on the replay's own clock: from the first arrival to the last completion
+                    span_s = float((arrivals.astype(np.float64) + lat.astype(np.float64)).max()) / 1e9
+                    same = bool(np.array_equal(s_out, out_h[idx]) and np.array_equal(s_st, st_h[idx]))
+                    lat_ms = lat.astype(np.float64) / 1e6
+                    if warm:
+                        warm = False
+                        continue
+                    served.append({"server": server, "load": load, "capacity_basis": "batched incl. transfers",
+                                   "offered_sigs_per_s": None if load == "saturated" else round(rate_items, 1),
+                                   "latency_from": "scheduled arrival", "requests": nreq,
+                                   "items": int(req_off[-1]), "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                                   "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                                   "sustained_sigs_per_s": round(float(req_off[-1]) / span_s, 1),
+                                   "results_identical_to_batch": same})
+                    log(f"served ({server}) load {load}: {nreq} requests, p50 {served[-1]['p50_ms']} ms, "
+                        f"p99 {served[-1]['p99_ms']} ms, {served[-1]['sustained_sigs_per_s'] / 1e6:.2f} M sigs/s "
+                        f"sustained, identical: {same}")
 
     # roofline of the dominant kernel (the persistent verification kernel): its VALU use per launch
     # from the counter profile of this build and launch shape, over its duration measured here

@@ -250,18 +250,33 @@ int sigops_batch_stage_ms(sigops_ctx* ctx, float* ms);
 int sigops_set_fault(sigops_ctx* ctx, uint32_t flags);
 
 /* ---- serving: continuous batching of verification requests. No counterpart in the reference.
-   Requests (each a small mixed batch in the sigops_batch layout) are queued on submit and packed,
-   in arrival order, into the next free slot (a context with its own stream and scratch) while the
-   batch stays within max_batch items and all batches in flight within the HBM budget; arrivals
-   during a launch go out with the next batch, and consecutive batches overlap on the device.
+   Requests (each a small mixed batch in the sigops_batch layout) are queued on submit and leave the
+   queue in arrival order while the bytes of the requests in flight stay within the HBM budget.
+   Two modes:
+   SIGOPS_SERVE_SLOTS: requests are packed into the next free slot (a context with its own stream
+     and scratch) while the batch stays within max_batch items; arrivals during a launch go out
+     with the next batch, and consecutive batches overlap on the device.
+   SIGOPS_SERVE_RING: one persistent launch drains a ring of tile positions in pinned,
+     device-mapped memory: the items of admitted requests are packed, by bucket, into tiles of up
+     to 64 items that mix requests, and published to the running launch, whose wavefronts claim
+     tiles from a device counter; full tiles go out at once, partial ones when wavefronts would
+     otherwise idle. A launch ends after idle_close_us without work (or when a tile needs the
+     careful pass, which runs after it) and the next request starts another.
    Thread-safe: any thread may submit or wait. ---- */
+#define SIGOPS_SERVE_SLOTS 0
+#define SIGOPS_SERVE_RING 1
 typedef struct sigops_server sigops_server;
 typedef struct {
   int device;
-  uint32_t slots;          /* batches in flight at once (0: 2) */
-  uint64_t hbm_budget;     /* device bytes for the request data of all batches in flight (0: 1 GiB) */
-  uint32_t max_batch;      /* items per launched batch (0: 1 << 20) */
-  uint32_t blocks_per_cu;  /* per-slot context setting (0: default) */
+  uint32_t slots;          /* slot mode: batches in flight at once (0: 2) */
+  uint64_t hbm_budget;     /* bytes of the requests in flight (0: 1 GiB) */
+  uint32_t max_batch;      /* items per request, and per launched batch in slot mode (0: 1 << 20) */
+  uint32_t blocks_per_cu;  /* slot mode: per-slot context setting (0: default) */
+  uint32_t mode;           /* SIGOPS_SERVE_SLOTS (0) or SIGOPS_SERVE_RING */
+  uint32_t ring_tiles;     /* ring mode: tile positions, a power of two in 64 .. 2^20 (0: 4096) */
+  uint64_t ring_msg_bytes; /* ring mode: message ring bytes, pinned and in HBM (0: 64 MiB); a request
+                              whose longest message exceeds a quarter of it is refused */
+  uint32_t idle_close_us;  /* ring mode: a launch with no work for this long ends (0: 1000) */
 } sigops_server_config;
 
 sigops_server* sigops_server_create(const sigops_server_config* cfg);
@@ -276,7 +291,7 @@ int64_t sigops_server_submit(sigops_server* srv, const uint8_t* schemes, const u
    Returns 0, 1 on timeout (the ticket stays valid), or a negative SIGOPS_E_* code. */
 int sigops_server_wait(sigops_server* srv, int64_t ticket, uint8_t* out, uint8_t* status, int timeout_ms,
                        uint64_t* t_submit_ns, uint64_t* t_done_ns);
-/* batches launched and items processed so far */
+/* batches launched (ring mode: persistent launches) and items processed so far */
 int sigops_server_stats(sigops_server* srv, uint64_t* batches, uint64_t* items);
 /* Load replay for latency measurement: native clients (4 threads, request r on thread r mod 4)
    submit request r (items req_off[r] .. req_off[r+1] of the given arrays, messages through the

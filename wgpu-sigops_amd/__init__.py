@@ -45,7 +45,13 @@ class _Config(ctypes.Structure):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
-                ("max_batch", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32)]
+                ("max_batch", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("mode", ctypes.c_uint32),
+                ("ring_tiles", ctypes.c_uint32), ("ring_msg_bytes", ctypes.c_uint64),
+                ("idle_close_us", ctypes.c_uint32)]
+
+
+SERVE_SLOTS = 0
+SERVE_RING = 1
 
 
 _lib = None
@@ -454,10 +460,14 @@ class Context:
 
 class Server:
     """Continuous-batching request server (sigops_server_*): submit() returns a ticket at once,
-    wait() returns (out, status, t_submit_ns, t_done_ns)."""
+    wait() returns (out, status, t_submit_ns, t_done_ns). mode SERVE_SLOTS packs requests into
+    batches of per-slot contexts; SERVE_RING feeds them, tile by tile, to one running persistent
+    launch (ring_tiles positions, ring_msg_bytes of messages, ended after idle_close_us idle)."""
 
-    def __init__(self, device=0, slots=0, hbm_budget=0, max_batch=0, blocks_per_cu=0):
-        cfg = _ServerConfig(device, slots, hbm_budget, max_batch, blocks_per_cu)
+    def __init__(self, device=0, slots=0, hbm_budget=0, max_batch=0, blocks_per_cu=0, mode=SERVE_SLOTS,
+                 ring_tiles=0, ring_msg_bytes=0, idle_close_us=0):
+        cfg = _ServerConfig(device, slots, hbm_budget, max_batch, blocks_per_cu, mode, ring_tiles, ring_msg_bytes,
+                            idle_close_us)
         self._h = lib().sigops_server_create(ctypes.byref(cfg))
         if not self._h:
             raise SigopsError("sigops_server_create failed")

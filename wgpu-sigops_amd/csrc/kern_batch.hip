@@ -342,37 +342,46 @@ struct StreamArgs {
   uint4* dmsgs;
   uint4* out;
   u8* st;
-  u32* tile_done;
+  u64* tile_done;
   const u64* ready;
   const u32* stop;
   u32* err;
   u32* ctr;
   u32* pend_list;
-  u32 tpc, slots, seq;
+  u32 pend_cap, slots, base, mask, seq;
 };
 
 // Polls between sleeps of 4 x 127 x 64 cycles (about 13 us); 2^20 polls are about 15 s, far beyond
 // any packing delay, so a wave gives up only when the host stopped feeding the launch
 constexpr u32 STREAM_SPIN_LIMIT = 1u << 20;
 
-// Waits until the host has published chunk c of this batch (seq); false when it gave up (the
-// host's stop flag, or the spin limit), after marking the launch failed. `known` is the count of
-// published chunks this wavefront last read: the pinned word is read only for a chunk beyond it
-// (the host usually packs well ahead of the kernel, so most tiles need no PCIe round trip). One
-// lane polls; the wavefront follows its answer.
-DEV bool stream_wait(const StreamArgs& a, u32 c, u32& known) {
-  if (c < known) return true;
-  u32 ok = 1, seen = known;
+// Waits until the host has published tile t of this launch (seq): 1 when it is, 0 when the launch
+// is over before it (the count was closed below t+1, or the host moved on to a later launch), -1
+// when it gave up (the host's stop flag, or the spin limit) after marking the launch failed.
+// `known` is the published count this wavefront last read: the pinned word is read only for a
+// tile beyond it (the host usually packs well ahead of the kernel, so most tiles need no PCIe
+// round trip). One lane polls; the wavefront follows its answer. The acquire is at system scope:
+// a ring position is rewritten by the host while the launch runs, so no cache may keep the line
+// that an earlier tile at the position left there.
+DEV int stream_wait(const StreamArgs& a, u32 t, u32& known) {
+  if (t < known) return 1;
+  int res = 1;
+  u32 seen = known;
   if ((threadIdx.x & 63u) == 0) {
     for (u32 spin = 0;; spin++) {
       const u64 r = __hip_atomic_load(a.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      if ((u32)(r >> 32) == a.seq && (u32)r > c) {
-        seen = (u32)r;
+      const u32 cnt = (u32)r & ~STREAM_CLOSED;
+      if ((u32)(r >> 32) != a.seq || ((u32)r & STREAM_CLOSED && cnt <= t)) {
+        res = 0;
+        break;
+      }
+      if (cnt > t) {
+        seen = cnt;
         break;
       }
       if (spin >= STREAM_SPIN_LIMIT || __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
         __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        ok = 0;
+        res = -1;
         break;
       }
       __builtin_amdgcn_s_sleep(127);
@@ -381,11 +390,10 @@ DEV bool stream_wait(const StreamArgs& a, u32 c, u32& known) {
       __builtin_amdgcn_s_sleep(127);
     }
   }
-  ok = __shfl(ok, 0);
+  res = __shfl(res, 0);
   known = __shfl(seen, 0);
-  // the tile's reads come after the published count (the pinned buffers are not cached)
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  return ok != 0;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+  return res;
 }
 
 // inputs of lane l of slot t from the tile layout: 16 signature words, 8 digest / key words
@@ -409,11 +417,12 @@ DEV void stream_store_out(const StreamArgs& a, u32 t, u32 l, const u32 o[16]) {
   for (int c = 0; c < 4; c++) out[c * 64] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
 }
 
-// Persistent: each wavefront takes the next tile slot, waits until its chunk is published, and
-// processes the tile's items (one per lane) straight from pinned memory; an Ed25519 tile first
-// copies its messages into HBM with 1 KB-wide loads (the hash reads them word by word), and the
-// results go back to pinned memory, after which the tile's done word is set. Secp items whose fast
-// ladder meets an exceptional addition go to the pending list for k_stream_careful.
+// Persistent: each wavefront takes the next tile, waits until the host has published it, and
+// processes its items (one per lane) straight from pinned memory; an Ed25519 tile first copies its
+// messages into HBM with 1 KB-wide loads (the hash reads them word by word), and the results go
+// back to pinned memory, after which the tile's done word is set. Secp items whose fast ladder
+// meets an exceptional addition go to the pending list for k_stream_careful. The launch ends when
+// its tiles run out: the batch's last slot, or the count the host closed.
 __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamArgs a, const uint4* __restrict__ gk1,
                                                                     const uint4* __restrict__ gr1,
                                                                     const uint4* __restrict__ gb,
@@ -435,17 +444,17 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
   ct.base = lane_tab;
   ct.slots = lane_slots;
   ct.slot = slot;
-  u32 known = 0;  // chunks this wavefront knows to be published
+  u32 known = 0;  // tiles this wavefront knows to be published
   for (;;) {
     u32 t = 0;
     if (lane == 0) t = atomicAdd(&a.ctr[0], 1u);
     t = __shfl(t, 0);
-    if (t >= a.slots) break;
-    if (!stream_wait(a, t / a.tpc, known)) break;
-    const uint4 d = a.desc[t];
+    if (t >= a.slots || stream_wait(a, t, known) <= 0) break;
+    const u32 p = (a.base + t) & a.mask;  // the tile's position in the layout
+    const uint4 d = a.desc[p];
     const u32 info = __builtin_amdgcn_readfirstlane(d.x);
     const u32 cnt = info & 0xFFu, b = info >> 8;
-    if (cnt == 0) continue;  // an unused slot of its chunk
+    if (cnt == 0) continue;  // an unused position
     if (b >= BUCKET_ED0) {
       // the tile's messages, [m0, m0 + mbytes) rounded up to 16 bytes, pinned -> HBM
       const u64 m0 = ((u64)d.w << 32) | d.z;
@@ -461,40 +470,45 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
         dst[q + 192] = v3;
       }
       for (; q < nq; q += 64) dst[q] = src[q];
-      // the wave's copy is complete and visible before its lanes hash the messages
+      // the wave's copy is complete and visible before its lanes hash the messages; the release
+      // also writes the lines back, so no L2 keeps a dirty copy that could later overwrite the
+      // region another tile at these offsets writes
       __threadfence();
       if (lane < cnt) {
         u32 sr[16], ar[8];
-        stream_load(a, t, lane, sr, ar);
+        stream_load(a, p, lane, sr, ar);
         const u32 st = ed25519_verify_item<false>(sr, ar, MemDev{(const u32*)a.dmsgs},
-                                                  m0 + a.mrel[(u64)t * 64 + lane], a.mlen[(u64)t * 64 + lane], ct,
+                                                  m0 + a.mrel[(u64)p * 64 + lane], a.mlen[(u64)p * 64 + lane], ct,
                                                   BTabLds{s_b});
-        a.st[(u64)t * 64 + lane] = (u8)st;
+        a.st[(u64)p * 64 + lane] = (u8)st;
       }
     } else if (lane < cnt) {
       u32 sr[16], ar[8], o[16];
-      stream_load(a, t, lane, sr, ar);
+      stream_load(a, p, lane, sr, ar);
       const u32 st = b == BUCKET_K1 ? secp_item<CurveK1>(sr, ar, jt, GTabOf<CurveK1>{s_k1}, o)
                                     : secp_item<CurveR1>(sr, ar, jt, GTabOf<CurveR1>{s_r1}, o);
-      if (st == ST_PENDING) a.pend_list[atomicAdd(&a.ctr[1], 1u)] = t * 64 + lane;
-      stream_store_out(a, t, lane, o);
-      a.st[(u64)t * 64 + lane] = (u8)st;
+      if (st == ST_PENDING) {
+        const u32 k = atomicAdd(&a.ctr[1], 1u);
+        if (k < a.pend_cap) a.pend_list[k] = p * 64 + lane;
+      }
+      stream_store_out(a, p, lane, o);
+      a.st[(u64)p * 64 + lane] = (u8)st;
     }
     // every lane's results reach host memory before the tile is marked done: the stores into the
     // pinned buffers may sit in L2, so the release is at system scope (waiting for the stores
     // alone left results the host could not yet see)
     __threadfence_system();
-    if (lane == 0) __hip_atomic_store(&a.tile_done[t], a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (lane == 0) __hip_atomic_store(&a.tile_done[p], ((u64)a.seq << 32) | t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
-// the careful pass of the streamed batch: pending slot lanes, inputs and results in the tile layout
+// the careful pass of a streamed launch: pending position lanes, inputs and results in the tile layout
 __global__ __launch_bounds__(64) void k_stream_careful(StreamArgs a, const uint4* __restrict__ gk1,
                                                        const uint4* __restrict__ gr1) {
-  const u32 cnt = a.ctr[1];
+  const u32 cnt = min(a.ctr[1], a.pend_cap);
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
     const u32 p = a.pend_list[j];
-    const u32 t = p >> 6, l = p & 63u;
+    const u32 t = p >> 6, l = p & 63u;  // position and lane
     const u32 b = a.desc[t].x >> 8;
     u32 sr[16], ar[8], o[16];
     stream_load(a, t, l, sr, ar);
@@ -534,8 +548,10 @@ hipError_t launch_stream(const StreamLaunch& L, hipStream_t s) {
   a.err = L.err;
   a.ctr = L.ctr;
   a.pend_list = L.pend_list;
-  a.tpc = L.tpc;
+  a.pend_cap = L.pend_cap;
   a.slots = L.slots;
+  a.base = L.base;
+  a.mask = L.mask;
   a.seq = L.seq;
   k_zero_u32<<<1, 64, 0, s>>>(L.ctr, 2);
   k_stream_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,

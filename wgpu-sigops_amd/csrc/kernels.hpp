@@ -70,30 +70,48 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
 // strict: ed25519-dalek verify_strict rules; L.gb is the fixed-base table
 hipError_t launch_ed_verify(const BatchLaunch& L, bool strict, hipStream_t s);
 
-// ---- streamed batch: the host packs chunks of the batch into pinned, device-mapped memory while
-// one persistent launch drains them (layout in stream_host.hpp) ----
+// ---- streamed tiles: the host packs tiles into pinned, device-mapped memory while one persistent
+// launch drains them (layout in stream_host.hpp). A streamed batch fills positions 0.. of the
+// layout once; a serving ring reuses positions base + t (mod positions) for the launch's tile t ----
+constexpr uint32_t STREAM_CLOSED = 1u << 31;  // in the published count: no tile beyond it will come
 struct StreamLaunch {
-  const void* desc;      // pinned: 16-byte tile descriptors, tpc per chunk
-  const void* in;        // pinned: inputs, [slot][6][64] 16-byte chunks
-  const uint32_t* mrel;  // pinned: per slot lane, message start relative to the tile's
-  const uint32_t* mlen;  // pinned: per slot lane, message length
+  const void* desc;      // pinned: 16-byte tile descriptors, one per position
+  const void* in;        // pinned: inputs, [position][6][64] 16-byte chunks
+  const uint32_t* mrel;  // pinned: per position lane, message start relative to the tile's
+  const uint32_t* mlen;  // pinned: per position lane, message length
   const void* hmsgs;     // pinned: the tiles' messages
   void* dmsgs;           // device: the messages copied by their tiles (same offsets), 16 bytes of slack
-  void* out;             // pinned: results, [slot][4][64] 16-byte chunks
-  uint8_t* st;           // pinned: statuses, [slot][64]
-  uint32_t* tile_done;   // pinned: per slot, set to seq once the tile's results are written
-  const uint64_t* ready; // pinned: seq << 32 | chunks published so far
+  void* out;             // pinned: results, [position][4][64] 16-byte chunks
+  uint8_t* st;           // pinned: statuses, [position][64]
+  uint64_t* tile_done;   // pinned: per position, seq << 32 | t once tile t's results are written
+  const uint64_t* ready; // pinned: seq << 32 | tiles published so far, | STREAM_CLOSED at the end
   const uint32_t* stop;  // pinned: nonzero makes waiting waves give up
   uint32_t* err;         // pinned: set to 1 by a wave that gave up waiting
-  uint32_t* ctr;         // device: [0] next tile slot, [1] pending count (cleared by the launch)
-  uint32_t* pend_list;   // device: slot lanes (slot * 64 + lane) the careful pass redoes
+  uint32_t* ctr;         // device: [0] next tile, [1] pending count (cleared by the launch)
+  uint32_t* pend_list;   // device: position lanes (position * 64 + lane) the careful pass redoes
+  uint32_t pend_cap;     // entries pend_list holds (further pending lanes are not recorded)
   const void* gk1;
   const void* gr1;
   const void* gb;
   void* lane_tab;
-  uint32_t tpc, slots, seq, lane_slots, grid;
+  uint32_t slots;        // tiles this launch may take at most (the ring: 2^31)
+  uint32_t base, mask;   // tile t sits at position (base + t) & mask (a batch: 0 and ~0)
+  uint32_t seq, lane_slots, grid;
 };
 hipError_t launch_stream(const StreamLaunch& L, hipStream_t s);
+
+// what a serving ring borrows from a context: the resident tables, the lane tables and the launch
+// shape they are sized for, the processing order of the buckets, and the context's stream
+struct RingResources {
+  const void* gk1;
+  const void* gr1;
+  const void* gb;
+  void* lane_tab;
+  uint32_t lane_slots, grid;
+  const uint32_t* bucket_order;  // processing rank -> bucket, NBUCKETS entries
+  hipStream_t stream;
+};
+
 
 // SHA-256: digests of messages (Fuel Message::new) and addresses of 64-byte public keys.
 // work: MSG_WORK_WORDS device words (cleared by the launch); order: n words, the longest-first
@@ -117,3 +135,9 @@ hipError_t launch_gen_ed25519(uint64_t seed, uint32_t n, uint32_t ppm, const voi
                               void* msgs, void* sigs, void* pks, void* corrupted, uint32_t grid, hipStream_t s);
 
 }  // namespace sigops
+
+// the resources of a context that a serving ring launches with (sigops_api.hip); SIGOPS_OK or an
+// error code
+struct sigops_ctx;
+int sigops_ring_resources(sigops_ctx* c, sigops::RingResources* r);
+

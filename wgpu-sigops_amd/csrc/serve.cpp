@@ -1,17 +1,29 @@
-// serve.cpp -- continuous-batching request server over the mixed-batch engine.
+// serve.cpp -- request servers over the mixed-batch engine.
 //
 // The reference has no serving layer: every call builds a device, compiles, runs one batch and
-// destroys the device (src/gpu.rs:5-35,167). Here requests of any size arrive while others run:
-//   * submit() copies a request into a host queue and returns a ticket at once;
-//   * the dispatcher packs queued requests FIFO into the next free slot (one sigops_ctx with its own
-//     stream, scratch, device buffers and captured graph), as long as the batch stays within
-//     max_batch items and the device bytes of all batches in flight stay within the HBM budget;
-//     it launches as soon as a slot is free, so arrivals during a launch go out with the next one
-//     and consecutive batches overlap on the device;
-//   * the completion thread polls the slots' events, so batches may finish out of order, scatters
-//     results back to their requests, timestamps them and frees the slot and the budget.
-// Every item is processed by the same kernels as a standalone batch, so a request's results do not
-// depend on what it was batched with.
+// destroys the device (src/gpu.rs:5-35,167). Here requests of any size arrive while others run.
+// submit() copies a request into a host queue and returns a ticket at once; requests leave the
+// queue in arrival order while the bytes of the requests in flight stay within the HBM budget.
+// Two ways to run them (sigops_server_config.mode):
+//   * slot batches: the dispatcher packs queued requests FIFO into the next free slot (one
+//     sigops_ctx with its own stream, scratch, device buffers and captured graph), as long as the
+//     batch stays within max_batch items; it launches as soon as a slot is free, so arrivals during
+//     a launch go out with the next one and consecutive batches overlap on the device. The
+//     completion thread polls the slots' events, scatters results back to their requests and
+//     frees the slot.
+//   * the device-drained ring: one persistent launch (an epoch) drains a ring of tile positions
+//     in pinned, device-mapped memory. The dispatcher sorts the items of admitted requests into
+//     per-bucket lists and packs them into ring positions as tiles of 64 items of one bucket, so
+//     a tile mixes items of several requests; full tiles go out at once, partial ones when some of
+//     the launch's wavefronts would otherwise idle. Each publication is one store of the tile
+//     count; the kernel's wavefronts claim tiles from a device counter, so a request that arrives
+//     while the GPU is busy joins the running launch at the next free wavefront instead of the
+//     next batch. The completion thread polls the tiles' done words, copies results to their
+//     requests, and frees positions in order. An epoch ends (its count is closed) when the ring
+//     has been idle for idle_close_us, or when a tile holds a lane that needs the careful pass,
+//     which runs after the epoch's persistent kernel; the next epoch is launched behind it.
+// Either way every item is processed by the same per-item code as a standalone batch, so a
+// request's results do not depend on what it was batched with.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +41,8 @@
 
 #include "../../include/sigops.h"
 #include "host_xfer.hpp"
+#include "kernels.hpp"
+#include "stream_host.hpp"
 
 namespace {
 
@@ -45,6 +59,7 @@ struct Request {
   std::vector<uint64_t> off;  // n + 1, starting at 0
   uint64_t t_submit = 0, t_done = 0;
   uint64_t bytes = 0;
+  size_t remaining = 0;  // ring: items whose results are not in yet (completion thread only)
   bool done = false;
   int err = 0;
 };
@@ -105,6 +120,8 @@ struct Slot {
   size_t items = 0;
 };
 
+struct RingEngine;
+
 }  // namespace
 
 struct sigops_server {
@@ -125,6 +142,8 @@ struct sigops_server {
   std::unique_ptr<sigops_host::Pool> unpack_pool;  // used by the completion thread only
   std::atomic<uint64_t> n_batches{0}, n_items{0};
   int fatal = 0;
+  std::unique_ptr<RingEngine> ring;  // mode SIGOPS_SERVE_RING
+  bool ring_wake = false;            // the ring made progress: the dispatcher looks again
 
   void dispatch_loop();
   void complete_loop();
@@ -321,6 +340,533 @@ void sigops_server::complete_loop() {
   }
 }
 
+namespace {
+
+using sigops::NBUCKETS;
+using sigops_host::StreamBufs;
+using sigops_host::StreamDesc;
+using sigops_host::StreamItem;
+using sigops_host::ST_LANES;
+
+constexpr uint8_t ST_PENDING = 0xFF;  // a lane whose item waits for the careful pass
+
+struct RingItem {
+  Request* r;
+  uint32_t idx;
+};
+
+// host bookkeeping of one ring position
+struct RingPos {
+  uint64_t tag = 0;        // the done word the kernel writes for the tile held here
+  uint64_t mend = 0;       // message ring: virtual end of the tile's region
+  uint32_t cnt = 0;
+  bool ed = false;
+  bool seen = false;       // its results were read
+  bool held = false;       // lanes wait for the epoch's careful pass; the position stays taken
+  uint64_t pend_mask = 0;  // those lanes
+  RingItem items[ST_LANES];
+};
+
+// one persistent launch: tiles [g0, g_end) of the ring's global tile numbering
+struct Epoch {
+  uint32_t seq = 0;
+  uint64_t g0 = 0, g_end = UINT64_MAX;
+  bool closed = false;
+  bool failed = false;  // the launch was not made: its tiles are lost
+  hipEvent_t ev = nullptr;
+};
+
+// regions of the ring's pinned block, each 256-byte aligned (positions' layout as in stream_host.hpp)
+struct RingRegions {
+  size_t desc, in, mrel, mlen, out, st, done, msgs, total;
+};
+
+RingRegions ring_regions(size_t R, uint64_t MB) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  RingRegions r;
+  size_t at = 0;
+  r.desc = at;
+  at = al(at + R * 16);
+  r.in = at;
+  at = al(at + R * sigops_host::ST_IN_CHUNKS * ST_LANES * 16);
+  r.mrel = at;
+  at = al(at + R * ST_LANES * 4);
+  r.mlen = at;
+  at = al(at + R * ST_LANES * 4);
+  r.out = at;
+  at = al(at + R * sigops_host::ST_OUT_CHUNKS * ST_LANES * 16);
+  r.st = at;
+  at = al(at + R * ST_LANES);
+  r.done = at;
+  at = al(at + R * 8);
+  r.msgs = at;
+  at = al(at + MB + 256);
+  r.total = at;
+  return r;
+}
+
+void* dev_addr(void* host) {
+  void* d = nullptr;
+  return hipHostGetDevicePointer(&d, host, 0) == hipSuccess ? d : nullptr;
+}
+
+struct RingEngine {
+  sigops_server* S = nullptr;
+  sigops_ctx* ctx = nullptr;  // the resident tables, the lane tables and the stream
+  sigops::RingResources res{};
+  uint32_t R = 0, mask = 0, W = 0;  // positions, positions - 1, wavefronts of a launch
+  uint64_t MB = 0;                  // message ring bytes
+  uint64_t idle_ns = 0;
+  RingRegions reg{};
+  uint8_t* pin = nullptr;  // positions' regions and the message ring
+  uint8_t* ctl = nullptr;  // READY_SLOTS u64 published counts (epoch seq % READY_SLOTS), u32 stop, u32 err
+  uint8_t* dpin = nullptr;
+  uint8_t* dctl = nullptr;
+  uint8_t* dmsgs = nullptr;  // HBM copies of the tiles' messages (same offsets)
+  uint32_t* dctr = nullptr;
+  uint32_t* dpend = nullptr;
+  StreamBufs B{};
+  uint64_t* done = nullptr;
+  std::vector<RingPos> pos;
+  std::vector<hipEvent_t> free_ev;
+  // dispatcher only
+  std::vector<RingItem> open[NBUCKETS];  // admitted items not yet in a tile
+  uint64_t m_head = 0;
+  uint32_t seq = 0;
+  bool ep_open = false, ep_launched = false;
+  uint64_t ep_g0 = 0, ep_count = 0;
+  uint64_t last_busy = 0;
+  // shared between the dispatcher and the completion thread
+  std::atomic<uint64_t> g_next{0}, g_reclaimed{0}, n_seen{0}, m_tail{0};
+  std::atomic<uint32_t> close_seq{0};  // an epoch with lanes waiting for its careful pass
+  std::atomic<bool> disp_done{false};
+  std::mutex emu;  // epochs, free_ev
+  std::deque<Epoch> epochs;
+  std::thread disp, comp;
+
+  int init(sigops_server* srv);
+  void release();
+  void admit(Request* r);
+  bool pack(uint32_t q, const RingItem* items, uint32_t c);
+  void publish();
+  void open_epoch();
+  void publish_count();
+  void close_epoch();
+  void dispatch_loop();
+  void read_tile(RingPos& P, uint32_t p, std::vector<Request*>& finished);
+  void finish_lane(RingPos& P, uint32_t p, uint32_t l, std::vector<Request*>& finished, int err);
+  void complete_loop();
+  // Each epoch publishes through its own word (seq % READY_SLOTS): a closed epoch's wavefronts may
+  // still be reading its count while the next epoch publishes, and at most READY_SLOTS - 1 epochs
+  // are outstanding at once
+  static constexpr uint32_t READY_SLOTS = 16;
+  uint64_t* ready() { return (uint64_t*)ctl + seq % READY_SLOTS; }
+};
+
+int RingEngine::init(sigops_server* srv) {
+  S = srv;
+  const sigops_server_config& cfg = S->cfg;
+  R = cfg.ring_tiles;
+  mask = R - 1;
+  MB = cfg.ring_msg_bytes;
+  idle_ns = (uint64_t)cfg.idle_close_us * 1000;
+  sigops_config cc{};
+  cc.device = cfg.device;
+  cc.max_batch = 1u << 16;  // the context's own batch buffers are not used by the ring
+  ctx = sigops_create(&cc);
+  if (!ctx) return SIGOPS_E_DEVICE;
+  if (sigops_ring_resources(ctx, &res)) return SIGOPS_E_DEVICE;
+  W = res.grid * 4;
+  reg = ring_regions(R, MB);
+  if (hipHostMalloc((void**)&pin, reg.total, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    pin = nullptr;
+    return SIGOPS_E_NOMEM;
+  }
+  if (hipHostMalloc((void**)&ctl, READY_SLOTS * 8 + 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    ctl = nullptr;
+    return SIGOPS_E_NOMEM;
+  }
+  memset(pin, 0, reg.total);
+  memset(ctl, 0, READY_SLOTS * 8 + 64);
+  dpin = (uint8_t*)dev_addr(pin);
+  dctl = (uint8_t*)dev_addr(ctl);
+  if (!dpin || !dctl) return SIGOPS_E_DEVICE;
+  if (hipMalloc((void**)&dmsgs, MB + 256) != hipSuccess || hipMalloc((void**)&dctr, 2 * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&dpend, (size_t)R * ST_LANES * sizeof(uint32_t)) != hipSuccess)
+    return SIGOPS_E_NOMEM;
+  B = StreamBufs{(StreamDesc*)(pin + reg.desc), pin + reg.in, (uint32_t*)(pin + reg.mrel), (uint32_t*)(pin + reg.mlen),
+                 pin + reg.msgs, pin + reg.out, pin + reg.st, nullptr};
+  done = (uint64_t*)(pin + reg.done);
+  pos.resize(R);
+  for (int k = 0; k < 4; k++) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SIGOPS_E_DEVICE;
+    free_ev.push_back(e);
+  }
+  disp = std::thread([this] { dispatch_loop(); });
+  comp = std::thread([this] { complete_loop(); });
+  return SIGOPS_OK;
+}
+
+void RingEngine::release() {
+  if (disp.joinable()) disp.join();
+  if (comp.joinable()) comp.join();
+  if (res.stream) (void)hipStreamSynchronize(res.stream);
+  for (auto& e : epochs)
+    if (e.ev) free_ev.push_back(e.ev);
+  for (hipEvent_t e : free_ev) (void)hipEventDestroy(e);
+  free_ev.clear();
+  if (dmsgs) (void)hipFree(dmsgs);
+  if (dctr) (void)hipFree(dctr);
+  if (dpend) (void)hipFree(dpend);
+  if (pin) (void)hipHostFree(pin);
+  if (ctl) (void)hipHostFree(ctl);
+  if (ctx) sigops_destroy(ctx);
+  dmsgs = nullptr;
+  dctr = dpend = nullptr;
+  pin = ctl = nullptr;
+  ctx = nullptr;
+}
+
+// a request's items into the bucket lists
+void RingEngine::admit(Request* r) {
+  r->remaining = r->n;
+  r->out.assign(r->n * 64, 0);
+  r->status.assign(r->n, 0);
+  for (size_t i = 0; i < r->n; i++) {
+    const uint32_t q = sigops_host::stream_bucket(r->schemes[i], r->off[i + 1] - r->off[i]);
+    open[q].push_back(RingItem{r, (uint32_t)i});
+  }
+}
+
+// packs c items of bucket q into the next ring position; false when no position or no message
+// space is free (the items stay where they are)
+bool RingEngine::pack(uint32_t q, const RingItem* items, uint32_t c) {
+  const uint64_t g = g_next.load(std::memory_order_relaxed);
+  if (g - g_reclaimed.load(std::memory_order_acquire) >= R) return false;
+  uint64_t bytes = 0;
+  for (uint32_t l = 0; l < c; l++) bytes += items[l].r->off[items[l].idx + 1] - items[l].r->off[items[l].idx];
+  const uint64_t need = sigops_host::stream_region_bytes(bytes);
+  uint64_t start = m_head;
+  if (start % MB + need > MB) start = (start / MB + 1) * MB;  // a region never wraps
+  if (start + need - m_tail.load(std::memory_order_acquire) > MB) return false;
+  if (!ep_open) {
+    std::lock_guard<std::mutex> lk(emu);
+    if (epochs.size() >= READY_SLOTS - 1) return false;  // the completion thread retires them first
+  }
+  if (!ep_open) open_epoch();
+  const uint32_t p = (uint32_t)(g & mask);
+  RingPos& P = pos[p];
+  sigops_host::stream_pack_tile(B, p, q, c, start % MB, [&](uint32_t l) {
+    const Request& r = *items[l].r;
+    const uint32_t i = items[l].idx;
+    return StreamItem{r.sigs.data() + (size_t)i * 64, r.aux.data() + (size_t)i * 32,
+                      r.msgs.data() + r.off[i], r.off[i + 1] - r.off[i]};
+  });
+  for (uint32_t l = 0; l < c; l++) P.items[l] = items[l];
+  P.cnt = c;
+  P.ed = q >= sigops::BUCKET_ED0;
+  P.seen = P.held = false;
+  P.pend_mask = 0;
+  P.mend = start + need;
+  P.tag = ((uint64_t)seq << 32) | ep_count;
+  ep_count++;
+  m_head = start + need;
+  S->n_items += c;
+  g_next.store(g + 1, std::memory_order_release);  // the completion thread reads P after this
+  return true;
+}
+
+void RingEngine::open_epoch() {
+  seq = seq + 1 ? seq + 1 : 1;  // never 0, the cleared done words
+  ep_open = true;
+  ep_launched = false;
+  ep_g0 = g_next.load(std::memory_order_relaxed);
+  ep_count = 0;
+}
+
+// publishes the epoch's packed tiles, launching the epoch with its first ones
+void RingEngine::publish_count() {
+  __atomic_store_n(ready(), ((uint64_t)seq << 32) | ep_count, __ATOMIC_RELEASE);
+  if (ep_launched) return;
+  ep_launched = true;
+  Epoch e;
+  e.seq = seq;
+  e.g0 = ep_g0;
+  {
+    std::lock_guard<std::mutex> g(emu);
+    if (!free_ev.empty()) {
+      e.ev = free_ev.back();
+      free_ev.pop_back();
+    }
+  }
+  if (!e.ev && hipEventCreateWithFlags(&e.ev, hipEventDisableTiming) != hipSuccess) e.ev = nullptr;
+  sigops::StreamLaunch L{};
+  L.desc = dpin + reg.desc;
+  L.in = dpin + reg.in;
+  L.mrel = (const uint32_t*)(dpin + reg.mrel);
+  L.mlen = (const uint32_t*)(dpin + reg.mlen);
+  L.hmsgs = dpin + reg.msgs;
+  L.dmsgs = dmsgs;
+  L.out = dpin + reg.out;
+  L.st = dpin + reg.st;
+  L.tile_done = (uint64_t*)(dpin + reg.done);
+  L.ready = (const uint64_t*)dctl + seq % READY_SLOTS;
+  L.stop = (const uint32_t*)(dctl + READY_SLOTS * 8);
+  L.err = (uint32_t*)(dctl + READY_SLOTS * 8 + 4);
+  L.ctr = dctr;
+  L.pend_list = dpend;
+  L.pend_cap = R * ST_LANES;
+  L.gk1 = res.gk1;
+  L.gr1 = res.gr1;
+  L.gb = res.gb;
+  L.lane_tab = res.lane_tab;
+  L.slots = sigops::STREAM_CLOSED - 1;
+  L.base = (uint32_t)(ep_g0 & mask);
+  L.mask = mask;
+  L.seq = seq;
+  L.lane_slots = res.lane_slots;
+  L.grid = res.grid;
+  const bool ok = e.ev && sigops::launch_stream(L, res.stream) == hipSuccess &&
+                  hipEventRecord(e.ev, res.stream) == hipSuccess;
+  if (!ok) {
+    // the tiles of this epoch will not run: the completion thread fails them
+    e.failed = true;
+    e.closed = true;
+    e.g_end = ep_g0 + ep_count;
+    ep_open = false;
+  }
+  S->n_batches++;
+  std::lock_guard<std::mutex> g(emu);
+  epochs.push_back(e);
+}
+
+void RingEngine::close_epoch() {
+  if (!ep_open) return;
+  ep_open = false;
+  if (!ep_launched) return;  // nothing was published
+  __atomic_store_n(ready(), ((uint64_t)seq << 32) | sigops::STREAM_CLOSED | ep_count, __ATOMIC_RELEASE);
+  std::lock_guard<std::mutex> g(emu);
+  for (auto& e : epochs)
+    if (e.seq == seq && !e.closed) {
+      e.g_end = ep_g0 + ep_count;
+      e.closed = true;
+    }
+}
+
+// Full tiles of every bucket go out at once; when fewer tiles are in flight than the launch has
+// wavefronts, the remainders go out too (one tile per secp bucket, the Ed25519 bins merged in
+// increasing length order, so a tile's lanes still hash similar lengths).
+void RingEngine::publish() {
+  const uint64_t before = ep_count;
+  const uint32_t before_seq = seq;
+  bool full = false;  // out of positions or message space
+  auto drain = [&](std::vector<RingItem>& v, uint32_t q, bool partial) {
+    size_t at = 0;
+    while (!full && at < v.size()) {
+      uint32_t c = (uint32_t)std::min<size_t>(ST_LANES, v.size() - at);
+      if (c < ST_LANES && !partial) break;
+      // keep a tile's messages within a quarter of the message ring
+      uint64_t bytes = 0;
+      for (uint32_t l = 0; l < c; l++) {
+        const Request& r = *v[at + l].r;
+        bytes += r.off[v[at + l].idx + 1] - r.off[v[at + l].idx];
+        if (l > 0 && bytes > MB / 4) {
+          c = l;
+          break;
+        }
+      }
+      if (!pack(q, v.data() + at, c)) {
+        full = true;
+        break;
+      }
+      at += c;
+    }
+    v.erase(v.begin(), v.begin() + (long)at);
+  };
+  for (uint32_t r = 0; r < NBUCKETS && !full; r++) {
+    const uint32_t q = res.bucket_order[r];
+    drain(open[q], q, false);
+  }
+  const uint64_t inflight = g_next.load(std::memory_order_relaxed) - n_seen.load(std::memory_order_acquire);
+  if (!full && inflight < W) {
+    drain(open[sigops::BUCKET_R1], sigops::BUCKET_R1, true);
+    drain(open[sigops::BUCKET_K1], sigops::BUCKET_K1, true);
+    std::vector<RingItem> ed;
+    uint32_t last = 0;
+    for (uint32_t b = sigops::BUCKET_ED0; b < NBUCKETS; b++) {
+      if (!open[b].empty()) last = b;
+      ed.insert(ed.end(), open[b].begin(), open[b].end());
+      open[b].clear();
+    }
+    if (!ed.empty()) {
+      drain(ed, last, true);
+      // what did not fit goes back to the bins
+      for (const RingItem& it : ed)
+        open[sigops_host::stream_bucket(2, it.r->off[it.idx + 1] - it.r->off[it.idx])].push_back(it);
+    }
+  }
+  if (seq != before_seq || ep_count != before) publish_count();
+}
+
+void RingEngine::dispatch_loop() {
+  (void)hipSetDevice(S->cfg.device);
+  std::vector<std::shared_ptr<Request>> taken;
+  for (;;) {
+    bool stopping, queue_empty;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      auto admissible = [&] {
+        return !S->queue.empty() && S->budget_used + S->queue.front()->bytes <= S->cfg.hbm_budget;
+      };
+      S->cv_work.wait_for(lk, std::chrono::microseconds(100),
+                          [&] { return S->stopping || S->ring_wake || admissible(); });
+      S->ring_wake = false;
+      while (admissible()) {
+        S->budget_used += S->queue.front()->bytes;
+        taken.push_back(S->queue.front());
+        S->queue.pop_front();
+      }
+      stopping = S->stopping;
+      queue_empty = S->queue.empty();
+    }
+    for (auto& r : taken) admit(r.get());
+    taken.clear();
+    publish();
+    bool open_items = false;
+    for (auto& v : open) open_items = open_items || !v.empty();
+    const uint64_t gn = g_next.load(std::memory_order_relaxed);
+    const bool idle = !open_items && queue_empty && n_seen.load(std::memory_order_acquire) == gn;
+    const uint64_t t = now_ns();
+    if (!idle) last_busy = t;
+    if (ep_open && (close_seq.load() == seq || ep_count >= (1u << 30) || (idle && t - last_busy >= idle_ns) ||
+                    (idle && stopping)))
+      close_epoch();
+    if (stopping && idle && !ep_open) break;
+  }
+  disp_done.store(true, std::memory_order_release);
+}
+
+void RingEngine::finish_lane(RingPos& P, uint32_t p, uint32_t l, std::vector<Request*>& finished, int err) {
+  Request& r = *P.items[l].r;
+  const uint32_t i = P.items[l].idx;
+  if (err) {
+    r.err = err;
+  } else {
+    const uint8_t st = B.st[(size_t)p * ST_LANES + l];
+    if (!P.ed) {
+      const uint8_t* o = B.out + (size_t)p * (sigops_host::ST_OUT_CHUNKS * ST_LANES * 16);
+      for (uint32_t k = 0; k < sigops_host::ST_OUT_CHUNKS; k++)
+        memcpy(r.out.data() + (size_t)i * 64 + k * 16, o + (k * ST_LANES + l) * 16, 16);
+    }
+    r.status[i] = st;
+    if (st > (P.ed ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY)) r.err = SIGOPS_E_CANARY;
+  }
+  if (--r.remaining == 0) finished.push_back(&r);
+}
+
+void RingEngine::read_tile(RingPos& P, uint32_t p, std::vector<Request*>& finished) {
+  for (uint32_t l = 0; l < P.cnt; l++) {
+    if (!P.ed && B.st[(size_t)p * ST_LANES + l] == ST_PENDING) {
+      P.pend_mask |= 1ull << l;
+      continue;
+    }
+    finish_lane(P, p, l, finished, 0);
+  }
+  if (P.pend_mask) {
+    P.held = true;
+    close_seq.store((uint32_t)(P.tag >> 32));
+  }
+  P.seen = true;
+}
+
+void RingEngine::complete_loop() {
+  (void)hipSetDevice(S->cfg.device);
+  uint64_t g_scan = 0;  // every tile below it has been read
+  std::vector<Request*> finished;
+  for (;;) {
+    bool progress = false;
+    const uint64_t gn = g_next.load(std::memory_order_acquire);
+    for (uint64_t g = g_scan; g < gn; g++) {
+      RingPos& P = pos[g & mask];
+      if (P.seen || __atomic_load_n(&done[g & mask], __ATOMIC_ACQUIRE) != P.tag) continue;
+      read_tile(P, (uint32_t)(g & mask), finished);
+      n_seen.fetch_add(1, std::memory_order_release);
+      progress = true;
+    }
+    while (g_scan < gn && pos[g_scan & mask].seen) g_scan++;
+    // epochs whose launch has ended: the careful pass has rewritten the held lanes; a tile the
+    // launch did not finish is lost (its requests fail)
+    for (;;) {
+      Epoch e;
+      {
+        std::lock_guard<std::mutex> g(emu);
+        if (epochs.empty() || !epochs.front().closed) break;
+        if (!epochs.front().failed && hipEventQuery(epochs.front().ev) == hipErrorNotReady) break;
+        e = epochs.front();
+        epochs.pop_front();
+      }
+      const bool dev_ok = !e.failed && hipEventQuery(e.ev) == hipSuccess;
+      for (uint64_t g = e.g0; g < e.g_end; g++) {
+        RingPos& P = pos[g & mask];
+        const uint32_t p = (uint32_t)(g & mask);
+        if (!P.seen && dev_ok && __atomic_load_n(&done[p], __ATOMIC_ACQUIRE) == P.tag) {
+          read_tile(P, p, finished);
+          n_seen.fetch_add(1, std::memory_order_release);
+        }
+        if (!P.seen) {
+          for (uint32_t l = 0; l < P.cnt; l++) finish_lane(P, p, l, finished, SIGOPS_E_DEVICE);
+          P.seen = true;
+          n_seen.fetch_add(1, std::memory_order_release);
+        }
+        if (P.held) {
+          for (uint32_t l = 0; l < P.cnt; l++)
+            if (P.pend_mask >> l & 1) finish_lane(P, p, l, finished, dev_ok ? 0 : SIGOPS_E_DEVICE);
+          P.held = false;
+        }
+      }
+      if (e.ev) {
+        std::lock_guard<std::mutex> g(emu);
+        free_ev.push_back(e.ev);
+      }
+      progress = true;
+    }
+    while (g_scan < gn && pos[g_scan & mask].seen) g_scan++;
+    // positions free in order; a held tile keeps its position and every later one
+    uint64_t gr = g_reclaimed.load(std::memory_order_relaxed);
+    const uint64_t gr0 = gr;
+    while (gr < g_scan && !pos[gr & mask].held) {
+      m_tail.store(pos[gr & mask].mend, std::memory_order_release);
+      gr++;
+    }
+    if (gr != gr0) {
+      g_reclaimed.store(gr, std::memory_order_release);
+      progress = true;
+    }
+    if (progress) {
+      const uint64_t t = now_ns();
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (Request* r : finished) {
+        r->t_done = t;
+        r->done = true;
+        S->budget_used -= r->bytes;
+      }
+      finished.clear();
+      S->ring_wake = true;
+      S->cv_work.notify_all();
+      S->cv_done.notify_all();
+      continue;
+    }
+    if (disp_done.load(std::memory_order_acquire)) {
+      std::lock_guard<std::mutex> g(emu);
+      if (epochs.empty() && g_scan == g_next.load() && g_reclaimed.load() == g_scan) break;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(gn == g_scan ? 50 : 10));
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 sigops_server* sigops_server_create(const sigops_server_config* cfg_in) {
@@ -329,9 +875,29 @@ sigops_server* sigops_server_create(const sigops_server_config* cfg_in) {
   if (!cfg.slots) cfg.slots = 2;
   if (!cfg.hbm_budget) cfg.hbm_budget = 1ull << 30;
   if (!cfg.max_batch) cfg.max_batch = 1u << 20;
+  if (!cfg.ring_tiles) cfg.ring_tiles = 4096;
+  if (!cfg.ring_msg_bytes) cfg.ring_msg_bytes = 64ull << 20;
+  if (!cfg.idle_close_us) cfg.idle_close_us = 1000;
+  if (cfg.mode > SIGOPS_SERVE_RING) return nullptr;
+  if (cfg.ring_tiles < 64 || cfg.ring_tiles > (1u << 20) || (cfg.ring_tiles & (cfg.ring_tiles - 1))) return nullptr;
+  if (cfg.ring_msg_bytes < (1u << 16) || cfg.ring_msg_bytes % 256) return nullptr;
   auto* S = new (std::nothrow) sigops_server();
   if (!S) return nullptr;
   S->cfg = cfg;
+  if (cfg.mode == SIGOPS_SERVE_RING) {
+    S->ring = std::make_unique<RingEngine>();
+    if (S->ring->init(S) != SIGOPS_OK) {
+      {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stopping = true;
+      }
+      S->cv_work.notify_all();
+      S->ring->release();
+      delete S;
+      return nullptr;
+    }
+    return S;
+  }
   S->pack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
   S->unpack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
   for (uint32_t k = 0; k < cfg.slots; k++) {
@@ -364,6 +930,11 @@ void sigops_server_destroy(sigops_server* S) {
   }
   S->cv_work.notify_all();
   S->cv_done.notify_all();
+  if (S->ring) {
+    S->ring->release();  // the dispatcher finishes the queued work first
+    delete S;
+    return;
+  }
   S->dispatcher.join();
   S->completer.join();
   for (auto& s : S->slots) {
@@ -390,6 +961,9 @@ int64_t sigops_server_submit(sigops_server* S, const uint8_t* schemes, const uin
     }
     const uint64_t base = msg_off[0], total = msg_off[n] - base;
     if (total && !msgs) return SIGOPS_E_ARG;
+    if (S->ring)
+      for (size_t i = 0; i < n; i++)
+        if (msg_off[i + 1] - msg_off[i] > S->cfg.ring_msg_bytes / 4) return SIGOPS_E_ARG;
     r->schemes.assign(schemes, schemes + n);
     r->sigs.assign(sigs, sigs + n * 64);
     r->aux.assign(aux, aux + n * 32);

@@ -816,7 +816,7 @@ StreamRegions stream_regions(const sigops_host::StreamGeom& g) {
   r.st = at;
   at = al(at + g.slots * 64);
   r.done = at;
-  at = al(at + g.slots * 4);
+  at = al(at + g.slots * 8);
   r.msgs = at;
   at = al(at + g.msg_base[g.nchunks] + 16);
   r.total = at;
@@ -888,7 +888,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   uint8_t* P = c->s_pin;
   StreamBufs B{(StreamDesc*)(P + R.desc), P + R.in, (uint32_t*)(P + R.mrel), (uint32_t*)(P + R.mlen), P + R.msgs,
                P + R.out, P + R.st, c->s_perm.data()};
-  uint32_t* done = (uint32_t*)(P + R.done);
+  uint64_t* done = (uint64_t*)(P + R.done);
   uint64_t* ready = (uint64_t*)c->s_ctl;
   uint32_t* stop = (uint32_t*)(c->s_ctl + 8);
   uint32_t* err = (uint32_t*)(c->s_ctl + 12);
@@ -905,7 +905,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
     packed[k].store(1, std::memory_order_release);
     std::lock_guard<std::mutex> lk(pub_mu);
     while (published < g.nchunks && packed[published].load(std::memory_order_acquire)) published++;
-    __atomic_store_n(ready, ((uint64_t)seq << 32) | published, __ATOMIC_RELEASE);
+    __atomic_store_n(ready, ((uint64_t)seq << 32) | (published * g.tpc), __ATOMIC_RELEASE);
     if (published == g.nchunks)
       t_packed.store(std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - T0).count());
   };
@@ -931,18 +931,20 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   L.dmsgs = c->msg.p;
   L.out = DP + R.out;
   L.st = DP + R.st;
-  L.tile_done = (uint32_t*)(DP + R.done);
+  L.tile_done = (uint64_t*)(DP + R.done);
   L.ready = (const uint64_t*)DC;
   L.stop = (const uint32_t*)(DC + 8);
   L.err = (uint32_t*)(DC + 12);
   L.ctr = c->d_sctr;
   L.pend_list = c->d_pend + 1;
+  L.pend_cap = (uint32_t)n;
   L.gk1 = c->d_gtab_k1;
   L.gr1 = c->d_gtab_r1;
   L.gb = c->d_btab;
   L.lane_tab = c->d_lane_tab;
-  L.tpc = (uint32_t)g.tpc;
   L.slots = (uint32_t)g.slots;
+  L.base = 0;
+  L.mask = ~0u;
   L.seq = seq;
   L.lane_slots = c->slots;
   L.grid = std::min<uint32_t>(c->grid, (uint32_t)((g.slots + 3) / 4));
@@ -962,10 +964,11 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
       const size_t t0 = k * g.tpc;
       bool ok = true;
       for (uint32_t j = 0; j < tiles[k] && ok; j++) {
-        for (unsigned spin = 1; __atomic_load_n(&done[t0 + j], __ATOMIC_ACQUIRE) != seq; spin++) {
+        const uint64_t tag = ((uint64_t)seq << 32) | (t0 + j);
+        for (unsigned spin = 1; __atomic_load_n(&done[t0 + j], __ATOMIC_ACQUIRE) != tag; spin++) {
           if (spin % 256 == 0) {
             // the launch has ended: a tile still not done will not be
-            if (hipEventQuery(c->s_ev) == hipSuccess && __atomic_load_n(&done[t0 + j], __ATOMIC_ACQUIRE) != seq) {
+            if (hipEventQuery(c->s_ev) == hipSuccess && __atomic_load_n(&done[t0 + j], __ATOMIC_ACQUIRE) != tag) {
               ok = false;
               break;
             }
@@ -995,7 +998,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   uint32_t ctr[2];
   HIPCHK(c, hipMemcpy(ctr, c->d_sctr, sizeof ctr, hipMemcpyDeviceToHost));
   if (ctr[1]) {
-    std::vector<uint32_t> pend(ctr[1]);
+    std::vector<uint32_t> pend(std::min<size_t>(ctr[1], n));
     HIPCHK(c, hipMemcpy(pend.data(), c->d_pend + 1, pend.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t p : pend) stream_unpack_lane(p, B, out, status);
   }
@@ -1004,6 +1007,19 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
 }
 
 }  // namespace
+
+int sigops_ring_resources(sigops_ctx* c, sigops::RingResources* r) {
+  if (!c || !r) return SIGOPS_E_ARG;
+  r->gk1 = c->d_gtab_k1;
+  r->gr1 = c->d_gtab_r1;
+  r->gb = c->d_btab;
+  r->lane_tab = c->d_lane_tab;
+  r->lane_slots = c->slots;
+  r->grid = c->grid;
+  r->bucket_order = c->bucket_order;
+  r->stream = c->stream;
+  return SIGOPS_OK;
+}
 
 extern "C" {
 

@@ -130,10 +130,42 @@ void stream_sort(Par par, unsigned parts, size_t n, const uint8_t* scheme, const
   });
 }
 
+// One item's inputs as the packing reads them
+struct StreamItem {
+  const uint8_t* sig;  // 64 bytes
+  const uint8_t* aux;  // 32 bytes: digest or public key
+  const uint8_t* msg;  // len bytes (Ed25519)
+  uint64_t len;
+};
+
+// Packs one tile at slot t: c (1..64) items of bucket q, lane l's from item(l); their messages go
+// back to back from msgs + m0. Statuses of the packed lanes are set to ST_CANARY so a lane the
+// kernel did not write shows. Returns the tile's message bytes.
+template <class ItemOf>
+inline uint64_t stream_pack_tile(StreamBufs& B, size_t t, uint32_t q, uint32_t c, uint64_t m0, ItemOf item) {
+  uint8_t* in = B.in + t * (ST_IN_CHUNKS * ST_LANES * 16);
+  uint64_t rel = 0;
+  for (uint32_t l = 0; l < c; l++) {
+    const StreamItem it = item(l);
+    for (uint32_t cc = 0; cc < 4; cc++) memcpy(in + (cc * ST_LANES + l) * 16, it.sig + cc * 16, 16);
+    for (uint32_t cc = 0; cc < 2; cc++) memcpy(in + ((4 + cc) * ST_LANES + l) * 16, it.aux + cc * 16, 16);
+    B.st[t * ST_LANES + l] = ST_CANARY;
+    B.mrel[t * ST_LANES + l] = (uint32_t)rel;
+    B.mlen[t * ST_LANES + l] = (uint32_t)it.len;
+    if (it.len) memcpy(B.msgs + m0 + rel, it.msg, it.len);
+    rel += it.len;
+  }
+  B.desc[t] = StreamDesc{(q << 8) | c, (uint32_t)rel, m0};
+  return rel;
+}
+
+// Bytes a tile's message region takes: 128-byte aligned, with room for the hash's reads past the
+// last message byte (at most one word), so no cache line and no read spans two tiles
+inline uint64_t stream_region_bytes(uint64_t msg_bytes) { return (msg_bytes + 4 + 127) & ~(uint64_t)127; }
+
 // Packs chunk k (items sorted[k * chunk ..] of the processing order) into its slots; returns the
 // number of tiles that hold items (the first ones of the chunk's slots), which the kernel marks
-// done one by one. A tile holds up to 64 consecutive items of one bucket. Statuses of the packed
-// lanes are set to ST_CANARY so a lane the kernel did not write shows.
+// done one by one. A tile holds up to 64 consecutive items of one bucket.
 inline uint32_t stream_pack_chunk(const StreamGeom& g, size_t k, const uint32_t* sorted, const uint8_t* bkt,
                                   const uint8_t* sigs, const uint8_t* aux, const uint8_t* msgs,
                                   const uint64_t* msg_off, StreamBufs& B) {
@@ -147,24 +179,12 @@ inline uint32_t stream_pack_chunk(const StreamGeom& g, size_t k, const uint32_t*
     uint32_t c = 0;
     while (c < ST_LANES && j + c < e && bkt[sorted[j + c]] == q) c++;
     const size_t t = t0 + tiles++;
-    uint8_t* in = B.in + t * (ST_IN_CHUNKS * ST_LANES * 16);
-    uint64_t rel = 0;
-    for (uint32_t l = 0; l < c; l++) {
+    for (uint32_t l = 0; l < c; l++) B.perm[t * ST_LANES + l] = sorted[j + l];
+    const uint64_t rel = stream_pack_tile(B, t, q, c, mcur, [&](uint32_t l) {
       const size_t i = sorted[j + l];
-      for (uint32_t cc = 0; cc < 4; cc++) memcpy(in + (cc * ST_LANES + l) * 16, sigs + i * 64 + cc * 16, 16);
-      for (uint32_t cc = 0; cc < 2; cc++) memcpy(in + ((4 + cc) * ST_LANES + l) * 16, aux + i * 32 + cc * 16, 16);
-      B.perm[t * ST_LANES + l] = (uint32_t)i;
-      B.st[t * ST_LANES + l] = ST_CANARY;
-      const uint64_t len = msg_off[i + 1] - msg_off[i];
-      B.mrel[t * ST_LANES + l] = (uint32_t)rel;
-      B.mlen[t * ST_LANES + l] = (uint32_t)len;
-      if (len) memcpy(B.msgs + mcur + rel, msgs + msg_off[i], len);
-      rel += len;
-    }
-    B.desc[t] = StreamDesc{(q << 8) | c, (uint32_t)rel, mcur};
-    // each tile's region starts 128-byte aligned with room for the hash's reads past the last
-    // message byte (at most one word), so no cache line and no read spans two tiles
-    mcur = (mcur + rel + 4 + 127) & ~(uint64_t)127;
+      return StreamItem{sigs + i * 64, aux + i * 32, msgs + msg_off[i], msg_off[i + 1] - msg_off[i]};
+    });
+    mcur += stream_region_bytes(rel);
     j += c;
   }
   for (size_t t = t0 + tiles; t < t0 + g.tpc; t++) B.desc[t] = StreamDesc{0, 0, 0};
