@@ -458,6 +458,13 @@ def main():
         ok = cor[ii] == 0
         roundtrip_ok &= bool(np.array_equal(out_h[ii][ok], pubs[ok]) and not st_h[ii][ok].any())
     roundtrip_ok &= bool(np.array_equal(st_h[mx.idx[2]] == 0, cor[mx.idx[2]] == 0))
+    # lanes that skip the scalar multiplication: secp items the prologue rejects (r or s out of
+    # range, r not an x-coordinate) and Ed25519 items with s >= l or an undecodable key. This is
+    # the most a wavefront-ballot compaction of rejected items could give back.
+    sec = sch < 2
+    early = int(np.isin(st_h[sec], (1, 2, 3)).sum()) + int(np.isin(st_h[~sec], (1, 2)).sum())
+    rejected = {"early_rejected_items": early, "fraction_of_lanes": round(early / n, 5),
+                "corrupted_items": int(cor.sum())}
 
     # plain per-scheme kernels on the same mixed workload
     plain = None
@@ -733,6 +740,7 @@ def main():
         "plain_incl_transfers": plain_incl,
         "configs": configs,
         "roundtrip_all_items_ok": roundtrip_ok,
+        "rejected_lanes": rejected,
         "served": served,
         "fuel_hashing": fuel,
         "predicates": preds,

@@ -256,12 +256,14 @@ int sigops_set_fault(sigops_ctx* ctx, uint32_t flags);
    SIGOPS_SERVE_SLOTS: requests are packed into the next free slot (a context with its own stream
      and scratch) while the batch stays within max_batch items; arrivals during a launch go out
      with the next batch, and consecutive batches overlap on the device.
-   SIGOPS_SERVE_RING: one persistent launch drains a ring of tile positions in pinned,
-     device-mapped memory: the items of admitted requests are packed, by bucket, into tiles of up
-     to 64 items that mix requests, and published to the running launch, whose wavefronts claim
-     tiles from a device counter; full tiles go out at once, partial ones when wavefronts would
-     otherwise idle. A launch ends after idle_close_us without work (or when a tile needs the
-     careful pass, which runs after it) and the next request starts another.
+   SIGOPS_SERVE_RING: one persistent launch drains tile positions in pinned, device-mapped
+     memory: the items of admitted requests are packed, by bucket, into tiles of up to 64 items
+     that mix requests, and published to the running launch through one queue per scheme; each
+     wavefront claims published tiles from its home scheme's queue (its CU's, so that a CU runs
+     one scheme's code), and from the others only when its own has none. Full tiles go out at
+     once, partial ones when wavefronts would otherwise idle. A launch ends after idle_close_us
+     without work (or when a tile needs the careful pass, which runs after it) and the next
+     request starts another.
    Thread-safe: any thread may submit or wait. ---- */
 #define SIGOPS_SERVE_SLOTS 0
 #define SIGOPS_SERVE_RING 1
@@ -274,8 +276,10 @@ typedef struct {
   uint32_t blocks_per_cu;  /* slot mode: per-slot context setting (0: default) */
   uint32_t mode;           /* SIGOPS_SERVE_SLOTS (0) or SIGOPS_SERVE_RING */
   uint32_t ring_tiles;     /* ring mode: tile positions, a power of two in 64 .. 2^20 (0: 4096) */
-  uint64_t ring_msg_bytes; /* ring mode: message ring bytes, pinned and in HBM (0: 64 MiB); a request
-                              whose longest message exceeds a quarter of it is refused */
+  uint64_t ring_msg_bytes; /* ring mode: message bytes, pinned and in HBM, split evenly over the tile
+                              positions (0: 256 MiB; at least 1 KiB per position, below 4 GiB); a
+                              request holding a message longer than a position's share less 132
+                              bytes is refused */
   uint32_t idle_close_us;  /* ring mode: a launch with no work for this long ends (0: 1000) */
 } sigops_server_config;
 

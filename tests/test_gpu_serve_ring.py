@@ -4,8 +4,8 @@ packed by bucket into tiles that mix requests and handed to one running persiste
 Every result must equal the oracle's, and a request's results must be the same alone and in the
 mix: concurrent submitters; the mixed workload's request sizes (1..4096 items, log-uniform) at
 three arrival rates through the native replay client; the recovery and Ed25519 edge cases (the
-exceptional additions end the launch and go through its careful pass); a ring of 64 positions
-and 64 KiB of messages, so positions and message space wrap and the packer waits for them;
+exceptional additions end the launch and go through its careful pass); a ring of 64 positions,
+so positions are reused many times and the packer waits for free ones;
 launches that end when the ring idles and the next request starts another; refusals at submit."""
 import time
 
@@ -139,19 +139,19 @@ def test_ring_edge_cases_and_careful_pass():
 
 
 def test_ring_small_ring_wraps():
-    """64 positions and 64 KiB of messages: the 20000 items wrap both rings many times"""
+    """64 positions with 4 KiB of messages each: the 20000 items reuse every position many times"""
     n = 20000
     _say(f"ring wrap n={n}")
     pool = _pool(n, 71)
     req = _requests(n, 19)
-    with _ring(ring_tiles=64, ring_msg_bytes=1 << 16) as srv:
+    with _ring(ring_tiles=64, ring_msg_bytes=1 << 18) as srv:
         tickets = [srv.submit(*_slice(pool, req[r], req[r + 1])) for r in range(len(req) - 1)]
         out = np.zeros((n, 64), np.uint8)
         st = np.zeros(n, np.uint8)
         for r, t in enumerate(tickets):
             o, s, _, _ = srv.wait(t)
             out[req[r]:req[r + 1]], st[req[r]:req[r + 1]] = o, s
-        # a message longer than a quarter of the message ring is refused
+        # a message longer than a position's message slot is refused
         sch = np.array([2], np.uint8)
         with pytest.raises(sigops.SigopsError):
             srv.submit(sch, np.zeros((1, 64), np.uint8), np.zeros((1, 32), np.uint8), np.zeros(20000, np.uint8),

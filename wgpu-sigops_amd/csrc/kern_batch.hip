@@ -343,55 +343,139 @@ struct StreamArgs {
   uint4* out;
   u8* st;
   u64* tile_done;
-  const u64* ready;
+  const u64* ready;  // nq words
+  const u32* qpos;   // nq x (mask + 1) positions, or null
   const u32* stop;
   u32* err;
   u32* ctr;
   u32* pend_list;
-  u32 pend_cap, slots, base, mask, seq;
+  u32 pend_cap, slots, base, mask, seq, nq, split0, split1;
+  u64 poll_ticks, limit_ticks;
 };
 
-// Polls between sleeps of 4 x 127 x 64 cycles (about 13 us); 2^20 polls are about 15 s, far beyond
-// any packing delay, so a wave gives up only when the host stopped feeding the launch
-constexpr u32 STREAM_SPIN_LIMIT = 1u << 20;
+// a non-home queue's tile is taken only after this many looks found none in the home queue
+constexpr u32 STREAM_STEAL_AFTER = 3;
 
-// Waits until the host has published tile t of this launch (seq): 1 when it is, 0 when the launch
-// is over before it (the count was closed below t+1, or the host moved on to a later launch), -1
-// when it gave up (the host's stop flag, or the spin limit) after marking the launch failed.
-// `known` is the published count this wavefront last read: the pinned word is read only for a
-// tile beyond it (the host usually packs well ahead of the kernel, so most tiles need no PCIe
-// round trip). One lane polls; the wavefront follows its answer. The acquire is at system scope:
-// a ring position is rewritten by the host while the launch runs, so no cache may keep the line
-// that an earlier tile at the position left there.
-DEV int stream_wait(const StreamArgs& a, u32 t, u32& known) {
-  if (t < known) return 1;
-  int res = 1;
-  u32 seen = known;
+// Claims a tile: a ticket (the tile's index in its queue) of the wavefront's home queue when that
+// has a published tile nobody took yet, else (after STREAM_STEAL_AFTER looks) of the queue with the
+// most such tiles; then waits until the ticket's tile is published. Returns the queue, with the
+// ticket in `ticket`; -1 when the launch is over (every queue closed and taken, the ticket's queue
+// closed below it, or the host moved on to a later launch); -2 when it gave up (the host's stop
+// flag, or limit_ticks) after marking the launch failed.
+// A ticket is an atomic add on the queue's counter, taken only when a free tile was seen, so a
+// claim never retries; wavefronts that race for the last free tile hold tickets of tiles still
+// to come and wait for them. The published counts are read from device copies: one wavefront at
+// a time, at most once per poll_ticks and only when the copies leave it nothing to do, reads the
+// pinned words over PCIe and folds them into the copies (an atomic max: a count only grows, and
+// the closed flag is its top bit); the others read the copy of their block's XCD. A waiting
+// wavefront sleeps longer the farther its tile is, or the longer it found nothing. (When every
+// idle wavefront read host memory itself, the PCIe link had no room left for the tiles' inputs.)
+// One lane works; the wavefront follows its answer. The acquire is at system scope: a ring
+// position is rewritten by the host while the launch runs, so no cache may keep a line of an
+// earlier tile at the position.
+DEV int stream_claim(const StreamArgs& a, u32 home, u32& ticket) {
+  int res = -1;
+  u32 tk = 0;
   if ((threadIdx.x & 63u) == 0) {
-    for (u32 spin = 0;; spin++) {
-      const u64 r = __hip_atomic_load(a.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      const u32 cnt = (u32)r & ~STREAM_CLOSED;
-      if ((u32)(r >> 32) != a.seq || ((u32)r & STREAM_CLOSED && cnt <= t)) {
-        res = 0;
-        break;
+    u8* base = (u8*)a.ctr;
+    u32* claim = a.ctr;  // claim counters of the queues
+    u64* last = (u64*)(base + 64);
+    u64* stop = (u64*)(base + 72);
+    u64* mirror = (u64*)(base + 256 + 256 * (blockIdx.x % STREAM_MIRRORS));
+    const u64 t0 = (u64)wall_clock64();
+    u32 looks = 0, idle = 0;
+    int q = -1;  // the queue whose ticket tk this wavefront holds
+    for (;;) {
+      u64 r[STREAM_QUEUES];
+      int pick = -1;
+      bool over = true, superseded = false;
+      for (int pass = 0; pass < 2; pass++) {
+        pick = -1;
+        over = true;
+        superseded = false;
+        u32 most = 0;
+        for (u32 k = 0; k < a.nq; k++) {
+          r[k] = __hip_atomic_load(mirror + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (r[k] == 0) {
+            over = false;
+            continue;
+          }
+          if ((u32)(r[k] >> 32) != a.seq) {
+            superseded = true;
+            continue;
+          }
+          const u32 cnt = (u32)r[k] & ~STREAM_CLOSED;
+          const u32 c = __hip_atomic_load(claim + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (!((u32)r[k] & STREAM_CLOSED) || c < cnt) over = false;
+          if (c < cnt && pick != (int)home && (k == home || cnt - c > most)) {
+            pick = (int)k;
+            most = cnt - c;
+          }
+        }
+        // nothing to do with the copies as they are: refresh them from pinned memory
+        const bool stuck = q >= 0 ? (r[q] == 0 || (!superseded && ((u32)r[q] & ~STREAM_CLOSED) <= tk &&
+                                                   !((u32)r[q] & STREAM_CLOSED)))
+                                  : (pick < 0 && !over && !superseded);
+        if (!stuck || pass) break;
+        const u64 now = (u64)wall_clock64();
+        u64 prev = __hip_atomic_load(last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (now - prev < a.poll_ticks ||
+            !__hip_atomic_compare_exchange_strong(last, &prev, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT))
+          break;
+        if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+          __hip_atomic_store(stop, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (u32 k = 0; k < a.nq; k++) {
+          const u64 h = __hip_atomic_load(a.ready + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          for (u32 m = 0; m < STREAM_MIRRORS; m++)
+            __hip_atomic_fetch_max((u64*)(base + 256 + 256 * m) + k, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
-      if (cnt > t) {
-        seen = cnt;
-        break;
-      }
-      if (spin >= STREAM_SPIN_LIMIT || __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
-        __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (superseded) {
         res = -1;
         break;
       }
-      __builtin_amdgcn_s_sleep(127);
-      __builtin_amdgcn_s_sleep(127);
-      __builtin_amdgcn_s_sleep(127);
-      __builtin_amdgcn_s_sleep(127);
+      u32 naps = 1;
+      if (q < 0) {
+        if (pick >= 0 && (pick == (int)home || looks >= STREAM_STEAL_AFTER)) {
+          q = pick;
+          tk = __hip_atomic_fetch_add(claim + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          continue;  // the ticket's tile is mostly published already: look at once
+        }
+        if (over) {
+          res = -1;
+          break;
+        }
+        if (pick >= 0) {
+          looks++;
+        } else {
+          naps = 1u << (idle < 5 ? idle : 5);
+          idle++;
+        }
+      } else if (r[q] != 0) {
+        const u32 cnt = (u32)r[q] & ~STREAM_CLOSED;
+        if (tk < cnt) {
+          res = q;
+          break;
+        }
+        if ((u32)r[q] & STREAM_CLOSED) {  // closed below the ticket
+          res = -1;
+          break;
+        }
+        const u32 d = tk - cnt;
+        naps = d / 64 < 31 ? 1 + d / 64 : 32;
+      }
+      const u64 now = (u64)wall_clock64();
+      if (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || now - t0 >= a.limit_ticks) {
+        __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        res = -2;
+        break;
+      }
+      for (u32 k = 0; k < naps; k++) __builtin_amdgcn_s_sleep(127);
     }
   }
   res = __shfl(res, 0);
-  known = __shfl(seen, 0);
+  ticket = __shfl(tk, 0);
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
   return res;
 }
@@ -444,20 +528,25 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
   ct.base = lane_tab;
   ct.slots = lane_slots;
   ct.slot = slot;
-  u32 known = 0;  // tiles this wavefront knows to be published
-  for (;;) {
+  // the home queue of this block: the blocks that share a CU share one (consecutive blocks go to
+  // different CUs, so block b and b + 256 meet), and each queue gets its share of them
+  const u32 hb = blockIdx.x % 256u;
+  const u32 home = a.nq == 1 ? 0u : (hb < a.split0 ? 0u : (hb < a.split1 ? 1u : 2u));
+  for (u32 n_tiles = 0; n_tiles < a.slots; n_tiles++) {
     u32 t = 0;
-    if (lane == 0) t = atomicAdd(&a.ctr[0], 1u);
-    t = __shfl(t, 0);
-    if (t >= a.slots || stream_wait(a, t, known) <= 0) break;
-    const u32 p = (a.base + t) & a.mask;  // the tile's position in the layout
+    const int q = stream_claim(a, home, t);
+    if (q < 0) break;
+    // the tile's position in the layout: the queue's position table, or base + t
+    const u32 p = a.qpos ? __hip_atomic_load(a.qpos + (u64)q * (a.mask + 1) + (t & a.mask), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_SYSTEM)
+                         : (a.base + t) & a.mask;
     const uint4 d = a.desc[p];
     const u32 info = __builtin_amdgcn_readfirstlane(d.x);
     const u32 cnt = info & 0xFFu, b = info >> 8;
     if (cnt == 0) continue;  // an unused position
     if (b >= BUCKET_ED0) {
       // the tile's messages, [m0, m0 + mbytes) rounded up to 16 bytes, pinned -> HBM
-      const u64 m0 = ((u64)d.w << 32) | d.z;
+      const u64 m0 = d.z;
       const u64 q0 = m0 >> 4, nq = ((u64)d.y + 15) >> 4;
       const uint4* src = a.hmsgs + q0;
       uint4* dst = a.dmsgs + q0;
@@ -488,7 +577,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
       const u32 st = b == BUCKET_K1 ? secp_item<CurveK1>(sr, ar, jt, GTabOf<CurveK1>{s_k1}, o)
                                     : secp_item<CurveR1>(sr, ar, jt, GTabOf<CurveR1>{s_r1}, o);
       if (st == ST_PENDING) {
-        const u32 k = atomicAdd(&a.ctr[1], 1u);
+        const u32 k = atomicAdd(&a.ctr[STREAM_CTR_PENDING], 1u);
         if (k < a.pend_cap) a.pend_list[k] = p * 64 + lane;
       }
       stream_store_out(a, p, lane, o);
@@ -498,14 +587,14 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
     // pinned buffers may sit in L2, so the release is at system scope (waiting for the stores
     // alone left results the host could not yet see)
     __threadfence_system();
-    if (lane == 0) __hip_atomic_store(&a.tile_done[p], ((u64)a.seq << 32) | t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (lane == 0) __hip_atomic_store(&a.tile_done[p], ((u64)a.seq << 32) | d.w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 // the careful pass of a streamed launch: pending position lanes, inputs and results in the tile layout
 __global__ __launch_bounds__(64) void k_stream_careful(StreamArgs a, const uint4* __restrict__ gk1,
                                                        const uint4* __restrict__ gr1) {
-  const u32 cnt = min(a.ctr[1], a.pend_cap);
+  const u32 cnt = min(a.ctr[STREAM_CTR_PENDING], a.pend_cap);
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
     const u32 p = a.pend_list[j];
     const u32 t = p >> 6, l = p & 63u;  // position and lane
@@ -544,6 +633,10 @@ hipError_t launch_stream(const StreamLaunch& L, hipStream_t s) {
   a.st = L.st;
   a.tile_done = L.tile_done;
   a.ready = L.ready;
+  a.qpos = L.qpos;
+  a.nq = L.nq ? L.nq : 1;
+  a.split0 = L.split0;
+  a.split1 = L.split1;
   a.stop = L.stop;
   a.err = L.err;
   a.ctr = L.ctr;
@@ -553,7 +646,9 @@ hipError_t launch_stream(const StreamLaunch& L, hipStream_t s) {
   a.base = L.base;
   a.mask = L.mask;
   a.seq = L.seq;
-  k_zero_u32<<<1, 64, 0, s>>>(L.ctr, 2);
+  a.poll_ticks = L.poll_ticks;
+  a.limit_ticks = L.limit_ticks;
+  k_zero_u32<<<1, 256, 0, s>>>(L.ctr, STREAM_CTR_BYTES / 4);
   k_stream_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                       (uint4*)L.lane_tab, L.lane_slots);
   k_stream_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);

@@ -74,6 +74,15 @@ hipError_t launch_ed_verify(const BatchLaunch& L, bool strict, hipStream_t s);
 // launch drains them (layout in stream_host.hpp). A streamed batch fills positions 0.. of the
 // layout once; a serving ring reuses positions base + t (mod positions) for the launch's tile t ----
 constexpr uint32_t STREAM_CLOSED = 1u << 31;  // in the published count: no tile beyond it will come
+// A launch drains up to STREAM_QUEUES queues of tiles (a serving ring: one per scheme; a batch: one).
+// Device words of a streamed launch, cleared by the launch: u32 [q] the claim counter of queue q,
+// [STREAM_CTR_PENDING] the pending count; u64 at byte 64 the wall-clock time of the last read of
+// the pinned counts, at byte 72 the stop flag seen there; STREAM_MIRRORS copies of the pinned
+// counts (u64 per queue) from byte 256 on, 256 bytes apart
+constexpr uint32_t STREAM_QUEUES = 3;
+constexpr uint32_t STREAM_CTR_PENDING = 4;
+constexpr uint32_t STREAM_MIRRORS = 8;
+constexpr uint32_t STREAM_CTR_BYTES = 256 + STREAM_MIRRORS * 256;
 struct StreamLaunch {
   const void* desc;      // pinned: 16-byte tile descriptors, one per position
   const void* in;        // pinned: inputs, [position][6][64] 16-byte chunks
@@ -83,22 +92,36 @@ struct StreamLaunch {
   void* dmsgs;           // device: the messages copied by their tiles (same offsets), 16 bytes of slack
   void* out;             // pinned: results, [position][4][64] 16-byte chunks
   uint8_t* st;           // pinned: statuses, [position][64]
-  uint64_t* tile_done;   // pinned: per position, seq << 32 | t once tile t's results are written
-  const uint64_t* ready; // pinned: seq << 32 | tiles published so far, | STREAM_CLOSED at the end
+  uint64_t* tile_done;   // pinned: per position, seq << 32 | the descriptor's tag once the tile's results are written
+  const uint64_t* ready; // pinned, per queue: seq << 32 | tiles published so far, | STREAM_CLOSED at the end
+  const uint32_t* qpos;  // pinned: per queue, mask + 1 entries: position of the queue's tile t (t & mask);
+                         // null: one queue whose tile t sits at position (base + t) & mask
+  uint32_t nq;           // queues (1 .. STREAM_QUEUES)
+  uint32_t split0, split1;  // home queue of block b: 0 if b % 256 < split0, 1 if < split1, else 2
   const uint32_t* stop;  // pinned: nonzero makes waiting waves give up
   uint32_t* err;         // pinned: set to 1 by a wave that gave up waiting
-  uint32_t* ctr;         // device: [0] next tile, [1] pending count (cleared by the launch)
+  uint32_t* ctr;         // device: STREAM_CTR_BYTES of counters and polling state (cleared by the launch)
   uint32_t* pend_list;   // device: position lanes (position * 64 + lane) the careful pass redoes
   uint32_t pend_cap;     // entries pend_list holds (further pending lanes are not recorded)
   const void* gk1;
   const void* gr1;
   const void* gb;
   void* lane_tab;
-  uint32_t slots;        // tiles this launch may take at most (the ring: 2^31)
-  uint32_t base, mask;   // tile t sits at position (base + t) & mask (a batch: 0 and ~0)
+  uint32_t slots;        // tiles a wavefront may take at most
+  uint32_t base, mask;   // see qpos (a batch: 0 and ~0)
   uint32_t seq, lane_slots, grid;
+  uint64_t poll_ticks;   // wall-clock ticks between reads of the pinned count (device-wide)
+  uint64_t limit_ticks;  // a wave that waits this long for a tile gives up
 };
 hipError_t launch_stream(const StreamLaunch& L, hipStream_t s);
+// the polling interval (2 us) and the give-up time (15 s) of a streamed launch in the device's
+// wall-clock ticks
+inline void stream_poll_ticks(int device, StreamLaunch& L) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0) khz = 100000;
+  L.poll_ticks = (uint64_t)khz * 2 / 1000;
+  L.limit_ticks = (uint64_t)khz * 15000;
+}
 
 // what a serving ring borrows from a context: the resident tables, the lane tables and the launch
 // shape they are sized for, the processing order of the buckets, and the context's stream

@@ -59,9 +59,9 @@ struct Request {
   std::vector<uint64_t> off;  // n + 1, starting at 0
   uint64_t t_submit = 0, t_done = 0;
   uint64_t bytes = 0;
-  size_t remaining = 0;  // ring: items whose results are not in yet (completion thread only)
+  std::atomic<size_t> remaining{0};  // ring: items whose results are not in yet
   bool done = false;
-  int err = 0;
+  std::atomic<int> err{0};
 };
 
 uint64_t request_bytes(size_t n, uint64_t msg_bytes) {
@@ -343,12 +343,20 @@ void sigops_server::complete_loop() {
 namespace {
 
 using sigops::NBUCKETS;
+using sigops::STREAM_QUEUES;
 using sigops_host::StreamBufs;
 using sigops_host::StreamDesc;
 using sigops_host::StreamItem;
 using sigops_host::ST_LANES;
 
 constexpr uint8_t ST_PENDING = 0xFF;  // a lane whose item waits for the careful pass
+constexpr uint32_t Q_K1 = 0, Q_R1 = 1, Q_ED = 2;  // the ring's queues
+// relative cost of a tile of each queue (per-scheme kernel times), for the home split
+constexpr double Q_COST[STREAM_QUEUES] = {1.0, 1.34, 1.3};
+
+uint32_t queue_of(uint32_t bucket) {
+  return bucket == sigops::BUCKET_K1 ? Q_K1 : bucket == sigops::BUCKET_R1 ? Q_R1 : Q_ED;
+}
 
 struct RingItem {
   Request* r;
@@ -357,31 +365,31 @@ struct RingItem {
 
 // host bookkeeping of one ring position
 struct RingPos {
-  uint64_t tag = 0;        // the done word the kernel writes for the tile held here
-  uint64_t mend = 0;       // message ring: virtual end of the tile's region
+  uint64_t expect = 0;     // the done word the kernel writes for the tile held here
+  uint32_t seq = 0;        // its launch
   uint32_t cnt = 0;
   bool ed = false;
   bool seen = false;       // its results were read
-  bool held = false;       // lanes wait for the epoch's careful pass; the position stays taken
+  bool held = false;       // lanes wait for the launch's careful pass; the position stays taken
   uint64_t pend_mask = 0;  // those lanes
   RingItem items[ST_LANES];
 };
 
-// one persistent launch: tiles [g0, g_end) of the ring's global tile numbering
+// one persistent launch
 struct Epoch {
   uint32_t seq = 0;
-  uint64_t g0 = 0, g_end = UINT64_MAX;
   bool closed = false;
   bool failed = false;  // the launch was not made: its tiles are lost
   hipEvent_t ev = nullptr;
 };
 
-// regions of the ring's pinned block, each 256-byte aligned (positions' layout as in stream_host.hpp)
+// regions of the ring's pinned block, each 256-byte aligned (positions' layout as in
+// stream_host.hpp; every position owns a fixed message slot of `ms` bytes)
 struct RingRegions {
-  size_t desc, in, mrel, mlen, out, st, done, msgs, total;
+  size_t desc, in, mrel, mlen, out, st, done, qpos, msgs, total;
 };
 
-RingRegions ring_regions(size_t R, uint64_t MB) {
+RingRegions ring_regions(size_t R, uint64_t ms) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   RingRegions r;
   size_t at = 0;
@@ -399,8 +407,10 @@ RingRegions ring_regions(size_t R, uint64_t MB) {
   at = al(at + R * ST_LANES);
   r.done = at;
   at = al(at + R * 8);
+  r.qpos = at;
+  at = al(at + STREAM_QUEUES * R * 4);
   r.msgs = at;
-  at = al(at + MB + 256);
+  at = al(at + R * ms + 256);
   r.total = at;
   return r;
 }
@@ -411,15 +421,21 @@ void* dev_addr(void* host) {
 }
 
 struct RingEngine {
+  // Each launch publishes through its own words (seq % READY_SLOTS, one per queue): a closed
+  // launch's wavefronts may still read its counts while the next launch publishes; at most
+  // READY_SLOTS - 1 launches are outstanding at once
+  static constexpr uint32_t READY_SLOTS = 16;
+  static constexpr size_t CTL_BYTES = READY_SLOTS * 4 * 8 + 64;  // then u32 stop, u32 err
+
   sigops_server* S = nullptr;
   sigops_ctx* ctx = nullptr;  // the resident tables, the lane tables and the stream
   sigops::RingResources res{};
   uint32_t R = 0, mask = 0, W = 0;  // positions, positions - 1, wavefronts of a launch
-  uint64_t MB = 0;                  // message ring bytes
+  uint64_t ms = 0;                  // message bytes per position
   uint64_t idle_ns = 0;
   RingRegions reg{};
-  uint8_t* pin = nullptr;  // positions' regions and the message ring
-  uint8_t* ctl = nullptr;  // READY_SLOTS u64 published counts (epoch seq % READY_SLOTS), u32 stop, u32 err
+  uint8_t* pin = nullptr;  // positions' regions, queue tables and message slots
+  uint8_t* ctl = nullptr;  // ready words, stop, err
   uint8_t* dpin = nullptr;
   uint8_t* dctl = nullptr;
   uint8_t* dmsgs = nullptr;  // HBM copies of the tiles' messages (same offsets)
@@ -427,40 +443,48 @@ struct RingEngine {
   uint32_t* dpend = nullptr;
   StreamBufs B{};
   uint64_t* done = nullptr;
+  uint32_t* qpos = nullptr;
   std::vector<RingPos> pos;
   std::vector<hipEvent_t> free_ev;
   // dispatcher only
   std::vector<RingItem> open[NBUCKETS];  // admitted items not yet in a tile
-  uint64_t m_head = 0;
   uint32_t seq = 0;
   bool ep_open = false, ep_launched = false;
-  uint64_t ep_g0 = 0, ep_count = 0;
+  uint32_t qcount[STREAM_QUEUES] = {};  // tiles published per queue in the open launch
+  uint32_t tag_next = 0;
+  double work[STREAM_QUEUES] = {0.43, 0.35 / 1.34, 0.22 / 1.3};  // recent tiles per queue (decaying)
   uint64_t last_busy = 0;
   // shared between the dispatcher and the completion thread
-  std::atomic<uint64_t> g_next{0}, g_reclaimed{0}, n_seen{0}, m_tail{0};
-  std::atomic<uint32_t> close_seq{0};  // an epoch with lanes waiting for its careful pass
+  std::mutex pmu;                  // free positions and newly published ones
+  std::vector<uint32_t> free_pos;  // positions no tile holds
+  std::vector<uint32_t> published; // positions published since the completion thread last looked
+  std::atomic<uint64_t> n_pub{0}, n_seen{0};
+  std::atomic<uint32_t> close_seq{0};  // a launch with lanes waiting for its careful pass
   std::atomic<bool> disp_done{false};
   std::mutex emu;  // epochs, free_ev
   std::deque<Epoch> epochs;
   std::thread disp, comp;
+  // packing and unpacking are spread over pools once there are a few tiles at a time
+  std::unique_ptr<sigops_host::Pool> pack_pool, unpack_pool;
+  struct Planned {
+    uint32_t p, bucket;
+  };
+  std::vector<Planned> planned;  // tiles placed in this publication round, packed together
 
   int init(sigops_server* srv);
   void release();
   void admit(Request* r);
-  bool pack(uint32_t q, const RingItem* items, uint32_t c);
+  bool plan(uint32_t q, const RingItem* items, uint32_t c);
+  void pack_planned();
   void publish();
   void open_epoch();
-  void publish_count();
+  void publish_counts();
   void close_epoch();
   void dispatch_loop();
   void read_tile(RingPos& P, uint32_t p, std::vector<Request*>& finished);
   void finish_lane(RingPos& P, uint32_t p, uint32_t l, std::vector<Request*>& finished, int err);
   void complete_loop();
-  // Each epoch publishes through its own word (seq % READY_SLOTS): a closed epoch's wavefronts may
-  // still be reading its count while the next epoch publishes, and at most READY_SLOTS - 1 epochs
-  // are outstanding at once
-  static constexpr uint32_t READY_SLOTS = 16;
-  uint64_t* ready() { return (uint64_t*)ctl + seq % READY_SLOTS; }
+  uint64_t* ready(uint32_t q) { return (uint64_t*)ctl + (seq % READY_SLOTS) * 4 + q; }
 };
 
 int RingEngine::init(sigops_server* srv) {
@@ -468,8 +492,9 @@ int RingEngine::init(sigops_server* srv) {
   const sigops_server_config& cfg = S->cfg;
   R = cfg.ring_tiles;
   mask = R - 1;
-  MB = cfg.ring_msg_bytes;
+  ms = (cfg.ring_msg_bytes / R) & ~(uint64_t)127;
   idle_ns = (uint64_t)cfg.idle_close_us * 1000;
+  if (ms < 1024) return SIGOPS_E_ARG;
   sigops_config cc{};
   cc.device = cfg.device;
   cc.max_batch = 1u << 16;  // the context's own batch buffers are not used by the ring
@@ -477,27 +502,32 @@ int RingEngine::init(sigops_server* srv) {
   if (!ctx) return SIGOPS_E_DEVICE;
   if (sigops_ring_resources(ctx, &res)) return SIGOPS_E_DEVICE;
   W = res.grid * 4;
-  reg = ring_regions(R, MB);
+  reg = ring_regions(R, ms);
   if (hipHostMalloc((void**)&pin, reg.total, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
     pin = nullptr;
     return SIGOPS_E_NOMEM;
   }
-  if (hipHostMalloc((void**)&ctl, READY_SLOTS * 8 + 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+  if (hipHostMalloc((void**)&ctl, CTL_BYTES, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
     ctl = nullptr;
     return SIGOPS_E_NOMEM;
   }
   memset(pin, 0, reg.total);
-  memset(ctl, 0, READY_SLOTS * 8 + 64);
+  memset(ctl, 0, CTL_BYTES);
   dpin = (uint8_t*)dev_addr(pin);
   dctl = (uint8_t*)dev_addr(ctl);
   if (!dpin || !dctl) return SIGOPS_E_DEVICE;
-  if (hipMalloc((void**)&dmsgs, MB + 256) != hipSuccess || hipMalloc((void**)&dctr, 2 * sizeof(uint32_t)) != hipSuccess ||
+  if (hipMalloc((void**)&dmsgs, R * ms + 256) != hipSuccess ||
+      hipMalloc((void**)&dctr, sigops::STREAM_CTR_BYTES) != hipSuccess ||
       hipMalloc((void**)&dpend, (size_t)R * ST_LANES * sizeof(uint32_t)) != hipSuccess)
     return SIGOPS_E_NOMEM;
   B = StreamBufs{(StreamDesc*)(pin + reg.desc), pin + reg.in, (uint32_t*)(pin + reg.mrel), (uint32_t*)(pin + reg.mlen),
                  pin + reg.msgs, pin + reg.out, pin + reg.st, nullptr};
   done = (uint64_t*)(pin + reg.done);
+  qpos = (uint32_t*)(pin + reg.qpos);
   pos.resize(R);
+  for (uint32_t p = R; p-- > 0;) free_pos.push_back(p);
+  pack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
+  unpack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
   for (int k = 0; k < 4; k++) {
     hipEvent_t e;
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SIGOPS_E_DEVICE;
@@ -539,60 +569,88 @@ void RingEngine::admit(Request* r) {
   }
 }
 
-// packs c items of bucket q into the next ring position; false when no position or no message
-// space is free (the items stay where they are)
-bool RingEngine::pack(uint32_t q, const RingItem* items, uint32_t c) {
-  const uint64_t g = g_next.load(std::memory_order_relaxed);
-  if (g - g_reclaimed.load(std::memory_order_acquire) >= R) return false;
-  uint64_t bytes = 0;
-  for (uint32_t l = 0; l < c; l++) bytes += items[l].r->off[items[l].idx + 1] - items[l].r->off[items[l].idx];
-  const uint64_t need = sigops_host::stream_region_bytes(bytes);
-  uint64_t start = m_head;
-  if (start % MB + need > MB) start = (start / MB + 1) * MB;  // a region never wraps
-  if (start + need - m_tail.load(std::memory_order_acquire) > MB) return false;
+// places c items of bucket q in a free position of its queue (packed by pack_planned); false
+// when no position is free (the items stay where they are)
+bool RingEngine::plan(uint32_t q, const RingItem* items, uint32_t c) {
   if (!ep_open) {
     std::lock_guard<std::mutex> lk(emu);
     if (epochs.size() >= READY_SLOTS - 1) return false;  // the completion thread retires them first
   }
+  uint32_t p;
+  {
+    std::lock_guard<std::mutex> lk(pmu);
+    if (free_pos.empty()) return false;
+    p = free_pos.back();
+    free_pos.pop_back();
+  }
   if (!ep_open) open_epoch();
-  const uint32_t p = (uint32_t)(g & mask);
   RingPos& P = pos[p];
-  sigops_host::stream_pack_tile(B, p, q, c, start % MB, [&](uint32_t l) {
-    const Request& r = *items[l].r;
-    const uint32_t i = items[l].idx;
-    return StreamItem{r.sigs.data() + (size_t)i * 64, r.aux.data() + (size_t)i * 32,
-                      r.msgs.data() + r.off[i], r.off[i + 1] - r.off[i]};
-  });
   for (uint32_t l = 0; l < c; l++) P.items[l] = items[l];
   P.cnt = c;
-  P.ed = q >= sigops::BUCKET_ED0;
+  P.ed = queue_of(q) == Q_ED;
   P.seen = P.held = false;
   P.pend_mask = 0;
-  P.mend = start + need;
-  P.tag = ((uint64_t)seq << 32) | ep_count;
-  ep_count++;
-  m_head = start + need;
-  S->n_items += c;
-  g_next.store(g + 1, std::memory_order_release);  // the completion thread reads P after this
+  P.seq = seq;
+  planned.push_back(Planned{p, q});
   return true;
+}
+
+// packs the planned tiles into their positions (on the pool when there are several), then enters
+// them in their queues' position tables in order; publish_counts makes them visible
+void RingEngine::pack_planned() {
+  auto pack_one = [&](const Planned& t, uint32_t tag) {
+    RingPos& P = pos[t.p];
+    sigops_host::stream_pack_tile(B, t.p, t.bucket, P.cnt, (uint64_t)t.p * ms, tag, [&](uint32_t l) {
+      const Request& r = *P.items[l].r;
+      const uint32_t i = P.items[l].idx;
+      return StreamItem{r.sigs.data() + (size_t)i * 64, r.aux.data() + (size_t)i * 32,
+                        r.msgs.data() + r.off[i], r.off[i + 1] - r.off[i]};
+    });
+  };
+  const uint32_t tag0 = tag_next;
+  tag_next += (uint32_t)planned.size();
+  if (planned.size() >= 8 && pack_pool->size() > 1) {
+    std::atomic<size_t> next{0};
+    pack_pool->run([&](unsigned) {
+      for (size_t k; (k = next.fetch_add(1)) < planned.size();) pack_one(planned[k], tag0 + (uint32_t)k);
+    });
+  } else {
+    for (size_t k = 0; k < planned.size(); k++) pack_one(planned[k], tag0 + (uint32_t)k);
+  }
+  {
+    std::lock_guard<std::mutex> lk(pmu);
+    for (size_t k = 0; k < planned.size(); k++) {
+      const Planned& t = planned[k];
+      RingPos& P = pos[t.p];
+      const uint32_t qu = queue_of(t.bucket);
+      P.expect = ((uint64_t)seq << 32) | (tag0 + (uint32_t)k);
+      qpos[(size_t)qu * R + (qcount[qu] & mask)] = t.p;
+      qcount[qu]++;
+      work[qu] += 1.0;
+      S->n_items += P.cnt;
+      published.push_back(t.p);
+    }
+  }
+  n_pub.fetch_add(planned.size(), std::memory_order_release);
+  planned.clear();
 }
 
 void RingEngine::open_epoch() {
   seq = seq + 1 ? seq + 1 : 1;  // never 0, the cleared done words
   ep_open = true;
   ep_launched = false;
-  ep_g0 = g_next.load(std::memory_order_relaxed);
-  ep_count = 0;
+  for (auto& k : qcount) k = 0;
+  for (uint32_t q = 0; q < STREAM_QUEUES; q++) __atomic_store_n(ready(q), (uint64_t)seq << 32, __ATOMIC_RELEASE);
 }
 
-// publishes the epoch's packed tiles, launching the epoch with its first ones
-void RingEngine::publish_count() {
-  __atomic_store_n(ready(), ((uint64_t)seq << 32) | ep_count, __ATOMIC_RELEASE);
+// publishes the open launch's tiles, launching it with its first ones
+void RingEngine::publish_counts() {
+  for (uint32_t q = 0; q < STREAM_QUEUES; q++)
+    __atomic_store_n(ready(q), ((uint64_t)seq << 32) | qcount[q], __ATOMIC_RELEASE);
   if (ep_launched) return;
   ep_launched = true;
   Epoch e;
   e.seq = seq;
-  e.g0 = ep_g0;
   {
     std::lock_guard<std::mutex> g(emu);
     if (!free_ev.empty()) {
@@ -601,6 +659,12 @@ void RingEngine::publish_count() {
     }
   }
   if (!e.ev && hipEventCreateWithFlags(&e.ev, hipEventDisableTiming) != hipSuccess) e.ev = nullptr;
+  // home queues of the launch's blocks in proportion to the recent work per queue, by tile cost
+  double share[STREAM_QUEUES], tot = 0;
+  for (uint32_t q = 0; q < STREAM_QUEUES; q++) tot += share[q] = work[q] * Q_COST[q];
+  for (auto& w : work) w *= 0.5;
+  const uint32_t s0 = (uint32_t)(256.0 * share[0] / tot + 0.5);
+  const uint32_t s1 = (uint32_t)(256.0 * (share[0] + share[1]) / tot + 0.5);
   sigops::StreamLaunch L{};
   L.desc = dpin + reg.desc;
   L.in = dpin + reg.in;
@@ -611,9 +675,13 @@ void RingEngine::publish_count() {
   L.out = dpin + reg.out;
   L.st = dpin + reg.st;
   L.tile_done = (uint64_t*)(dpin + reg.done);
-  L.ready = (const uint64_t*)dctl + seq % READY_SLOTS;
-  L.stop = (const uint32_t*)(dctl + READY_SLOTS * 8);
-  L.err = (uint32_t*)(dctl + READY_SLOTS * 8 + 4);
+  L.ready = (const uint64_t*)dctl + (seq % READY_SLOTS) * 4;
+  L.qpos = (const uint32_t*)(dpin + reg.qpos);
+  L.nq = STREAM_QUEUES;
+  L.split0 = s0;
+  L.split1 = std::max(s0, s1);
+  L.stop = (const uint32_t*)(dctl + READY_SLOTS * 4 * 8);
+  L.err = (uint32_t*)(dctl + READY_SLOTS * 4 * 8 + 4);
   L.ctr = dctr;
   L.pend_list = dpend;
   L.pend_cap = R * ST_LANES;
@@ -622,18 +690,18 @@ void RingEngine::publish_count() {
   L.gb = res.gb;
   L.lane_tab = res.lane_tab;
   L.slots = sigops::STREAM_CLOSED - 1;
-  L.base = (uint32_t)(ep_g0 & mask);
+  L.base = 0;
   L.mask = mask;
   L.seq = seq;
   L.lane_slots = res.lane_slots;
   L.grid = res.grid;
+  sigops::stream_poll_ticks(S->cfg.device, L);
   const bool ok = e.ev && sigops::launch_stream(L, res.stream) == hipSuccess &&
                   hipEventRecord(e.ev, res.stream) == hipSuccess;
   if (!ok) {
-    // the tiles of this epoch will not run: the completion thread fails them
+    // the tiles of this launch will not run: the completion thread fails them
     e.failed = true;
     e.closed = true;
-    e.g_end = ep_g0 + ep_count;
     ep_open = false;
   }
   S->n_batches++;
@@ -645,38 +713,35 @@ void RingEngine::close_epoch() {
   if (!ep_open) return;
   ep_open = false;
   if (!ep_launched) return;  // nothing was published
-  __atomic_store_n(ready(), ((uint64_t)seq << 32) | sigops::STREAM_CLOSED | ep_count, __ATOMIC_RELEASE);
+  for (uint32_t q = 0; q < STREAM_QUEUES; q++)
+    __atomic_store_n(ready(q), ((uint64_t)seq << 32) | sigops::STREAM_CLOSED | qcount[q], __ATOMIC_RELEASE);
   std::lock_guard<std::mutex> g(emu);
   for (auto& e : epochs)
-    if (e.seq == seq && !e.closed) {
-      e.g_end = ep_g0 + ep_count;
-      e.closed = true;
-    }
+    if (e.seq == seq) e.closed = true;
 }
 
 // Full tiles of every bucket go out at once; when fewer tiles are in flight than the launch has
 // wavefronts, the remainders go out too (one tile per secp bucket, the Ed25519 bins merged in
-// increasing length order, so a tile's lanes still hash similar lengths).
+// increasing length order, so a tile's lanes still hash similar lengths). A tile's messages fit
+// its position's message slot.
 void RingEngine::publish() {
-  const uint64_t before = ep_count;
-  const uint32_t before_seq = seq;
-  bool full = false;  // out of positions or message space
+  const uint64_t before = n_pub.load(std::memory_order_relaxed);
+  bool full = false;  // out of positions
   auto drain = [&](std::vector<RingItem>& v, uint32_t q, bool partial) {
     size_t at = 0;
     while (!full && at < v.size()) {
       uint32_t c = (uint32_t)std::min<size_t>(ST_LANES, v.size() - at);
       if (c < ST_LANES && !partial) break;
-      // keep a tile's messages within a quarter of the message ring
       uint64_t bytes = 0;
       for (uint32_t l = 0; l < c; l++) {
         const Request& r = *v[at + l].r;
         bytes += r.off[v[at + l].idx + 1] - r.off[v[at + l].idx];
-        if (l > 0 && bytes > MB / 4) {
+        if (l > 0 && sigops_host::stream_region_bytes(bytes) > ms) {
           c = l;
           break;
         }
       }
-      if (!pack(q, v.data() + at, c)) {
+      if (!plan(q, v.data() + at, c)) {
         full = true;
         break;
       }
@@ -688,7 +753,8 @@ void RingEngine::publish() {
     const uint32_t q = res.bucket_order[r];
     drain(open[q], q, false);
   }
-  const uint64_t inflight = g_next.load(std::memory_order_relaxed) - n_seen.load(std::memory_order_acquire);
+  const uint64_t inflight =
+      n_pub.load(std::memory_order_relaxed) + planned.size() - n_seen.load(std::memory_order_acquire);
   if (!full && inflight < W) {
     drain(open[sigops::BUCKET_R1], sigops::BUCKET_R1, true);
     drain(open[sigops::BUCKET_K1], sigops::BUCKET_K1, true);
@@ -706,7 +772,8 @@ void RingEngine::publish() {
         open[sigops_host::stream_bucket(2, it.r->off[it.idx + 1] - it.r->off[it.idx])].push_back(it);
     }
   }
-  if (seq != before_seq || ep_count != before) publish_count();
+  if (!planned.empty()) pack_planned();
+  if (n_pub.load(std::memory_order_relaxed) != before) publish_counts();
 }
 
 void RingEngine::dispatch_loop() {
@@ -735,12 +802,11 @@ void RingEngine::dispatch_loop() {
     publish();
     bool open_items = false;
     for (auto& v : open) open_items = open_items || !v.empty();
-    const uint64_t gn = g_next.load(std::memory_order_relaxed);
-    const bool idle = !open_items && queue_empty && n_seen.load(std::memory_order_acquire) == gn;
+    const bool idle = !open_items && queue_empty &&
+                      n_seen.load(std::memory_order_acquire) == n_pub.load(std::memory_order_relaxed);
     const uint64_t t = now_ns();
     if (!idle) last_busy = t;
-    if (ep_open && (close_seq.load() == seq || ep_count >= (1u << 30) || (idle && t - last_busy >= idle_ns) ||
-                    (idle && stopping)))
+    if (ep_open && (close_seq.load() == seq || (idle && t - last_busy >= idle_ns) || (idle && stopping)))
       close_epoch();
     if (stopping && idle && !ep_open) break;
   }
@@ -762,7 +828,7 @@ void RingEngine::finish_lane(RingPos& P, uint32_t p, uint32_t l, std::vector<Req
     r.status[i] = st;
     if (st > (P.ed ? SIGOPS_ED_MISMATCH : SIGOPS_ST_INFINITY)) r.err = SIGOPS_E_CANARY;
   }
-  if (--r.remaining == 0) finished.push_back(&r);
+  if (r.remaining.fetch_sub(1) == 1) finished.push_back(&r);
 }
 
 void RingEngine::read_tile(RingPos& P, uint32_t p, std::vector<Request*>& finished) {
@@ -775,28 +841,48 @@ void RingEngine::read_tile(RingPos& P, uint32_t p, std::vector<Request*>& finish
   }
   if (P.pend_mask) {
     P.held = true;
-    close_seq.store((uint32_t)(P.tag >> 32));
+    close_seq.store(P.seq);
   }
   P.seen = true;
 }
 
 void RingEngine::complete_loop() {
   (void)hipSetDevice(S->cfg.device);
-  uint64_t g_scan = 0;  // every tile below it has been read
+  std::vector<uint32_t> live;  // published positions not yet given back
+  std::vector<uint32_t> freed, ready_tiles;
   std::vector<Request*> finished;
+  std::vector<std::vector<Request*>> fin(unpack_pool->size());  // per pool thread
   for (;;) {
     bool progress = false;
-    const uint64_t gn = g_next.load(std::memory_order_acquire);
-    for (uint64_t g = g_scan; g < gn; g++) {
-      RingPos& P = pos[g & mask];
-      if (P.seen || __atomic_load_n(&done[g & mask], __ATOMIC_ACQUIRE) != P.tag) continue;
-      read_tile(P, (uint32_t)(g & mask), finished);
-      n_seen.fetch_add(1, std::memory_order_release);
+    {
+      std::lock_guard<std::mutex> lk(pmu);
+      live.insert(live.end(), published.begin(), published.end());
+      published.clear();
+    }
+    ready_tiles.clear();
+    for (uint32_t p : live) {
+      const RingPos& P = pos[p];
+      if (!P.seen && __atomic_load_n(&done[p], __ATOMIC_ACQUIRE) == P.expect) ready_tiles.push_back(p);
+    }
+    if (ready_tiles.size() >= 8 && unpack_pool->size() > 1) {
+      std::atomic<size_t> next{0};
+      unpack_pool->run([&](unsigned t) {
+        for (size_t k; (k = next.fetch_add(1)) < ready_tiles.size();)
+          read_tile(pos[ready_tiles[k]], ready_tiles[k], fin[t]);
+      });
+      for (auto& f : fin) {
+        finished.insert(finished.end(), f.begin(), f.end());
+        f.clear();
+      }
+    } else {
+      for (uint32_t p : ready_tiles) read_tile(pos[p], p, finished);
+    }
+    if (!ready_tiles.empty()) {
+      n_seen.fetch_add(ready_tiles.size(), std::memory_order_release);
       progress = true;
     }
-    while (g_scan < gn && pos[g_scan & mask].seen) g_scan++;
-    // epochs whose launch has ended: the careful pass has rewritten the held lanes; a tile the
-    // launch did not finish is lost (its requests fail)
+    // launches that have ended: the careful pass has rewritten the held lanes; a tile the launch
+    // did not finish is lost (its requests fail)
     for (;;) {
       Epoch e;
       {
@@ -807,10 +893,16 @@ void RingEngine::complete_loop() {
         epochs.pop_front();
       }
       const bool dev_ok = !e.failed && hipEventQuery(e.ev) == hipSuccess;
-      for (uint64_t g = e.g0; g < e.g_end; g++) {
-        RingPos& P = pos[g & mask];
-        const uint32_t p = (uint32_t)(g & mask);
-        if (!P.seen && dev_ok && __atomic_load_n(&done[p], __ATOMIC_ACQUIRE) == P.tag) {
+      {
+        // every tile of the launch was published before it was closed
+        std::lock_guard<std::mutex> lk(pmu);
+        live.insert(live.end(), published.begin(), published.end());
+        published.clear();
+      }
+      for (uint32_t p : live) {
+        RingPos& P = pos[p];
+        if (P.seq != e.seq) continue;
+        if (!P.seen && dev_ok && __atomic_load_n(&done[p], __ATOMIC_ACQUIRE) == P.expect) {
           read_tile(P, p, finished);
           n_seen.fetch_add(1, std::memory_order_release);
         }
@@ -831,16 +923,19 @@ void RingEngine::complete_loop() {
       }
       progress = true;
     }
-    while (g_scan < gn && pos[g_scan & mask].seen) g_scan++;
-    // positions free in order; a held tile keeps its position and every later one
-    uint64_t gr = g_reclaimed.load(std::memory_order_relaxed);
-    const uint64_t gr0 = gr;
-    while (gr < g_scan && !pos[gr & mask].held) {
-      m_tail.store(pos[gr & mask].mend, std::memory_order_release);
-      gr++;
+    // positions whose results are read go back to the free list (held ones after their launch)
+    size_t k = 0;
+    for (uint32_t p : live) {
+      if (pos[p].seen && !pos[p].held)
+        freed.push_back(p);
+      else
+        live[k++] = p;
     }
-    if (gr != gr0) {
-      g_reclaimed.store(gr, std::memory_order_release);
+    live.resize(k);
+    if (!freed.empty()) {
+      std::lock_guard<std::mutex> lk(pmu);
+      free_pos.insert(free_pos.end(), freed.begin(), freed.end());
+      freed.clear();
       progress = true;
     }
     if (progress) {
@@ -857,11 +952,12 @@ void RingEngine::complete_loop() {
       S->cv_done.notify_all();
       continue;
     }
-    if (disp_done.load(std::memory_order_acquire)) {
+    if (disp_done.load(std::memory_order_acquire) && live.empty()) {
       std::lock_guard<std::mutex> g(emu);
-      if (epochs.empty() && g_scan == g_next.load() && g_reclaimed.load() == g_scan) break;
+      std::lock_guard<std::mutex> lk(pmu);
+      if (epochs.empty() && published.empty()) break;
     }
-    std::this_thread::sleep_for(std::chrono::microseconds(gn == g_scan ? 50 : 10));
+    std::this_thread::sleep_for(std::chrono::microseconds(live.empty() ? 50 : 10));
   }
 }
 
@@ -876,11 +972,11 @@ sigops_server* sigops_server_create(const sigops_server_config* cfg_in) {
   if (!cfg.hbm_budget) cfg.hbm_budget = 1ull << 30;
   if (!cfg.max_batch) cfg.max_batch = 1u << 20;
   if (!cfg.ring_tiles) cfg.ring_tiles = 4096;
-  if (!cfg.ring_msg_bytes) cfg.ring_msg_bytes = 64ull << 20;
+  if (!cfg.ring_msg_bytes) cfg.ring_msg_bytes = 256ull << 20;
   if (!cfg.idle_close_us) cfg.idle_close_us = 1000;
   if (cfg.mode > SIGOPS_SERVE_RING) return nullptr;
   if (cfg.ring_tiles < 64 || cfg.ring_tiles > (1u << 20) || (cfg.ring_tiles & (cfg.ring_tiles - 1))) return nullptr;
-  if (cfg.ring_msg_bytes < (1u << 16) || cfg.ring_msg_bytes % 256) return nullptr;
+  if (cfg.ring_msg_bytes / cfg.ring_tiles < 1024 || cfg.ring_msg_bytes >= (1ull << 32)) return nullptr;
   auto* S = new (std::nothrow) sigops_server();
   if (!S) return nullptr;
   S->cfg = cfg;
@@ -963,7 +1059,7 @@ int64_t sigops_server_submit(sigops_server* S, const uint8_t* schemes, const uin
     if (total && !msgs) return SIGOPS_E_ARG;
     if (S->ring)
       for (size_t i = 0; i < n; i++)
-        if (msg_off[i + 1] - msg_off[i] > S->cfg.ring_msg_bytes / 4) return SIGOPS_E_ARG;
+        if (sigops_host::stream_region_bytes(msg_off[i + 1] - msg_off[i]) > S->ring->ms) return SIGOPS_E_ARG;
     r->schemes.assign(schemes, schemes + n);
     r->sigs.assign(sigs, sigs + n * 64);
     r->aux.assign(aux, aux + n * 32);
@@ -1003,7 +1099,7 @@ int sigops_server_wait(sigops_server* S, int64_t ticket, uint8_t* out, uint8_t* 
   lk.unlock();
   if (t_submit_ns) *t_submit_ns = r->t_submit;
   if (t_done_ns) *t_done_ns = r->t_done;
-  if (r->err) return r->err;
+  if (const int e = r->err.load()) return e;
   if (out && r->n) memcpy(out, r->out.data(), r->n * 64);
   if (status && r->n) memcpy(status, r->status.data(), r->n);
   return 0;

@@ -865,7 +865,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
     if ((e = hipHostMalloc((void**)&c->s_ctl, 256, hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess)
       return xfer_fail(c, e, "streamed batch control");
     memset(c->s_ctl, 0, 256);
-    if ((e = hipMalloc((void**)&c->d_sctr, 2 * sizeof(uint32_t))) != hipSuccess) return hip_fail(c, e, "hipMalloc");
+    if ((e = hipMalloc((void**)&c->d_sctr, sigops::STREAM_CTR_BYTES)) != hipSuccess) return hip_fail(c, e, "hipMalloc");
     if ((e = hipEventCreateWithFlags(&c->s_ev, hipEventDisableTiming)) != hipSuccess) return hip_fail(c, e, "event");
     for (hipEvent_t& te : c->s_tev)
       if ((e = hipEventCreate(&te)) != hipSuccess) return hip_fail(c, e, "event");
@@ -905,7 +905,10 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
     packed[k].store(1, std::memory_order_release);
     std::lock_guard<std::mutex> lk(pub_mu);
     while (published < g.nchunks && packed[published].load(std::memory_order_acquire)) published++;
-    __atomic_store_n(ready, ((uint64_t)seq << 32) | (published * g.tpc), __ATOMIC_RELEASE);
+    // the last chunk closes the count: the launch's wavefronts end once every tile is taken
+    __atomic_store_n(ready,
+                     ((uint64_t)seq << 32) | (published * g.tpc) | (published == g.nchunks ? sigops::STREAM_CLOSED : 0),
+                     __ATOMIC_RELEASE);
     if (published == g.nchunks)
       t_packed.store(std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - T0).count());
   };
@@ -942,12 +945,14 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   L.gr1 = c->d_gtab_r1;
   L.gb = c->d_btab;
   L.lane_tab = c->d_lane_tab;
-  L.slots = (uint32_t)g.slots;
+  L.slots = sigops::STREAM_CLOSED - 1;
   L.base = 0;
   L.mask = ~0u;
+  L.nq = 1;
   L.seq = seq;
   L.lane_slots = c->slots;
   L.grid = std::min<uint32_t>(c->grid, (uint32_t)((g.slots + 3) / 4));
+  sigops::stream_poll_ticks(c->device, L);
   HIPCHK(c, hipEventRecord(c->s_tev[0], s));
   if (!(c->fault & SIGOPS_FAULT_SKIP_COMPUTE)) HIPCHK(c, sigops::launch_stream(L, s));
   HIPCHK(c, hipEventRecord(c->s_tev[1], s));
@@ -995,10 +1000,10 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return fail(c, SIGOPS_E_CANARY, "a batch result was not written");
   if (lost.load() || __atomic_load_n(err, __ATOMIC_ACQUIRE)) return 1;
   // items the careful pass redid
-  uint32_t ctr[2];
+  uint32_t ctr[sigops::STREAM_CTR_PENDING + 1];
   HIPCHK(c, hipMemcpy(ctr, c->d_sctr, sizeof ctr, hipMemcpyDeviceToHost));
-  if (ctr[1]) {
-    std::vector<uint32_t> pend(std::min<size_t>(ctr[1], n));
+  if (ctr[sigops::STREAM_CTR_PENDING]) {
+    std::vector<uint32_t> pend(std::min<size_t>(ctr[sigops::STREAM_CTR_PENDING], n));
     HIPCHK(c, hipMemcpy(pend.data(), c->d_pend + 1, pend.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t p : pend) stream_unpack_lane(p, B, out, status);
   }

@@ -8,7 +8,7 @@
 // running, takes tiles of published chunks and reads them over PCIe, so the packing, the transfers
 // and the compute overlap, and the device needs no bucketing pass. Layout, per tile slot t (a
 // chunk owns `tpc` consecutive slots, the unused ones empty):
-//   desc[t]                  {info = bucket << 8 | item count (0: empty), message bytes, message start}
+//   desc[t]                  {info = bucket << 8 | item count (0: empty), message bytes, message start, tag}
 //   in[(t * 6 + k) * 64 + l] 16-byte chunk k of lane l's item: signature (k = 0..3), digest or key (4..5)
 //   mrel / mlen[t * 64 + l]  the item's message: start relative to the tile's, and length
 //   msgs                     the tiles' messages back to back, each tile's region 128-byte aligned
@@ -35,7 +35,8 @@ constexpr uint8_t ST_CANARY = 0xEE;
 struct StreamDesc {
   uint32_t info;    // bucket << 8 | count
   uint32_t mbytes;  // message bytes of the tile (its region in msgs, before alignment padding)
-  uint64_t m0;      // start of the tile's region in msgs (128-byte aligned)
+  uint32_t m0;      // start of the tile's region in msgs (128-byte aligned)
+  uint32_t tag;     // the kernel sets the position's done word to seq << 32 | tag
 };
 static_assert(sizeof(StreamDesc) == 16, "one 16-byte load per descriptor");
 
@@ -142,7 +143,8 @@ struct StreamItem {
 // back to back from msgs + m0. Statuses of the packed lanes are set to ST_CANARY so a lane the
 // kernel did not write shows. Returns the tile's message bytes.
 template <class ItemOf>
-inline uint64_t stream_pack_tile(StreamBufs& B, size_t t, uint32_t q, uint32_t c, uint64_t m0, ItemOf item) {
+inline uint64_t stream_pack_tile(StreamBufs& B, size_t t, uint32_t q, uint32_t c, uint64_t m0, uint32_t tag,
+                                 ItemOf item) {
   uint8_t* in = B.in + t * (ST_IN_CHUNKS * ST_LANES * 16);
   uint64_t rel = 0;
   for (uint32_t l = 0; l < c; l++) {
@@ -155,7 +157,7 @@ inline uint64_t stream_pack_tile(StreamBufs& B, size_t t, uint32_t q, uint32_t c
     if (it.len) memcpy(B.msgs + m0 + rel, it.msg, it.len);
     rel += it.len;
   }
-  B.desc[t] = StreamDesc{(q << 8) | c, (uint32_t)rel, m0};
+  B.desc[t] = StreamDesc{(q << 8) | c, (uint32_t)rel, (uint32_t)m0, tag};
   return rel;
 }
 
@@ -180,14 +182,14 @@ inline uint32_t stream_pack_chunk(const StreamGeom& g, size_t k, const uint32_t*
     while (c < ST_LANES && j + c < e && bkt[sorted[j + c]] == q) c++;
     const size_t t = t0 + tiles++;
     for (uint32_t l = 0; l < c; l++) B.perm[t * ST_LANES + l] = sorted[j + l];
-    const uint64_t rel = stream_pack_tile(B, t, q, c, mcur, [&](uint32_t l) {
+    const uint64_t rel = stream_pack_tile(B, t, q, c, mcur, (uint32_t)t, [&](uint32_t l) {
       const size_t i = sorted[j + l];
       return StreamItem{sigs + i * 64, aux + i * 32, msgs + msg_off[i], msg_off[i + 1] - msg_off[i]};
     });
     mcur += stream_region_bytes(rel);
     j += c;
   }
-  for (size_t t = t0 + tiles; t < t0 + g.tpc; t++) B.desc[t] = StreamDesc{0, 0, 0};
+  for (size_t t = t0 + tiles; t < t0 + g.tpc; t++) B.desc[t] = StreamDesc{0, 0, 0, (uint32_t)t};
   return tiles;
 }
 
