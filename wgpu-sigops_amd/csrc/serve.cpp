@@ -363,6 +363,32 @@ struct RingItem {
   uint32_t idx;
 };
 
+// admitted items not yet in a tile, of one bucket: taken from the front, the taken part dropped
+// now and then (erasing it every time moved millions of items per round when requests piled up)
+struct OpenList {
+  std::vector<RingItem> v;
+  size_t head = 0;
+  size_t size() const { return v.size() - head; }
+  bool empty() const { return head == v.size(); }
+  const RingItem* data() const { return v.data() + head; }
+  const RingItem& operator[](size_t k) const { return v[head + k]; }
+  void push(const RingItem& it) { v.push_back(it); }
+  void take(size_t k) {
+    head += k;
+    if (head == v.size()) {
+      v.clear();
+      head = 0;
+    } else if (head >= 4096 && head * 2 >= v.size()) {
+      v.erase(v.begin(), v.begin() + (long)head);
+      head = 0;
+    }
+  }
+  void clear() {
+    v.clear();
+    head = 0;
+  }
+};
+
 // host bookkeeping of one ring position
 struct RingPos {
   uint64_t expect = 0;     // the done word the kernel writes for the tile held here
@@ -447,7 +473,7 @@ struct RingEngine {
   std::vector<RingPos> pos;
   std::vector<hipEvent_t> free_ev;
   // dispatcher only
-  std::vector<RingItem> open[NBUCKETS];  // admitted items not yet in a tile
+  OpenList open[NBUCKETS];  // admitted items not yet in a tile, per bucket
   uint32_t seq = 0;
   bool ep_open = false, ep_launched = false;
   uint32_t qcount[STREAM_QUEUES] = {};  // tiles published per queue in the open launch
@@ -565,7 +591,7 @@ void RingEngine::admit(Request* r) {
   r->status.assign(r->n, 0);
   for (size_t i = 0; i < r->n; i++) {
     const uint32_t q = sigops_host::stream_bucket(r->schemes[i], r->off[i + 1] - r->off[i]);
-    open[q].push_back(RingItem{r, (uint32_t)i});
+    open[q].push(RingItem{r, (uint32_t)i});
   }
 }
 
@@ -720,14 +746,14 @@ void RingEngine::close_epoch() {
     if (e.seq == seq) e.closed = true;
 }
 
-// Full tiles of every bucket go out at once; when fewer tiles are in flight than the launch has
-// wavefronts, the remainders go out too (one tile per secp bucket, the Ed25519 bins merged in
+// Full tiles of every bucket go out at once (see below for the order); when fewer tiles are in
+// flight than the launch has wavefronts, the remainders go out too (one tile per secp bucket, the Ed25519 bins merged in
 // increasing length order, so a tile's lanes still hash similar lengths). A tile's messages fit
 // its position's message slot.
 void RingEngine::publish() {
   const uint64_t before = n_pub.load(std::memory_order_relaxed);
   bool full = false;  // out of positions
-  auto drain = [&](std::vector<RingItem>& v, uint32_t q, bool partial) {
+  auto drain = [&](OpenList& v, uint32_t q, bool partial) {
     size_t at = 0;
     while (!full && at < v.size()) {
       uint32_t c = (uint32_t)std::min<size_t>(ST_LANES, v.size() - at);
@@ -747,29 +773,64 @@ void RingEngine::publish() {
       }
       at += c;
     }
-    v.erase(v.begin(), v.begin() + (long)at);
+    v.take(at);
   };
-  for (uint32_t r = 0; r < NBUCKETS && !full; r++) {
-    const uint32_t q = res.bucket_order[r];
-    drain(open[q], q, false);
+  // full tiles one at a time from the queue with the most of them (within a queue, its buckets in
+  // processing order), so that when positions run short every queue still gets its share and the
+  // wavefronts at home in it stay busy
+  for (;;) {
+    size_t most = 0;
+    uint32_t take = NBUCKETS;
+    for (uint32_t qu = 0; qu < STREAM_QUEUES; qu++) {
+      size_t tiles = 0;
+      uint32_t first = NBUCKETS;
+      for (uint32_t r = 0; r < NBUCKETS; r++) {
+        const uint32_t b = res.bucket_order[r];
+        if (queue_of(b) != qu || open[b].size() < ST_LANES) continue;
+        tiles += open[b].size() / ST_LANES;
+        if (first == NBUCKETS) first = b;
+      }
+      if (tiles > most) {
+        most = tiles;
+        take = first;
+      }
+    }
+    if (take == NBUCKETS || full) break;
+    OpenList& v = open[take];
+    uint32_t c = ST_LANES;
+    uint64_t bytes = 0;
+    for (uint32_t l = 0; l < c; l++) {
+      bytes += v[l].r->off[v[l].idx + 1] - v[l].r->off[v[l].idx];
+      if (l > 0 && sigops_host::stream_region_bytes(bytes) > ms) {
+        c = l;
+        break;
+      }
+    }
+    if (!plan(take, v.data(), c)) {
+      full = true;
+      break;
+    }
+    v.take(c);
   }
   const uint64_t inflight =
       n_pub.load(std::memory_order_relaxed) + planned.size() - n_seen.load(std::memory_order_acquire);
   if (!full && inflight < W) {
     drain(open[sigops::BUCKET_R1], sigops::BUCKET_R1, true);
     drain(open[sigops::BUCKET_K1], sigops::BUCKET_K1, true);
-    std::vector<RingItem> ed;
+    OpenList ed;
     uint32_t last = 0;
     for (uint32_t b = sigops::BUCKET_ED0; b < NBUCKETS; b++) {
       if (!open[b].empty()) last = b;
-      ed.insert(ed.end(), open[b].begin(), open[b].end());
+      ed.v.insert(ed.v.end(), open[b].data(), open[b].data() + open[b].size());
       open[b].clear();
     }
     if (!ed.empty()) {
       drain(ed, last, true);
       // what did not fit goes back to the bins
-      for (const RingItem& it : ed)
-        open[sigops_host::stream_bucket(2, it.r->off[it.idx + 1] - it.r->off[it.idx])].push_back(it);
+      for (size_t k = 0; k < ed.size(); k++) {
+        const RingItem& it = ed[k];
+        open[sigops_host::stream_bucket(2, it.r->off[it.idx + 1] - it.r->off[it.idx])].push(it);
+      }
     }
   }
   if (!planned.empty()) pack_planned();
