@@ -368,6 +368,9 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--tile-rounds", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-API runs (transfers included, serving): the persistent kernel is then "
+                         "launched only as the timed steps launch it, for profiling")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the plain per-scheme configurations (secp256k1 1024, secp256r1 / Ed25519 65536)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
@@ -536,17 +539,24 @@ def main():
 
     # copy-then-launch (transfers, then the graph, then transfers back), then streamed (the
     # default: the host packs chunks into pinned memory while the persistent kernel drains them)
-    copy_ms, copy_ok = host_batch(False)
-    incl_ms, incl_ok = host_batch(True)
-    phases = ctx.batch_host_ms()
-    log("streamed batch phases (ms): " + ", ".join(f"{k} {v}" for k, v in phases.items()))
-    log(f"with transfers: streamed {incl_ms:.2f} ms per batch ({n / incl_ms * 1e3 / 1e6:.2f} M sigs/s), "
-        f"copy-then-launch {copy_ms:.2f} ms; identical to the device path: {incl_ok}, {copy_ok}")
+    incl = None
+    if not args.no_host:
+        copy_ms, copy_ok = host_batch(False)
+        incl_ms, incl_ok = host_batch(True)
+        phases = ctx.batch_host_ms()
+        log("streamed batch phases (ms): " + ", ".join(f"{k} {v}" for k, v in phases.items()))
+        log(f"with transfers: streamed {incl_ms:.2f} ms per batch ({n / incl_ms * 1e3 / 1e6:.2f} M sigs/s), "
+            f"copy-then-launch {copy_ms:.2f} ms; identical to the device path: {incl_ok}, {copy_ok}")
+        incl = {"ms_per_batch": round(incl_ms, 3), "sigs_per_s": round(n / incl_ms * 1e3, 1),
+                "identical_to_device_path": incl_ok, "method": "streamed host batch (sigops_batch)",
+                "phases_ms_last_call": phases,
+                "copy_then_launch": {"ms_per_batch": round(copy_ms, 3), "sigs_per_s": round(n / copy_ms * 1e3, 1),
+                                     "identical_to_device_path": copy_ok}}
 
     # the same items through the three plain host entry points (one call per scheme, inputs split
     # by scheme beforehand): what a caller of the per-scheme API pays, transfers included
     plain_incl = None
-    if not args.no_plain:
+    if not args.no_plain and not args.no_host:
         ii = [np.nonzero(h_sch == s)[0] for s in (0, 1, 2)]
         ed_off = np.concatenate([h_off[ii[2]], h_off[ii[2][-1] + 1: ii[2][-1] + 2]]) if len(ii[2]) else \
             np.zeros(1, np.uint64)
@@ -582,7 +592,7 @@ def main():
     # into per-slot batches) and the device-drained ring (requests packed into tiles of one running
     # persistent launch)
     served = None
-    if not args.no_serve:
+    if not args.no_serve and not args.no_host:
         served = []
         rng = np.random.default_rng(7)
         # serving moves every request over PCIe, so its capacity is the transfer-inclusive batch rate
@@ -731,12 +741,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain": plain,
-        "incl_transfers": {"ms_per_batch": round(incl_ms, 3), "sigs_per_s": round(n / incl_ms * 1e3, 1),
-                           "identical_to_device_path": incl_ok, "method": "streamed host batch (sigops_batch)",
-                           "phases_ms_last_call": phases,
-                           "copy_then_launch": {"ms_per_batch": round(copy_ms, 3),
-                                                "sigs_per_s": round(n / copy_ms * 1e3, 1),
-                                                "identical_to_device_path": copy_ok}},
+        "incl_transfers": incl,
         "plain_incl_transfers": plain_incl,
         "configs": configs,
         "roundtrip_all_items_ok": roundtrip_ok,

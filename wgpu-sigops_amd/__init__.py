@@ -128,14 +128,25 @@ def lib():
 
 
 def source_digest():
-    """SHA-256 over the library's sources (csrc/ and the Makefile): ties a committed counter profile
-    to the build it measured"""
+    """SHA-256 over the sources the mixed-batch kernel's code object is built from (csrc/kern_batch.hip,
+    the headers it includes, directly or not, and the Makefile): ties a committed counter profile of
+    that kernel to the build it measured, while host-only sources may change"""
     import hashlib
-    h = hashlib.sha256()
+    import re
     csrc = os.path.join(PKG_DIR, "csrc")
-    for name in sorted(os.listdir(csrc)) + ["../Makefile"]:
+    names, todo = set(), ["kern_batch.hip"]
+    while todo:
+        name = todo.pop()
         path = os.path.join(csrc, name)
-        if name.endswith((".hip", ".hpp", ".cpp", ".h", ".py", "Makefile")) and os.path.isfile(path):
+        if name in names or not os.path.isfile(path):
+            continue
+        names.add(name)
+        with open(path, encoding="utf-8", errors="replace") as f:
+            todo.extend(re.findall(r'^\s*#\s*include\s+"([^"/]+)"', f.read(), re.M))
+    h = hashlib.sha256()
+    for name in sorted(names) + ["../Makefile"]:
+        path = os.path.join(csrc, name)
+        if os.path.isfile(path):
             h.update(name.encode())
             with open(path, "rb") as f:
                 h.update(f.read())
