@@ -128,6 +128,25 @@ def test_recover_generated_vs_openssl(curve):
             assert want == (0, pubs[i].tobytes())
 
 
+@pytest.mark.parametrize("curve", [0, 1])
+def test_recover_agrees_with_verification_under_the_signers_key(curve):
+    """On corrupted signatures, OpenSSL's ECDSA verification under the signer's key accepts exactly
+    the signatures from which recovery returns the signer's key, except those whose recovery bit
+    (bit 7 of s's first byte) was flipped: verification ignores that bit, and recovery with it
+    restored returns the signer. (The benchmark checks its OpenSSL baseline by the same rule.)"""
+    sigs, digs, pubs, cor = O.gen_secp(curve, 77 + curve, 12000, corrupt_ppm=250000, nthreads=8)
+    out, st = O.recover_batch(curve, sigs, digs, nthreads=8)
+    _, valid = S.time_ecdsa_verify(curve, sigs, digs, pubs, 8)
+    signer = (st == 0) & (out == pubs).all(axis=1)
+    assert signer[cor == 0].all()
+    odd = np.nonzero(valid.astype(bool) != signer)[0]
+    fixed = sigs[odd].copy()
+    fixed[:, 32] ^= 0x80
+    o2, t2 = O.recover_batch(curve, fixed, digs[odd])
+    assert (valid.astype(bool)[odd] & (t2 == 0) & (o2 == pubs[odd]).all(axis=1)).all()
+    assert int(cor.sum()) > 2000
+
+
 def test_secp256k1_reference_nonce_kat():
     """the reference signs with sk = 1 and its fixed nonce k() and asserts the signature; the
     oracle's signer with the same inputs gives it byte for byte, with recovery parity 0"""
