@@ -642,6 +642,13 @@ def main():
                         f"p99 {served[-1]['p99_ms']} ms, {served[-1]['sustained_sigs_per_s'] / 1e6:.2f} M sigs/s "
                         f"sustained, identical: {same}")
 
+    # the served form's capacity (everything submitted at once) over the three plain per-scheme host
+    # calls on the same items, transfers included on both sides
+    served_vs_plain = None
+    if served and plain_incl:
+        served_vs_plain = {srv_name: round(e["sustained_sigs_per_s"] / plain_incl["sigs_per_s"], 3)
+                           for e in served if e["load"] == "saturated" for srv_name in [e["server"]]}
+
     # roofline of the dominant kernel (the persistent verification kernel): its VALU use per launch
     # from the counter profile of this build and launch shape, over its duration measured here
     mac_sig = mixed_mac_per_sig(sch, mx.lens)
@@ -757,6 +764,7 @@ def main():
         "roundtrip_all_items_ok": roundtrip_ok,
         "rejected_lanes": rejected,
         "served": served,
+        "served_over_plain_incl_transfers": served_vs_plain,
         "fuel_hashing": fuel,
         "predicates": preds,
         "probe": {"mad_per_s": mad_per_s, "fma64_per_s": fma_per_s},

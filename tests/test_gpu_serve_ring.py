@@ -3,7 +3,7 @@ packed by bucket into tiles that mix requests and handed to one running persiste
 
 Every result must equal the oracle's, and a request's results must be the same alone and in the
 mix: concurrent submitters; the mixed workload's request sizes (1..4096 items, log-uniform) at
-three arrival rates through the native replay client; the recovery and Ed25519 edge cases (the
+three arrival rates through the native replay client (also for the slot-batch server); the recovery and Ed25519 edge cases (the
 exceptional additions end the launch and go through its careful pass); a ring of 64 positions,
 so positions are reused many times and the packer waits for free ones;
 launches that end when the ring idles and the next request starts another; refusals at submit."""
@@ -76,10 +76,12 @@ def test_ring_alone_and_mixed_are_identical():
     assert items == 2 * n
 
 
+@pytest.mark.parametrize("mode", ["ring", "slots"])
 @pytest.mark.parametrize("rate", [2000, 10000, 40000])
-def test_ring_mixed_request_sizes_at_three_rates(rate):
+def test_mixed_request_sizes_at_three_rates(rate, mode):
     """requests of 1..4096 items (log-uniform) drawn from a pool, Poisson arrivals at `rate` per
-    second; every request's results equal the oracle's and the same request submitted alone"""
+    second, through either server; every request's results equal the oracle's and the same
+    request submitted alone"""
     n = 8192
     pool = _pool(n, 61)
     rng = np.random.default_rng(rate)
@@ -89,9 +91,9 @@ def test_ring_mixed_request_sizes_at_three_rates(rate):
     idx = np.concatenate([np.arange(a, a + k) for a, k in zip(starts, sizes)])
     req_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     arrivals = np.cumsum(rng.exponential(1e9 / rate, nreq)).astype(np.uint64)
-    _say(f"ring replay rate {rate}/s: {nreq} requests, {len(idx)} items")
+    _say(f"{mode} replay rate {rate}/s: {nreq} requests, {len(idx)} items")
     arrs = _gather(pool, idx)
-    with _ring() as srv:
+    with (_ring() if mode == "ring" else sigops.Server(slots=3)) as srv:
         lat, out, st = srv.replay(*arrs, req_off, arrivals)
         _expect_at(pool, idx, out, st)
         assert (lat > 0).all() and (lat < 10e9).all()
