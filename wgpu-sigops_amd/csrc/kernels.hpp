@@ -84,12 +84,14 @@ constexpr uint32_t STREAM_CTR_PENDING = 4;
 constexpr uint32_t STREAM_MIRRORS = 8;
 constexpr uint32_t STREAM_CTR_BYTES = 256 + STREAM_MIRRORS * 256;
 struct StreamLaunch {
-  const void* desc;      // pinned: 16-byte tile descriptors, one per position
-  const void* in;        // pinned: inputs, [position][6][64] 16-byte chunks
-  const uint32_t* mrel;  // pinned: per position lane, message start relative to the tile's
-  const uint32_t* mlen;  // pinned: per position lane, message length
-  const void* hmsgs;     // pinned: the tiles' messages
-  void* dmsgs;           // device: the messages copied by their tiles (same offsets), 16 bytes of slack
+  const void* desc;      // 16-byte tile descriptors, one per position (pinned, or an HBM copy)
+  const void* in;        // inputs, [position][6][64] 16-byte chunks (pinned, or an HBM copy)
+  const uint32_t* mrel;  // per position lane, message start relative to the tile's
+  const uint32_t* mlen;  // per position lane, message length
+  const void* hmsgs;     // the tiles' messages: pinned, copied by each tile to dmsgs; or equal to dmsgs
+                         // when the host has copied them to HBM before publishing the tile
+  void* dmsgs;           // device: the messages the hash reads (same offsets)
+  uint64_t msg_cap;      // bytes of hmsgs and dmsgs: a tile's region [m0, m0 + bytes + 16) lies within
   void* out;             // pinned: results, [position][4][64] 16-byte chunks
   uint8_t* st;           // pinned: statuses, [position][64]
   uint64_t* tile_done;   // pinned: per position, seq << 32 | the descriptor's tag once the tile's results are written

@@ -372,4 +372,26 @@ static __global__ void k_fill_u32(u32* p, u32 v, u32 n) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+// one recovery on the fast ladder: the status (ST_PENDING when the ladder met an exceptional
+// addition and the careful pass must redo the item) and the key X || Y (zeros unless ST_OK)
+template <class C, class GT>
+DEV u32 secp_item(const u32 sr[16], const u32 dr[8], STabDev& tab, const GT& gt, u32 o[16]) {
+  secp_pro pro;
+  secp_prologue<C>(sr, dr, pro);
+  u32 st = pro.status;
+  if (st == ST_OK) {
+    bool exc = false;
+    const jac Q = secp_ladder<C>(pro, tab, gt, exc);
+    if (exc)
+      st = ST_PENDING;
+    else
+      secp_output<C>(Q, o);
+  }
+  if (st != ST_OK) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) o[k] = 0;
+  }
+  return st;
+}
+
 }  // namespace sigops

@@ -698,6 +698,7 @@ void RingEngine::publish_counts() {
   L.mlen = (const uint32_t*)(dpin + reg.mlen);
   L.hmsgs = dpin + reg.msgs;
   L.dmsgs = dmsgs;
+  L.msg_cap = R * ms + 256;  // both message buffers hold every position's slot plus 256 bytes
   L.out = dpin + reg.out;
   L.st = dpin + reg.st;
   L.tile_done = (uint64_t*)(dpin + reg.done);

@@ -900,6 +900,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   std::vector<uint32_t> tiles(g.nchunks, 0);
   std::mutex pub_mu;
   size_t published = 0;
+  std::atomic<bool> lost{false};
   auto pack = [&](size_t k) {
     tiles[k] = stream_pack_chunk(g, k, sorted, bkt, sigs, aux, msgs, msg_off, B);
     packed[k].store(1, std::memory_order_release);
@@ -932,6 +933,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   L.mlen = (const uint32_t*)(DP + R.mlen);
   L.hmsgs = DP + R.msgs;
   L.dmsgs = c->msg.p;
+  L.msg_cap = g.msg_base[g.nchunks] + 16;
   L.out = DP + R.out;
   L.st = DP + R.st;
   L.tile_done = (uint64_t*)(DP + R.done);
@@ -960,7 +962,6 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   const auto T1 = clk::now();
   // the rest of the chunks, then the results of every chunk as its tiles finish
   std::atomic<size_t> next_unpack{0};
-  std::atomic<bool> lost{false};
   next.store(head);
   pool.run([&](unsigned) {
     for (size_t k; (k = next.fetch_add(1)) < g.nchunks;) pack(k);
