@@ -60,6 +60,10 @@ struct Request {
   uint64_t t_submit = 0, t_done = 0;
   uint64_t bytes = 0;
   std::atomic<size_t> remaining{0};  // ring: items whose results are not in yet
+  // ring: the items in bucket order (order[bstart[b] .. bstart[b + 1]) are bucket b's), computed
+  // by the submitting thread
+  std::vector<uint32_t> order;
+  uint32_t bstart[sigops::NBUCKETS + 1] = {};
   bool done = false;
   std::atomic<int> err{0};
 };
@@ -363,29 +367,46 @@ struct RingItem {
   uint32_t idx;
 };
 
-// admitted items not yet in a tile, of one bucket: taken from the front, the taken part dropped
-// now and then (erasing it every time moved millions of items per round when requests piled up)
+// admitted items not yet in a tile, of one bucket: runs of a request's items (its order[begin ..
+// end)), taken from the front. Admission appends one run per request and bucket, not one entry per
+// item.
+struct Run {
+  Request* r;
+  uint32_t begin, end;
+  uint32_t bucket;
+};
+
 struct OpenList {
-  std::vector<RingItem> v;
-  size_t head = 0;
-  size_t size() const { return v.size() - head; }
-  bool empty() const { return head == v.size(); }
-  const RingItem* data() const { return v.data() + head; }
-  const RingItem& operator[](size_t k) const { return v[head + k]; }
-  void push(const RingItem& it) { v.push_back(it); }
-  void take(size_t k) {
-    head += k;
-    if (head == v.size()) {
-      v.clear();
-      head = 0;
-    } else if (head >= 4096 && head * 2 >= v.size()) {
-      v.erase(v.begin(), v.begin() + (long)head);
-      head = 0;
+  std::deque<Run> runs;
+  size_t count = 0;
+  size_t size() const { return count; }
+  bool empty() const { return count == 0; }
+  void push(const Run& x) {
+    if (x.end > x.begin) {
+      runs.push_back(x);
+      count += x.end - x.begin;
+    }
+  }
+  // the first (up to) c items, not removed
+  uint32_t peek(RingItem* out, uint32_t c) const {
+    uint32_t k = 0;
+    for (auto it = runs.begin(); it != runs.end() && k < c; ++it)
+      for (uint32_t j = it->begin; j < it->end && k < c; j++) out[k++] = RingItem{it->r, it->r->order[j]};
+    return k;
+  }
+  void take(uint32_t c) {
+    count -= c;
+    while (c) {
+      Run& f = runs.front();
+      const uint32_t k = std::min(c, f.end - f.begin);
+      f.begin += k;
+      c -= k;
+      if (f.begin == f.end) runs.pop_front();
     }
   }
   void clear() {
-    v.clear();
-    head = 0;
+    runs.clear();
+    count = 0;
   }
 };
 
@@ -584,15 +605,10 @@ void RingEngine::release() {
   ctx = nullptr;
 }
 
-// a request's items into the bucket lists
+// a request's items into the bucket lists: one run per bucket it has items of
 void RingEngine::admit(Request* r) {
   r->remaining = r->n;
-  r->out.assign(r->n * 64, 0);
-  r->status.assign(r->n, 0);
-  for (size_t i = 0; i < r->n; i++) {
-    const uint32_t q = sigops_host::stream_bucket(r->schemes[i], r->off[i + 1] - r->off[i]);
-    open[q].push(RingItem{r, (uint32_t)i});
-  }
+  for (uint32_t b = 0; b < NBUCKETS; b++) open[b].push(Run{r, r->bstart[b], r->bstart[b + 1], b});
 }
 
 // places c items of bucket q in a free position of its queue (packed by pack_planned); false
@@ -754,85 +770,66 @@ void RingEngine::close_epoch() {
 void RingEngine::publish() {
   const uint64_t before = n_pub.load(std::memory_order_relaxed);
   bool full = false;  // out of positions
-  auto drain = [&](OpenList& v, uint32_t q, bool partial) {
-    size_t at = 0;
-    while (!full && at < v.size()) {
-      uint32_t c = (uint32_t)std::min<size_t>(ST_LANES, v.size() - at);
-      if (c < ST_LANES && !partial) break;
-      uint64_t bytes = 0;
-      for (uint32_t l = 0; l < c; l++) {
-        const Request& r = *v[at + l].r;
-        bytes += r.off[v[at + l].idx + 1] - r.off[v[at + l].idx];
-        if (l > 0 && sigops_host::stream_region_bytes(bytes) > ms) {
-          c = l;
-          break;
-        }
-      }
-      if (!plan(q, v.data() + at, c)) {
-        full = true;
-        break;
-      }
-      at += c;
-    }
-    v.take(at);
-  };
-  // full tiles one at a time from the queue with the most of them (within a queue, its buckets in
-  // processing order), so that when positions run short every queue still gets its share and the
-  // wavefronts at home in it stay busy
-  for (;;) {
-    size_t most = 0;
-    uint32_t take = NBUCKETS;
-    for (uint32_t qu = 0; qu < STREAM_QUEUES; qu++) {
-      size_t tiles = 0;
-      uint32_t first = NBUCKETS;
-      for (uint32_t r = 0; r < NBUCKETS; r++) {
-        const uint32_t b = res.bucket_order[r];
-        if (queue_of(b) != qu || open[b].size() < ST_LANES) continue;
-        tiles += open[b].size() / ST_LANES;
-        if (first == NBUCKETS) first = b;
-      }
-      if (tiles > most) {
-        most = tiles;
-        take = first;
-      }
-    }
-    if (take == NBUCKETS || full) break;
-    OpenList& v = open[take];
-    uint32_t c = ST_LANES;
+  RingItem buf[ST_LANES];
+  // one tile of up to 64 items from the front of v (bucket q), cut where its messages would
+  // overflow a position's message slot
+  auto one = [&](OpenList& v, uint32_t q) {
+    uint32_t c = v.peek(buf, ST_LANES);
     uint64_t bytes = 0;
     for (uint32_t l = 0; l < c; l++) {
-      bytes += v[l].r->off[v[l].idx + 1] - v[l].r->off[v[l].idx];
+      bytes += buf[l].r->off[buf[l].idx + 1] - buf[l].r->off[buf[l].idx];
       if (l > 0 && sigops_host::stream_region_bytes(bytes) > ms) {
         c = l;
         break;
       }
     }
-    if (!plan(take, v.data(), c)) {
+    if (!plan(q, buf, c)) {
       full = true;
-      break;
+      return;
     }
     v.take(c);
+  };
+  // full tiles one at a time from the queue with the most of them (within a queue, its buckets in
+  // processing order), so that when positions run short every queue still gets its share and the
+  // wavefronts at home in it stay busy
+  auto full_tiles = [&](uint32_t qu, uint32_t& first) {
+    size_t t = 0;
+    first = NBUCKETS;
+    for (uint32_t r = 0; r < NBUCKETS; r++) {
+      const uint32_t b = res.bucket_order[r];
+      if (queue_of(b) != qu || open[b].size() < ST_LANES) continue;
+      t += open[b].size() / ST_LANES;
+      if (first == NBUCKETS) first = b;
+    }
+    return t;
+  };
+  size_t tiles[STREAM_QUEUES];
+  uint32_t first[STREAM_QUEUES];
+  for (uint32_t qu = 0; qu < STREAM_QUEUES; qu++) tiles[qu] = full_tiles(qu, first[qu]);
+  while (!full) {
+    uint32_t qu = 0;
+    for (uint32_t k = 1; k < STREAM_QUEUES; k++)
+      if (tiles[k] > tiles[qu]) qu = k;
+    if (!tiles[qu]) break;
+    one(open[first[qu]], first[qu]);
+    tiles[qu] = full_tiles(qu, first[qu]);
   }
   const uint64_t inflight =
       n_pub.load(std::memory_order_relaxed) + planned.size() - n_seen.load(std::memory_order_acquire);
   if (!full && inflight < W) {
-    drain(open[sigops::BUCKET_R1], sigops::BUCKET_R1, true);
-    drain(open[sigops::BUCKET_K1], sigops::BUCKET_K1, true);
+    // the remainders: one tile per secp bucket, the Ed25519 bins merged in increasing length order
+    while (!full && !open[sigops::BUCKET_R1].empty()) one(open[sigops::BUCKET_R1], sigops::BUCKET_R1);
+    while (!full && !open[sigops::BUCKET_K1].empty()) one(open[sigops::BUCKET_K1], sigops::BUCKET_K1);
     OpenList ed;
     uint32_t last = 0;
     for (uint32_t b = sigops::BUCKET_ED0; b < NBUCKETS; b++) {
-      if (!open[b].empty()) last = b;
-      ed.v.insert(ed.v.end(), open[b].data(), open[b].data() + open[b].size());
+      if (open[b].empty()) continue;
+      last = b;
+      for (const Run& x : open[b].runs) ed.push(x);
       open[b].clear();
     }
-    if (!ed.empty()) {
-      drain(ed, last, true);
-      // what did not fit goes back to the bins
-      for (size_t k = 0; k < ed.size(); k++) {
-        const RingItem& it = ed[k];
-        open[sigops_host::stream_bucket(2, it.r->off[it.idx + 1] - it.r->off[it.idx])].push(it);
-      }
-    }
+    while (!full && !ed.empty()) one(ed, last);
+    for (const Run& x : ed.runs) open[x.bucket].push(x);  // what did not fit goes back to its bin
   }
   if (!planned.empty()) pack_planned();
   if (n_pub.load(std::memory_order_relaxed) != before) publish_counts();
@@ -1131,6 +1128,24 @@ int64_t sigops_server_submit(sigops_server* S, const uint8_t* schemes, const uin
   }
   r->bytes = request_bytes(n, r->msgs.size());
   if (r->bytes > S->cfg.hbm_budget) return SIGOPS_E_ARG;
+  if (S->ring && n) {
+    // the ring's admission takes a request's items bucket by bucket: a counting sort here, on the
+    // submitting thread, and the result buffers
+    std::vector<uint8_t> bk(n);
+    uint32_t cnt[sigops::NBUCKETS] = {};
+    for (size_t i = 0; i < n; i++) {
+      bk[i] = (uint8_t)sigops_host::stream_bucket(r->schemes[i], r->off[i + 1] - r->off[i]);
+      cnt[bk[i]]++;
+    }
+    r->bstart[0] = 0;
+    for (uint32_t b = 0; b < sigops::NBUCKETS; b++) r->bstart[b + 1] = r->bstart[b] + cnt[b];
+    uint32_t at[sigops::NBUCKETS];
+    for (uint32_t b = 0; b < sigops::NBUCKETS; b++) at[b] = r->bstart[b];
+    r->order.resize(n);
+    for (size_t i = 0; i < n; i++) r->order[at[bk[i]]++] = (uint32_t)i;
+    r->out.assign(n * 64, 0);
+    r->status.assign(n, 0);
+  }
   std::lock_guard<std::mutex> g(S->mu);
   r->ticket = S->next_ticket++;
   S->tickets[r->ticket] = r;
