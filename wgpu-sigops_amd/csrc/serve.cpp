@@ -517,6 +517,7 @@ struct RingEngine {
     uint32_t p, bucket;
   };
   std::vector<Planned> planned;  // tiles placed in this publication round, packed together
+  std::vector<uint32_t> qbuckets[STREAM_QUEUES];  // each queue's buckets in processing order
 
   int init(sigops_server* srv);
   void release();
@@ -573,7 +574,8 @@ int RingEngine::init(sigops_server* srv) {
   qpos = (uint32_t*)(pin + reg.qpos);
   pos.resize(R);
   for (uint32_t p = R; p-- > 0;) free_pos.push_back(p);
-  pack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
+  pack_pool = std::make_unique<sigops_host::Pool>(std::min(8u, sigops_host::host_threads()));
+  for (uint32_t r = 0; r < NBUCKETS; r++) qbuckets[queue_of(res.bucket_order[r])].push_back(res.bucket_order[r]);
   unpack_pool = std::make_unique<sigops_host::Pool>(std::min(4u, sigops_host::host_threads()));
   for (int k = 0; k < 4; k++) {
     hipEvent_t e;
@@ -795,9 +797,8 @@ void RingEngine::publish() {
   auto full_tiles = [&](uint32_t qu, uint32_t& first) {
     size_t t = 0;
     first = NBUCKETS;
-    for (uint32_t r = 0; r < NBUCKETS; r++) {
-      const uint32_t b = res.bucket_order[r];
-      if (queue_of(b) != qu || open[b].size() < ST_LANES) continue;
+    for (uint32_t b : qbuckets[qu]) {
+      if (open[b].size() < ST_LANES) continue;
       t += open[b].size() / ST_LANES;
       if (first == NBUCKETS) first = b;
     }
