@@ -5,7 +5,8 @@ secp: r / s range violations, r that is not an x-coordinate, digests >= n and ==
 at infinity, and signatures crafted so that R is a small multiple of G and (u1, u2) are chosen
 values, which makes the windowed GPU ladders hit doubling / inverse-point additions.
 ed25519: s >= l, small-order and mixed-order public keys, non-canonical key encodings, the
-x = 0 / sign-bit key, undecodable keys, and SHA-512 padding-boundary message lengths.
+x = 0 / sign-bit key, undecodable keys, SHA-512 padding-boundary message lengths, and encodings
+of R (non-canonical y, sign bit on x = 0, order-4 points).
 """
 import hashlib
 import json
@@ -311,6 +312,21 @@ def ed_cases(seed=11):
     for L in (0, 1, 47, 48, 63, 64, 111, 112, 127, 128, 175, 176, 239, 240, 1000, 1024, 2047):
         m = bytes(rnd.randrange(256) for _ in range(L))
         out.append((f"len{L}", pyref.ed25519_sign(seedk, m), pk, m))
+    # encodings of R: with A the identity and s = 0, [s]B - [k]A is the identity for every k, so only
+    # the bytes of R decide: compress() yields y = 1 with the sign bit clear and nothing else
+    out.append(("R_identity", ident + bytes(32), ident, msg))
+    out.append(("R_identity_noncanonical", ident_nc + bytes(32), ident, msg))
+    out.append(("R_identity_signbit", bytes(ident_sign) + bytes(32), ident, msg))
+    # A of order 4 (y = 0) and s = 0: [s]B - [k]A is A, the identity or -A by k mod 4; R spelled with
+    # y = 0 or the non-canonical y = p, and either sign bit
+    T4 = pyref.e_mul(T8, 2)
+    A4 = pyref.e_compress(T4)
+    for i in range(4):
+        for yb, name in ((0, "y0"), (EP, "yp")):
+            for sgn in (0, 1):
+                Rb = bytearray(yb.to_bytes(32, "little"))
+                Rb[31] |= sgn << 7
+                out.append((f"R_order4_{name}_s{sgn}_{i}", bytes(Rb) + bytes(32), A4, msg + bytes([0xB0 + i])))
     return out
 
 
