@@ -654,7 +654,7 @@ def main():
     mac_sig = mixed_mac_per_sig(sch, mx.lens)
     kern_ms = stage_ms["verify_kernel"]
     shape = (f"waves={ctx.waves} blocks_per_cu={ctx.blocks_per_cu} tile_rounds={ctx.tile_rounds} "
-             f"bucket_order={ctx.bucket_order} n={n} seed={args.seed}")
+             f"bucket_order={ctx.bucket_order} g_window={ctx.g_window} n={n} seed={args.seed}")
     prof = read_counter_profile(sigops.source_digest(), shape)
     roof = roofline(n, mac_sig, kern_ms, stage_ms, prof)
     log(f"roofline: frac {roof['frac']}, achieved {roof['achieved']} of {roof['peak']} T lane-op slots/s; "
@@ -753,7 +753,7 @@ def main():
                                "(messages log-uniform 0..4096 B), interleaved, 5% corrupted; one graph-captured "
                                "persistent launch per step",
                    "n": n, "seed": args.seed, "waves_per_simd": ctx.waves, "blocks_per_cu": ctx.blocks_per_cu,
-                   "tile_rounds": ctx.tile_rounds, "bucket_order": ctx.bucket_order,
+                   "tile_rounds": ctx.tile_rounds, "bucket_order": ctx.bucket_order, "g_window": ctx.g_window,
                    "launch_shape_from": "autotune (tuned.json)" if tuned_shape else "arguments or library defaults"},
         "roofline": roof,
         "cpu_baseline": cpu,

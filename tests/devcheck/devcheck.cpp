@@ -88,8 +88,8 @@ int recover_one(const unsigned char* sig, const unsigned char* dig, unsigned cha
   if (pro.status != ST_OK) return (int)pro.status;
   bool exc = false;
   STabHost tab;
-  jac Q = secp_ladder<C>(pro, tab, gt, exc);
-  int careful = exc || force_careful;
+  jac Q = (force_careful & 2) ? secp_ladder<C, 9>(pro, tab, gt, exc) : secp_ladder<C>(pro, tab, gt, exc);
+  int careful = exc || (force_careful & 1);
   if (!careful) {
     secp_output<C>(Q, o);
     memcpy(out, o, 64);
@@ -114,8 +114,8 @@ int verify_one(const unsigned char* sig, const unsigned char* dig, const unsigne
   if (pro.status != ST_OK) return (int)pro.status;
   bool exc = false;
   STabHost tab;
-  jac P = secp_ladder<C>(pro, tab, gt, exc);
-  if (!exc && !force_careful) return (int)secp_verify_x<C>(P, sr);
+  jac P = (force_careful & 2) ? secp_ladder<C, 9>(pro, tab, gt, exc) : secp_ladder<C>(pro, tab, gt, exc);
+  if (!exc && !(force_careful & 1)) return (int)secp_verify_x<C>(P, sr);
   jacf Pc = secp_ladder_careful<C>(pro, gt);
   const int st = Pc.inf ? (int)ST_INFINITY : (int)secp_verify_x<C>(Pc.P, sr);
   return st | (exc ? 0x100 : 0);
@@ -165,25 +165,27 @@ unsigned dc_table(int which, const u32* limbs, unsigned lbits, u32* out) {
   return table_ed_host(limbs, lbits, out);
 }
 
-// the resident generator tables as uploaded to HBM (same layout as dc_table's output)
+// the resident generator tables as uploaded to HBM (same layout as dc_table's output; the secp
+// tables hold 256 entries, of which 8-bit windows use the first 128)
 void dc_resident_table(int which, u32* out) {
   if (which == 0) memcpy(out, K1_GTABLE, sizeof K1_GTABLE);
   else if (which == 1) memcpy(out, R1_GTABLE, sizeof R1_GTABLE);
   else memcpy(out, ED_BTABLE, sizeof ED_BTABLE);
 }
 
-// verification with a known key: status; bit 8 set when the fast ladder met an exceptional addition
+// verification with a known key: status; bit 8 set when the fast ladder met an exceptional addition.
+// flags (also of dc_secp_recover): bit 0 forces the careful path, bit 1 selects 9-bit G windows
 int dc_secp_verify(int curve, const unsigned char* sig, const unsigned char* dig, const unsigned char* pub,
-                   int force_careful) {
-  if (curve == 0) return verify_one<CurveK1>(sig, dig, pub, K1_GTABLE, force_careful);
-  return verify_one<CurveR1>(sig, dig, pub, R1_GTABLE, force_careful);
+                   int flags) {
+  if (curve == 0) return verify_one<CurveK1>(sig, dig, pub, K1_GTABLE, flags);
+  return verify_one<CurveR1>(sig, dig, pub, R1_GTABLE, flags);
 }
 
 // returns status; bit 8 set when the fast ladder met an exceptional addition
 int dc_secp_recover(int curve, const unsigned char* sig, const unsigned char* dig, unsigned char* out,
-                    int force_careful) {
-  if (curve == 0) return recover_one<CurveK1>(sig, dig, out, K1_GTABLE, force_careful);
-  return recover_one<CurveR1>(sig, dig, out, R1_GTABLE, force_careful);
+                    int flags) {
+  if (curve == 0) return recover_one<CurveK1>(sig, dig, out, K1_GTABLE, flags);
+  return recover_one<CurveR1>(sig, dig, out, R1_GTABLE, flags);
 }
 
 int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const unsigned char* msg, unsigned len,

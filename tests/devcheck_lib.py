@@ -45,9 +45,9 @@ def lib():
     return _lib
 
 
-def secp_recover(curve, sig, dig, force_careful=False):
+def secp_recover(curve, sig, dig, force_careful=False, g_window=8):
     out = ctypes.create_string_buffer(64)
-    st = lib().dc_secp_recover(curve, sig, dig, out, int(force_careful))
+    st = lib().dc_secp_recover(curve, sig, dig, out, int(force_careful) | (2 if g_window == 9 else 0))
     return st, out.raw
 
 
@@ -95,9 +95,9 @@ def glv_split(k):
     return (-h1 if sg & 1 else h1), (-h2 if sg & 2 else h2)
 
 
-def secp_verify(curve, sig, dig, pub, force_careful=False):
+def secp_verify(curve, sig, dig, pub, force_careful=False, g_window=8):
     """(status, careful path ran) of the device verification built for the host"""
-    st = lib().dc_secp_verify(curve, bytes(sig), bytes(dig), bytes(pub), int(force_careful))
+    st = lib().dc_secp_verify(curve, bytes(sig), bytes(dig), bytes(pub), int(force_careful) | (2 if g_window == 9 else 0))
     return st & 0xFF, bool(st >> 8)
 
 
@@ -113,7 +113,7 @@ def table(which, limbs, lbits):
 
 def resident_table(which):
     import numpy as np
-    out = np.zeros((128, 28 if which == 2 else 20), np.uint32)
+    out = np.zeros((128, 28) if which == 2 else (256, 20), np.uint32)
     lib().dc_resident_table(which, out.ctypes.data)
     return out
 

@@ -147,8 +147,38 @@ def test_autotune_variants_agree():
     assert {k: entry[k] for k in at.DEFAULT} in list(at.CANDIDATES)
 
 
-def test_batch_edge_cases(ctx):
-    """secp edge cases of both curves and the Ed25519 edge cases in one interleaved batch"""
+@pytest.fixture(scope="module")
+def ctx9():
+    c = sigops.Context(tuned=False, g_window=9)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx8():
+    c = sigops.Context(tuned=False, g_window=8)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("g_window", [8, 9])
+def test_batch_windows_match_oracle(ctx8, ctx9, g_window):
+    """the mixed-batch variants with 8- and 9-bit fixed-base windows against the oracle"""
+    n = 12000
+    _say(f"batch mixed, {g_window}-bit G windows, n={n}")
+    schemes, sigs, aux, msgs, off, out_exp, st_exp, idx, valid = _oracle_mixed(n, 901)
+    out, st = (ctx9 if g_window == 9 else ctx8).batch(schemes, sigs, aux, msgs, off)
+    sec = np.concatenate([idx[0], idx[1]])
+    assert np.array_equal(out[sec], out_exp[sec])
+    assert np.array_equal(st[sec], st_exp[sec])
+    assert np.array_equal(st[idx[2]] == 0, valid.astype(bool))
+
+
+@pytest.mark.parametrize("g_window", [8, 9])
+def test_batch_edge_cases(ctx8, ctx9, g_window):
+    """secp edge cases of both curves and the Ed25519 edge cases in one interleaved batch (fixed-base
+    windows of 8 and 9 bits)"""
+    ctx = ctx9 if g_window == 9 else ctx8
     items = []
     for curve in (0, 1):
         cases = edge_cases.secp_cases(curve)

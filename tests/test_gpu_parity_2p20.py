@@ -87,3 +87,25 @@ def test_mixed_batch_every_item_2p20(ctx):
     ref = O.verify_batch(sigs[ii], aux[ii], msgs, sub_off, nthreads=THREADS).astype(bool)
     assert np.array_equal(st_h[ii] == 0, ref)
     assert not out_h[ii].any()
+
+
+def test_mixed_batch_both_windows_2p20(ctx):
+    """the same 2^20 mixed batch through the variants with 8- and 9-bit fixed-base windows: every
+    output and status is the same (the test above compares the tuned variant with the oracle)"""
+    _say(f"mixed batch, 8- and 9-bit G windows, n={N}")
+    mx = W.mixed(ctx, N, seed=6060)
+    res = []
+    ctxs = [sigops.Context(tuned=False, g_window=w) for w in (8, 9)]
+    try:
+        for c in ctxs:
+            out = torch.zeros((N, 64), dtype=torch.uint8, device="cuda:0")
+            st = torch.full((N,), 0xEE, dtype=torch.uint8, device="cuda:0")
+            c.batch_dev(mx.schemes.data_ptr(), mx.sigs.data_ptr(), mx.aux.data_ptr(), mx.msgs.data_ptr(),
+                        mx.msg_off.data_ptr(), N, out.data_ptr(), st.data_ptr(), torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            res.append((out.cpu(), st.cpu()))
+    finally:
+        for c in ctxs:
+            c.close()
+    assert torch.equal(res[0][1], res[1][1]) and torch.equal(res[0][0], res[1][0])
+    assert int((res[1][1] == 0xEE).sum()) == 0

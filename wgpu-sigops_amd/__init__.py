@@ -40,7 +40,7 @@ class SigopsError(RuntimeError):
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("blocks_per_cu", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("tile_rounds", ctypes.c_uint32), ("no_graph", ctypes.c_uint32), ("waves_per_simd", ctypes.c_uint32),
-                ("bucket_order", ctypes.c_uint32)]
+                ("bucket_order", ctypes.c_uint32), ("g_window", ctypes.c_uint32)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -198,19 +198,22 @@ class Context:
     consecutive calls of at most max_batch items."""
 
     def __init__(self, device=0, blocks_per_cu=0, max_batch=0, tile_rounds=0, use_graph=True, tuned=True, waves=0,
-                 bucket_order=0):
+                 bucket_order=0, g_window=0):
         self._lock = threading.RLock()
         self.max_batch = max_batch or DEFAULT_MAX_BATCH
-        if tuned and not (blocks_per_cu or tile_rounds or waves or bucket_order):
+        if tuned and not (blocks_per_cu or tile_rounds or waves or bucket_order or g_window):
             # the variant the autotuner measured fastest on this device (autotune.py), if any
             t = _tuned(device)
             if t:
                 waves, tile_rounds, bucket_order = t.get("waves", 0), t.get("tile_rounds", 0), t.get("bucket_order", 0)
+                g_window = t.get("g_window", 0)
         self.waves = waves or 2
         self.blocks_per_cu = blocks_per_cu or self.waves
         self.tile_rounds = tile_rounds or 1
         self.bucket_order = bucket_order
-        cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1, waves, bucket_order)
+        self.g_window = g_window or 8
+        cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1, waves, bucket_order,
+                      g_window)
         self._h = lib().sigops_create(ctypes.byref(cfg))
         if not self._h:
             raise SigopsError("sigops_create failed (no usable GPU?)")

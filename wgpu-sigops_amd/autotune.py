@@ -8,6 +8,9 @@ Candidates (each a Context setting; the library holds every variant's code):
                   atomics but a coarser tail at the end of the batch);
   * bucket_order: the processing order of the buckets (0: secp256r1, secp256k1, Ed25519 by
                   decreasing block count; 1: secp256r1, Ed25519, secp256k1; 2: secp256k1 first).
+  * g_window:     the fixed-base window of the secp streams, 8 bits (128 odd multiples of G in LDS)
+                  or 9 (256, 1.75x the LDS; 4 fewer G additions per secp256k1 item and 3 per P-256
+                  item, on a ladder that takes one doubling per loop iteration).
 blocks_per_cu follows waves (one 256-thread workgroup per SIMD wave): more workgroups than fit
 only queue behind the resident ones.
 
@@ -20,8 +23,8 @@ beats the default by more than the larger of 0.5 % and both candidates' spread; 
 records the default.
 
 Fixed by design rather than tuned: 29-bit limbs (one v_mad_u64_u32 per partial product with an
-even-aligned 64-bit accumulator), windows of 4 bits for per-lane tables and 8 bits for the LDS
-generator tables.
+even-aligned 64-bit accumulator), windows of 4 bits for per-lane tables, 8 bits for Ed25519's LDS
+table.
 
 Run on a GPU:  python wgpu-sigops_amd/autotune.py [--n 1048576] [--steps 5] [--rounds 3]
 """
@@ -34,9 +37,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED_PATH = os.path.join(HERE, "tuned.json")
-DEFAULT = {"waves": 2, "tile_rounds": 1, "bucket_order": 0}
-CANDIDATES = (DEFAULT, {"waves": 1, "tile_rounds": 1, "bucket_order": 0}, {"waves": 2, "tile_rounds": 2, "bucket_order": 0},
-              {"waves": 2, "tile_rounds": 1, "bucket_order": 1}, {"waves": 2, "tile_rounds": 1, "bucket_order": 2})
+DEFAULT = {"waves": 2, "tile_rounds": 1, "bucket_order": 0, "g_window": 8}
+CANDIDATES = (DEFAULT, dict(DEFAULT, waves=1), dict(DEFAULT, tile_rounds=2), dict(DEFAULT, bucket_order=1),
+              dict(DEFAULT, bucket_order=2), dict(DEFAULT, g_window=9))
 
 
 def device_key(device=0):
@@ -57,7 +60,8 @@ def load(device=0, path=TUNED_PATH):
 
 
 def _name(c):
-    return f"waves {c['waves']} tile_rounds {c['tile_rounds']} bucket_order {c['bucket_order']}"
+    return (f"waves {c['waves']} tile_rounds {c['tile_rounds']} bucket_order {c['bucket_order']} "
+            f"g_window {c.get('g_window', 8)}")
 
 
 def tune(n=1 << 20, seed=2024, steps=5, rounds=3, candidates=CANDIDATES, write=True, path=TUNED_PATH, log=print):

@@ -466,11 +466,12 @@ def main():
             nl = 256 // lb + 1
             vals += limbs(pow(2, -nl * lb, p) * 2**522 % p)
         o.write(arr(f"REFCONV_{tag}", vals))
-    o.write("\n// host-side generator tables: entries of 20 u32 (x[9], y[9], pad[2]) for secp,\n")
+    o.write("\n// host-side generator tables: entries of 20 u32 (x[9], y[9], pad[2]) for secp (odd multiples 1..511),\n")
     o.write("// 28 u32 (ypx[9], ymx[9], t2d[9], pad) for ed25519; Montgomery form\n")
     ntab = 1 << (WG - 1)
+    # secp: odd multiples 1..511 of G, for fixed-base windows of 8 bits (the first 128) or 9
     for tag, G, p, a in [("K1", G_K1, P_K1, 0), ("R1", G_R1, P_R1, -3)]:
-        pts = odd_multiples_w(G, p, a, ntab)
+        pts = odd_multiples_w(G, p, a, 2 * ntab)
         vals = []
         for (x, y) in pts:
             vals += limbs(mont(x, p)) + limbs(mont(y, p)) + [0, 0]

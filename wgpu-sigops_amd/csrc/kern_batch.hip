@@ -154,12 +154,12 @@ DEV void store16(uint4* p, const u32 o[16]) {
   for (int c = 0; c < 4; c++) p[c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
 }
 
-template <class C, class GT>
+template <class C, int WGB, class GT>
 DEV void batch_secp_item(const BatchArgs& a, u32 i, STabDev& tab, const GT& gt) {
   u32 sr[16], dr[8], o[16];
   load_words(a.sigs + 4 * (u64)i, sr, 4);
   load_words(a.aux + 2 * (u64)i, dr, 2);
-  const u32 st = secp_item<C>(sr, dr, tab, gt, o);
+  const u32 st = secp_item<C, WGB>(sr, dr, tab, gt, o);
   if (st == ST_PENDING) a.pend_list[atomicAdd(a.pend_count, 1u)] = i;
   store16(a.out + 4 * (u64)i, o);
   a.status[i] = (u8)st;
@@ -174,19 +174,21 @@ constexpr u32 RUN_MIXED = RUN_K1 | RUN_R1 | RUN_ED;
 // WAVES: waves per SIMD the kernel is built for, which sets its register budget per lane (512 /
 // WAVES): 2 by default (256 registers, some values spilled to scratch memory); 1 lets the
 // compiler keep them in registers (accumulation registers included) with half the waves to hide
-// latencies. The autotuner measures both.
-template <u32 MASK, int WAVES = SIGOPS_WAVES_PER_EU>
+// latencies. WGB: the secp streams' fixed-base window, 8 bits (128 odd multiples of G staged in
+// LDS) or 9 (256: 63.5 KB of LDS per workgroup, still two per CU). The autotuner measures them.
+template <u32 MASK, int WAVES = SIGOPS_WAVES_PER_EU, int WGB = 8>
 __global__ __launch_bounds__(256, WAVES) void k_batch_run(BatchArgs a, const uint4* __restrict__ gk1,
                                                    const uint4* __restrict__ gr1, const uint4* __restrict__ gb,
                                                    uint4* __restrict__ lane_tab, u32 slots) {
   static_assert(!((MASK & RUN_ED) && (MASK & RUN_ED_STRICT)), "one Ed25519 rule set per launch");
   constexpr bool K1 = MASK & RUN_K1, R1 = MASK & RUN_R1, ED = MASK & (RUN_ED | RUN_ED_STRICT);
+  constexpr u32 GN = WGB == 9 ? GTAB9_ENTRIES : GTAB_ENTRIES;
   bind_io(a);
-  __shared__ uint4 s_k1[K1 ? GTAB_ENTRIES * gtab_chunks<CurveK1>() : 1];
-  __shared__ uint4 s_r1[R1 ? GTAB_ENTRIES * gtab_chunks<CurveR1>() : 1];
+  __shared__ uint4 s_k1[K1 ? GN * gtab_chunks<CurveK1>() : 1];
+  __shared__ uint4 s_r1[R1 ? GN * gtab_chunks<CurveR1>() : 1];
   __shared__ uint4 s_b[ED ? BTAB_ENTRIES * 7 : 1];
-  if constexpr (K1) stage_gtab<CurveK1>(s_k1, gk1);
-  if constexpr (R1) stage_gtab<CurveR1>(s_r1, gr1);
+  if constexpr (K1) stage_gtab<CurveK1, GN>(s_k1, gk1);
+  if constexpr (R1) stage_gtab<CurveR1, GN>(s_r1, gr1);
   if constexpr (ED)
     for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
   __syncthreads();
@@ -216,9 +218,9 @@ __global__ __launch_bounds__(256, WAVES) void k_batch_run(BatchArgs a, const uin
       const u32 i = a.order[(u64)t * a.tile_items + k + lane];
       if (i == DUMMY) continue;
       if (K1 && b == BUCKET_K1) {
-        if constexpr (K1) batch_secp_item<CurveK1>(a, i, jt, GTabOf<CurveK1>{s_k1});
+        if constexpr (K1) batch_secp_item<CurveK1, WGB>(a, i, jt, GTabOf<CurveK1>{s_k1});
       } else if (R1 && b == BUCKET_R1) {
-        if constexpr (R1) batch_secp_item<CurveR1>(a, i, jt, GTabOf<CurveR1>{s_r1});
+        if constexpr (R1) batch_secp_item<CurveR1, WGB>(a, i, jt, GTabOf<CurveR1>{s_r1});
       } else if constexpr (ED) {
         // the challenge is hashed here, inside the persistent kernel: the tile's items have equal
         // SHA-512 block counts (bucketed), and the hash's memory latency hides behind the other
@@ -328,6 +330,9 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   if (L.waves == 1)
     k_batch_run<RUN_MIXED, 1><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                                     (uint4*)L.lane_tab, L.slots);
+  else if (L.g_window == 9)
+    k_batch_run<RUN_MIXED, 2, 9><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1,
+                                                       (const uint4*)L.gb, (uint4*)L.lane_tab, L.slots);
   else
     k_batch_run<RUN_MIXED><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
                                                  (uint4*)L.lane_tab, L.slots);

@@ -8,6 +8,7 @@ namespace sigops {
 typedef uint8_t u8;
 
 constexpr uint32_t GTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of G
+constexpr uint32_t GTAB9_ENTRIES = 1u << (9 - 1);  // odd multiples 1..511 of G (9-bit windows; the HBM table's size)
 constexpr uint32_t BTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of B
 constexpr uint32_t SECP_LANE_TAB_BYTES = 8 * 9 * 16;  // 8 entries x 9 uint4 (Jacobian + Z product, then affine)
 constexpr uint32_t ED_LANE_TAB_BYTES = 8 * 9 * 16;    // 8 cached entries x 9 uint4
@@ -61,6 +62,7 @@ struct BatchLaunch {
   void* lane_tab;
   uint32_t slots, grid, aux_grid, tile_rounds;
   uint32_t waves;        // mixed batch: waves per SIMD the persistent kernel is built for (1 or 2)
+  uint32_t g_window;     // mixed batch: fixed-base window of the secp streams, 8 or 9 bits (9: waves 2 only)
   hipEvent_t* stage_ev;  // null, or 5 events recorded before, between and after the 4 stages
 };
 

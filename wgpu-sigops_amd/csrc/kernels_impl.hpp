@@ -61,14 +61,15 @@ using GTabOf = GTabLdsN<C::GLV ? 7 : 5>;
 template <class C>
 constexpr int gtab_chunks() { return C::GLV ? 7 : 5; }
 
-// copies the HBM table g (5 chunks per entry) into LDS s in curve C's layout, computing beta x
-// for secp256k1 (one product per entry, GTAB_ENTRIES per workgroup); the caller syncs afterwards
-template <class C>
+// copies the first N entries of the HBM table g (5 chunks per entry) into LDS s in curve C's
+// layout, computing beta x for secp256k1 (one product per entry, N per workgroup); the caller
+// syncs afterwards
+template <class C, u32 N = GTAB_ENTRIES>
 DEV void stage_gtab(uint4* s, const uint4* g) {
   if constexpr (C::GLV) {
     typedef typename C::Fp F;
     const fe beta = fe_from_const(K1_BETA);
-    for (u32 e = threadIdx.x; e < GTAB_ENTRIES; e += blockDim.x) {
+    for (u32 e = threadIdx.x; e < N; e += blockDim.x) {
       u32 w[28];
 #pragma unroll
       for (int c = 0; c < 5; c++) {
@@ -86,7 +87,7 @@ DEV void stage_gtab(uint4* s, const uint4* g) {
       for (int c = 0; c < 7; c++) s[e * 7 + c] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
     }
   } else {
-    for (u32 t = threadIdx.x; t < GTAB_ENTRIES * 5; t += blockDim.x) s[t] = g[t];
+    for (u32 t = threadIdx.x; t < N * 5; t += blockDim.x) s[t] = g[t];
   }
 }
 
@@ -374,14 +375,15 @@ static __global__ void k_fill_u32(u32* p, u32 v, u32 n) {
 
 // one recovery on the fast ladder: the status (ST_PENDING when the ladder met an exceptional
 // addition and the careful pass must redo the item) and the key X || Y (zeros unless ST_OK)
-template <class C, class GT>
+// (WGB: the fixed-base window, 8 or 9 bits, which the table gt must match)
+template <class C, int WGB = 8, class GT>
 DEV u32 secp_item(const u32 sr[16], const u32 dr[8], STabDev& tab, const GT& gt, u32 o[16]) {
   secp_pro pro;
   secp_prologue<C>(sr, dr, pro);
   u32 st = pro.status;
   if (st == ST_OK) {
     bool exc = false;
-    const jac Q = secp_ladder<C>(pro, tab, gt, exc);
+    const jac Q = secp_ladder<C, WGB>(pro, tab, gt, exc);
     if (exc)
       st = ST_PENDING;
     else

@@ -455,6 +455,119 @@ DEV u32 secp_verify_x(const jac& P, const u32 sig_raw[16]) {
 
 DEV u32 nibble(const u32 c[8], int i) { return (c[i >> 3] >> ((i & 7) * 4)) & 15u; }
 DEV u32 byte8(const u32 c[8], int j) { return (c[j >> 2] >> ((j & 3) * 8)) & 255u; }
+// 9-bit digit j of a carrier (bits 9j .. 9j + 8, which may straddle two words)
+DEV u32 digit9(const u32* c, int j) {
+  const int b = 9 * j, w = b >> 5, sh = b & 31;
+  u32 v = c[w] >> sh;
+  if (sh > 23) v |= c[w + 1] << (32 - sh);
+  return v & 511u;
+}
+
+// fixed-base windows of 9 bits: 256 odd multiples of G (LDS 1.75x larger, 4 fewer G additions per
+// secp256k1 item and 3 per P-256 item). The carriers come from the 8-bit ones: a carrier is
+// (h >> 1) | 2^(B-1) for the odd h it spells, so moving the marker bit re-spells h over B' bits.
+constexpr int GLV_BG9 = 135;  // G streams of secp256k1: 15 windows of 9 bits
+constexpr int R1_BG9 = 261;   // the G stream of P-256: 29 windows of 9 bits
+
+// acc + the odd multiple of the G table's point that 9-bit digit d spells (negated per digit and
+// stream sign); PHI: the entry read as phi(entry) = (beta x, y)
+template <class C, bool PHI, class GTab>
+DEV jac g_add9(const jac& acc, const GTab& gt, u32 d, bool neg_stream, bool& exc) {
+  typedef typename C::Fp F;
+  fe gx, gy;
+  const u32 idx = ((d < 256) ? ~d : d) & 255u;
+  if constexpr (PHI)
+    gt.load_phi(idx, gx, gy);
+  else
+    gt.load(idx, gx, gy);
+  return jmadd<C>(acc, gx, fneg_if<F>((d < 256) != neg_stream, gy), exc);
+}
+
+// GLV ladder with 9-bit G windows (see secp_ladder_glv): one doubling per iteration over bits
+// 127..0; the R digits enter at multiples of 4, the G digits at multiples of 9
+template <class C, class Tab, class GTab>
+DEV jac secp_ladder_glv9(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
+  typedef typename C::Fp F;
+  u32 cg1[5], cg2[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    cg1[i] = o.cg1[i];
+    cg2[i] = o.cg2[i];
+  }
+  // marker bit GLV_BG - 1 = 135 -> GLV_BG9 - 1 = 134 (the halves are below 2^130)
+  cg1[4] = (cg1[4] & ~(1u << 7)) | (1u << 6);
+  cg2[4] = (cg2[4] & ~(1u << 7)) | (1u << 6);
+  jac acc;
+  {
+    fe x, y;
+    tab.load_affine(nibble(o.cr1, 32) - 8, x, y);
+    acc.X = fnorm_red<F>(x);
+    acc.Y = fnorm_red<F>(fneg_if<F>(o.sr1, y));
+    acc.Z = fe_from_const(F::ONE);
+  }
+  {
+    fe bx, ey;
+    tab.load_affine_phi(nibble(o.cr2, 32) - 8, bx, ey);
+    acc = jmadd<C>(acc, bx, fneg_if<F>(o.sr2, ey), exc);
+  }
+  fe x1, y1;
+  u32 c1 = 0;
+#pragma unroll 1
+  for (int p = 127; p >= 0; p--) {
+    if ((p & 3) == 3) {  // the window's first table entry, loaded ahead of its four doublings
+      c1 = nibble(o.cr1, p >> 2);
+      tab.load_affine((c1 < 8 ? ~c1 : c1) & 7u, x1, y1);
+    }
+    acc = jdbl<C>(acc);
+    if ((p & 3) == 0) {
+      const u32 c2 = nibble(o.cr2, p >> 2);
+      fe x2, y2;
+      tab.load_affine_phi((c2 < 8 ? ~c2 : c2) & 7u, x2, y2);  // x2 = beta x
+      acc = jmadd<C>(acc, x1, fneg_if<F>((c1 < 8) != o.sr1, y1), exc);
+      acc = jmadd<C>(acc, x2, fneg_if<F>((c2 < 8) != o.sr2, y2), exc);
+    }
+    if (p % 9 == 0) {
+      acc = g_add9<C, false>(acc, gt, digit9(cg1, p / 9), o.sg1, exc);
+      acc = g_add9<C, true>(acc, gt, digit9(cg2, p / 9), o.sg2, exc);
+    }
+  }
+  return acc;
+}
+
+// P-256 ladder with 9-bit G windows (see secp_ladder): one doubling per iteration over bits
+// 251..0; the R digits enter at multiples of 4, the G digits at multiples of 9 (the top one, at
+// bit 252, with the top R digit)
+template <class C, class Tab, class GTab>
+DEV jac secp_ladder9(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
+  typedef typename C::Fp F;
+  u32 cg[9];
+#pragma unroll
+  for (int i = 0; i < 8; i++) cg[i] = o.c1[i];
+  cg[7] &= 0x7fffffffu;  // marker bit 255 -> R1_BG9 - 1 = 260 (u1 < n < 2^256)
+  cg[8] = 1u << (R1_BG9 - 1 - 256);
+  jac acc;
+  {
+    fe x, y;
+    tab.load_affine(nibble(o.c2, 63) - 8, x, y);
+    acc.X = fnorm_red<F>(x);
+    acc.Y = fnorm_red<F>(y);
+    acc.Z = fe_from_const(F::ONE);
+  }
+  acc = g_add9<C, false>(acc, gt, digit9(cg, 28), o.g_neg, exc);
+  fe ex, ey;
+  u32 c = 0;
+#pragma unroll 1
+  for (int p = 251; p >= 0; p--) {
+    if ((p & 3) == 3) {
+      c = nibble(o.c2, p >> 2);
+      tab.load_affine((c < 8 ? ~c : c) & 7u, ex, ey);
+    }
+    acc = jdbl<C>(acc);
+    if ((p & 3) == 0) acc = jmadd<C>(acc, ex, fneg_if<F>(c < 8, ey), exc);
+    if (p % 9 == 0) acc = g_add9<C, false>(acc, gt, digit9(cg, p / 9), o.g_neg, exc);
+  }
+  return acc;
+}
 
 // GLV ladder (secp256k1): Q = h1 G + h2 phi(G) + h3 R + h4 phi(R) on one doubling chain of 132
 // bits, with phi(x, y) = (beta x, y): both tables store beta x beside x (Tab::load_affine_phi,
@@ -564,11 +677,18 @@ DEV void secp_build_table(const secp_pro& o, Tab& tab, bool& exc) {
 }
 
 // R-table stored per lane (secp_build_table). GTab provides load(idx, fe& x, fe& y) for the odd
-// multiples (2 idx + 1) G.
-template <class C, class Tab, class GTab>
+// multiples (2 idx + 1) G: 128 of them for fixed-base windows of WGB = 8 bits, 256 for 9.
+template <class C, int WGB = 8, class Tab, class GTab>
 DEV jac secp_ladder(const secp_pro& o, Tab& tab, const GTab& gt, bool& exc) {
   typedef typename C::Fp F;
+  static_assert(WGB == 8 || WGB == 9, "fixed-base windows of 8 or 9 bits");
   secp_build_table<C>(o, tab, exc);
+  if constexpr (WGB == 9) {
+    if constexpr (C::GLV)
+      return secp_ladder_glv9<C>(o, tab, gt, exc);
+    else
+      return secp_ladder9<C>(o, tab, gt, exc);
+  }
   if constexpr (C::GLV) return secp_ladder_glv<C>(o, tab, gt, exc);
   // top R digit is always positive (C's bit 255 is set); the accumulator enters jmadd as a
   // subtrahend, so its coordinates are folded to < (1 + 2^-19) M

@@ -64,6 +64,7 @@ struct sigops_ctx {
   // mixed batch
   uint32_t tile_rounds = 1;
   uint32_t waves = 2;  // waves per SIMD the mixed batch's persistent kernel is built for
+  uint32_t g_window = 8;  // the mixed batch's fixed-base window (secp streams), 8 or 9 bits
   bool use_graph = true;
   uint8_t* d_rank = nullptr;
   uint32_t* d_work = nullptr;
@@ -341,12 +342,15 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
   int cus = 0;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)) != hipSuccess)
     return bad(e, "hipDeviceGetAttribute");
-  if (cfg->waves_per_simd > 2 || cfg->bucket_order > 2) {
-    fprintf(stderr, "sigops_create: waves_per_simd must be 0..2 and bucket_order 0..2\n");
+  if (cfg->waves_per_simd > 2 || cfg->bucket_order > 2 || (cfg->g_window && cfg->g_window != 8 && cfg->g_window != 9) ||
+      (cfg->g_window == 9 && cfg->waves_per_simd == 1)) {
+    fprintf(stderr,
+            "sigops_create: waves_per_simd must be 0..2, bucket_order 0..2 and g_window 0, 8 or 9 (9 with 2 waves)\n");
     sigops_destroy(c);
     return nullptr;
   }
   c->waves = cfg->waves_per_simd ? cfg->waves_per_simd : 2;
+  c->g_window = cfg->g_window ? cfg->g_window : 8;
   const uint32_t bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : c->waves;
   c->grid = (uint32_t)cus * bpc;
   c->slots = c->grid * 256;
@@ -659,6 +663,7 @@ int sigops_batch_dev(sigops_ctx* c, const void* d_schemes, const void* d_sigs, c
   L.aux_grid = std::min<uint32_t>(c->grid, (uint32_t)((n + 255) / 256));
   L.tile_rounds = c->tile_rounds;
   L.waves = c->waves;
+  L.g_window = c->g_window;
   L.stage_ev = nullptr;
   if (c->stage_timing) {
     L.stage_ev = c->stage_ev;
