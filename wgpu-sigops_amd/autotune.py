@@ -43,10 +43,12 @@ CANDIDATES = (DEFAULT, dict(DEFAULT, waves=1), dict(DEFAULT, tile_rounds=2), dic
 
 
 def device_key(device=0):
+    """the ISA target (with its feature flags) and the CU count; not the marketing name, which the
+    runtime reads from a table that some hosts lack"""
     import torch
 
     p = torch.cuda.get_device_properties(device)
-    return f"{getattr(p, 'gcnArchName', '?')}|{p.name}|{p.multi_processor_count}"
+    return f"{getattr(p, 'gcnArchName', '?')}|{p.multi_processor_count}"
 
 
 def load(device=0, path=TUNED_PATH):
