@@ -155,3 +155,15 @@ def test_inputs_above_max_batch_are_split():
         ko, kt = O.recover_batch(0, ks, kd)
         assert np.array_equal(out[: n // 2], ko) and np.array_equal(bst[: n // 2], kt)
         assert np.array_equal(bst[n // 2:] == 0, valid[n // 2:])
+
+
+def test_context_refuses_unknown_variants():
+    """sigops_create refuses a fixed-base window other than 8 or 9 bits, and 9-bit windows with the
+    one-wave kernel (that variant is not built); the valid combinations create contexts"""
+    for kw in ({"g_window": 7}, {"g_window": 10}, {"g_window": 9, "waves": 1}, {"waves": 3}, {"bucket_order": 3}):
+        with pytest.raises(sigops.SigopsError):
+            sigops.Context(tuned=False, max_batch=1024, **kw)
+    for kw in ({"g_window": 8}, {"g_window": 9}, {"g_window": 8, "waves": 1}):
+        c = sigops.Context(tuned=False, max_batch=1024, **kw)
+        assert c.g_window == kw["g_window"]
+        c.close()
