@@ -66,7 +66,7 @@ struct STabHost {
   void load_affine_phi(u32 e, fe& bx, fe& y) const { bx = abx[e]; y = ay[e]; }
 };
 struct CTabHost {
-  cached t[8];
+  cached t[16];
   void store(int e, const cached& p) { t[e] = p; }
   cached load(u32 e) const { return t[e]; }
 };
@@ -152,13 +152,15 @@ u32 table_ed_host(const u32* limbs, u32 lbits, u32* out) {
   }
   for (u32 j = 0; j < 128; j++)
     tab_ed_build_entry(j, p0, j < 16 ? xs[j] : fe_zero(), j < 16 ? ys[j] : fe_zero(), bad, out + 28 * j);
+  for (u32 j = 0; j < 128; j++) tab_ed_build_hi_entry(j, p0, out + 28 * (128 + j));
   return bad;
 }
 }  // namespace
 
 extern "C" {
 // caller-table conversion (host build of kern_table.hip): returns the TAB_BAD_* bits, writes the
-// 128-entry table (20 words per secp entry, 28 per Ed25519 entry); which: 0 k1, 1 r1, 2 ed25519
+// 128-entry table (20 words per secp entry), or for Ed25519 (28 words per entry) the 128 odd
+// multiples of P0 followed by the 128 of [2^136] P0; which: 0 k1, 1 r1, 2 ed25519
 unsigned dc_table(int which, const u32* limbs, unsigned lbits, u32* out) {
   if (which == 0) return table_secp_host<CurveK1>(limbs, lbits, out);
   if (which == 1) return table_secp_host<CurveR1>(limbs, lbits, out);
@@ -188,8 +190,22 @@ int dc_secp_recover(int curve, const unsigned char* sig, const unsigned char* di
   return recover_one<CurveR1>(sig, dig, out, R1_GTABLE, flags);
 }
 
+// the lattice reduction of the Ed25519 check: returns 1 when a pair within the bounds was found;
+// c0 (5 words, magnitude), c1 (5 words), *c0_neg
+int dc_ed_lattice(const u32* k, u32* c0, u32* c1, int* c0_neg) {
+  bool neg = false;
+  const bool ok = ed_lattice(k, c0, c1, neg);
+  *c0_neg = neg;
+  return ok;
+}
+
+// 1: every Ed25519 item takes the full-length check (the lattice reduction reports failure)
+void dc_set_ed_force_full(int on) { ed_force_full() = on != 0; }
+
+// flags: bit 0 verify_strict rules, bit 1 the full-length check only (the mixed-scheme kernels'
+// form); otherwise the lattice-reduced check of the Ed25519-only kernels
 int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const unsigned char* msg, unsigned len,
-                      int strict) {
+                      int flags) {
   u32 sr[16], pr[8];
   memcpy(sr, sig, 64);
   memcpy(pr, pk, 32);
@@ -199,8 +215,13 @@ int dc_ed25519_verify(const unsigned char* sig, const unsigned char* pk, const u
   MemHost mem{buf};
   CTabHost tab;
   BTabHost bt;
-  u32 st = strict ? ed25519_verify_item<true>(sr, pr, mem, 0, len, tab, bt)
-                  : ed25519_verify_item<false>(sr, pr, mem, 0, len, tab, bt);
+  u32 st;
+  if (flags & 2)
+    st = (flags & 1) ? ed25519_verify_item<true, false>(sr, pr, mem, 0, len, tab, bt)
+                     : ed25519_verify_item<false, false>(sr, pr, mem, 0, len, tab, bt);
+  else
+    st = (flags & 1) ? ed25519_verify_item<true, true>(sr, pr, mem, 0, len, tab, bt)
+                     : ed25519_verify_item<false, true>(sr, pr, mem, 0, len, tab, bt);
   delete[] buf;
   return (int)st;
 }

@@ -41,6 +41,8 @@ def lib():
         L.dc_table.restype = ctypes.c_uint
         L.dc_resident_table.argtypes = [ctypes.c_int, ctypes.c_void_p]
         L.dc_sha256_64.argtypes = [c_p, c_p]
+        L.dc_ed_lattice.argtypes = [u32p, u32p, u32p, ctypes.POINTER(ctypes.c_int)]
+        L.dc_set_ed_force_full.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -51,8 +53,26 @@ def secp_recover(curve, sig, dig, force_careful=False, g_window=8):
     return st, out.raw
 
 
-def ed25519_verify(sig, pk, msg, strict=False):
-    return lib().dc_ed25519_verify(sig, pk, msg, len(msg), int(strict))
+def ed25519_verify(sig, pk, msg, strict=False, lattice=True):
+    """status of the device verification built for the host: the lattice-reduced check of the
+    Ed25519-only kernels, or (lattice=False) the full-length check of the mixed-scheme kernels"""
+    return lib().dc_ed25519_verify(sig, pk, msg, len(msg), int(strict) | (0 if lattice else 2))
+
+
+def ed_lattice(k):
+    """the device's lattice reduction for challenge k: (found, c0, c1) with c0 signed"""
+    kw = (ctypes.c_uint32 * 8)(*[(k >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+    c0 = (ctypes.c_uint32 * 5)()
+    c1 = (ctypes.c_uint32 * 5)()
+    neg = ctypes.c_int(0)
+    ok = lib().dc_ed_lattice(kw, c0, c1, ctypes.byref(neg))
+    v0 = sum(int(c0[i]) << (32 * i) for i in range(5))
+    v1 = sum(int(c1[i]) << (32 * i) for i in range(5))
+    return bool(ok), -v0 if neg.value else v0, v1
+
+
+def set_ed_force_full(on):
+    lib().dc_set_ed_force_full(int(on))
 
 
 def field(op, mod, a, b=0):
@@ -102,18 +122,18 @@ def secp_verify(curve, sig, dig, pub, force_careful=False, g_window=8):
 
 
 def table(which, limbs, lbits):
-    """caller-table conversion (0 k1, 1 r1, 2 ed25519): (TAB_BAD_* bits, (128, 20|28) uint32 table)"""
+    """caller-table conversion (0 k1, 1 r1, 2 ed25519): (TAB_BAD_* bits, (128, 20) uint32 table, or
+    (256, 28) for Ed25519: the odd multiples of the point, then those of [2^136] times it)"""
     import numpy as np
-    width = 28 if which == 2 else 20
     limbs = np.ascontiguousarray(limbs, dtype=np.uint32)
-    out = np.zeros((128, width), np.uint32)
+    out = np.zeros((256, 28) if which == 2 else (128, 20), np.uint32)
     bad = lib().dc_table(which, limbs.ctypes.data, lbits, out.ctypes.data)
     return bad, out
 
 
 def resident_table(which):
     import numpy as np
-    out = np.zeros((128, 28) if which == 2 else (256, 20), np.uint32)
+    out = np.zeros((256, 28) if which == 2 else (256, 20), np.uint32)
     lib().dc_resident_table(which, out.ctypes.data)
     return out
 

@@ -5,8 +5,9 @@ secp: r / s range violations, r that is not an x-coordinate, digests >= n and ==
 at infinity, and signatures crafted so that R is a small multiple of G and (u1, u2) are chosen
 values, which makes the windowed GPU ladders hit doubling / inverse-point additions.
 ed25519: s >= l, small-order and mixed-order public keys, non-canonical key encodings, the
-x = 0 / sign-bit key, undecodable keys, SHA-512 padding-boundary message lengths, and encodings
-of R (non-canonical y, sign bit on x = 0, order-4 points).
+x = 0 / sign-bit key, undecodable keys, SHA-512 padding-boundary message lengths, encodings of R
+(non-canonical y, sign bit on x = 0, order-4 points), and challenges that the lattice-reduced
+check hands to its full-length check (tests/golden/ed25519_long_lattice.json).
 """
 import hashlib
 import json
@@ -327,6 +328,12 @@ def ed_cases(seed=11):
                 Rb = bytearray(yb.to_bytes(32, "little"))
                 Rb[31] |= sgn << 7
                 out.append((f"R_order4_{name}_s{sgn}_{i}", bytes(Rb) + bytes(32), A4, msg + bytes([0xB0 + i])))
+    # challenges without a short lattice pair: the Ed25519-only kernels' lattice-reduced check hands
+    # them to its full-length check (tests/golden/make_ed25519_long_lattice.py found them)
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ed25519_long_lattice.json")) as f:
+        for j, it in enumerate(json.load(f)["items"]):
+            out.append((f"long_lattice_{j}_{it['name']}", bytes.fromhex(it["sig"]), bytes.fromhex(it["pk"]),
+                        bytes.fromhex(it["msg"])))
     return out
 
 

@@ -178,17 +178,60 @@ def test_recover_random_and_edges(curve, g_window):
         assert (st & 0xFF, out) == O.secp_recover(curve, bytes(sigs[i]), bytes(dig[i]))
 
 
-def test_ed25519_random_and_edges():
+@pytest.mark.parametrize("lattice", [True, False])
+def test_ed25519_random_and_edges(lattice):
+    """both forms of the check: lattice-reduced (Ed25519-only kernels) and full-length (mixed)"""
     cases = edge_cases.ed_cases()
     for (name, sig, pk, m), exp in zip(cases, edge_cases.ed_expected(cases)):
-        assert (D.ed25519_verify(sig, pk, m) == 0) == exp, name
+        assert (D.ed25519_verify(sig, pk, m, lattice=lattice) == 0) == exp, name
     rng = np.random.default_rng(4)
     lens = rng.integers(0, 300, 48)
     off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
     sigs, pks, msgs, cor = O.gen_ed25519(8, 48, off, corrupt_ppm=200000)
     for i in range(48):
         m = bytes(msgs[off[i]:off[i + 1]])
-        assert (D.ed25519_verify(bytes(sigs[i]), bytes(pks[i]), m) == 0) == (not cor[i])
+        assert (D.ed25519_verify(bytes(sigs[i]), bytes(pks[i]), m, lattice=lattice) == 0) == (not cor[i])
+
+
+def test_ed25519_lattice_pairs():
+    """the lattice reduction of the Ed25519 check: every pair it returns has c0 = c1 k (mod 8l), c1 odd
+    and positive, both below 2^136 (checked with Python integers); it finds one for every k below
+    2^128 (the pair (k, 1)) and for all but a few in 10^5 random k (about 3 in 10^6 have none)"""
+    N8 = 8 * pyref.EL
+    rnd = random.Random(21)
+    ks = [0, 1, 2, 3, 8, 2**127, 2**128 - 1, pyref.EL - 1, pyref.EL - 2, (pyref.EL - 1) // 2, 2**252 - 1,
+          N8 // 16, N8 // 24, 3 * 2**250]
+    ks += [rnd.randrange(pyref.EL) for _ in range(40000)]
+    ks += [rnd.randrange(2**rnd.randrange(1, 253)) for _ in range(2000)]
+    random_misses = 0
+    for j, k in enumerate(ks):
+        ok, c0, c1 = D.ed_lattice(k)
+        if ok:
+            assert c1 > 0 and c1 & 1 and c1 < 2**136 and abs(c0) < 2**136, k
+            assert (c0 - c1 * k) % N8 == 0, k
+        else:
+            assert k >= 2**128, k
+            random_misses += 14 <= j < 40014
+    assert random_misses <= 3
+
+
+def test_ed25519_full_length_check():
+    """the full-length check that takes over when the reduction finds no pair: forced for every item,
+    and reached on its own by the fixture's challenges"""
+    cases = edge_cases.ed_cases()
+    exp = edge_cases.ed_expected(cases)
+    D.set_ed_force_full(True)
+    try:
+        for (name, sig, pk, m), e in zip(cases, exp):
+            assert (D.ed25519_verify(sig, pk, m) == 0) == e, name
+    finally:
+        D.set_ed_force_full(False)
+    long = [(c, e) for c, e in zip(cases, exp) if c[0].startswith("long_lattice_")]
+    assert len(long) >= 8
+    for (name, sig, pk, m), e in long:
+        k = int.from_bytes(hashlib.sha512(sig[:32] + pk + m).digest(), "little") % pyref.EL
+        assert not D.ed_lattice(k)[0], name
+        assert (D.ed25519_verify(sig, pk, m) == 0) == e, name
 
 
 def test_ed25519_strict_cases():
@@ -200,6 +243,7 @@ def test_ed25519_strict_cases():
         st = D.ed25519_verify(sig, pk, m, strict=True)
         codes.add(st)
         assert (st == 0) == exp == O.ed25519_verify_strict(sig, pk, m), name
+        assert D.ed25519_verify(sig, pk, m, strict=True, lattice=False) == st, name
         assert (D.ed25519_verify(sig, pk, m) == 0) == O.ed25519_verify(sig, pk, m), name
     assert {0, 1, 2, 3, 4, 5} <= codes
 
@@ -218,7 +262,7 @@ def test_caller_tables_canonical(lbits):
     for which, tab in ((0, P.secp256k1_bases(lbits)), (1, P.secp256r1_bases(lbits)), (2, P.ed25519_bases(lbits))):
         bad, out = D.table(which, tab, lbits)
         assert bad == 0, (which, lbits)
-        assert np.array_equal(out, D.resident_table(which)[:128]), (which, lbits)
+        assert np.array_equal(out, D.resident_table(which)[:len(out)]), (which, lbits)
 
 
 def test_caller_tables_other_points_and_bad_tables():

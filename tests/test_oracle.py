@@ -8,7 +8,7 @@ Each part of the oracle is checked against something independent of its own code
   - secp256k1 recovery: the reference's known-answer vector (src/curve_algos/secp256k1_ecdsa.rs:264-322),
     OpenSSL-made signatures, the round trip recover(sign(sk)) == sk*G (src/tests/secp256k1_ecdsa.rs:94-100),
     and 158 hand-built edge cases per curve against pyref;
-  - Ed25519: RFC 8032 section 7.1 vectors, OpenSSL signatures, 109 edge cases against pyref.
+  - Ed25519: RFC 8032 section 7.1 vectors, OpenSSL signatures, 117 edge cases against pyref.
 """
 import hashlib
 import json

@@ -17,6 +17,7 @@ NLIMBS = 9
 RBITS = LIMB_BITS * NLIMBS  # 261
 MASK = (1 << LIMB_BITS) - 1
 WG = 8  # fixed-base window (odd multiples 1..2^W-1)
+ED_LAT_BITS = 136  # Ed25519 lattice-reduced check: bound on its short scalars (34 windows of 4 bits)
 
 P_K1 = 2**256 - 2**32 - 977
 N_K1 = 0xFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFEBAAEDCE6AF48A03BBFD25E8CD0364141
@@ -397,7 +398,8 @@ def sha256_pad_kw(bitlen):
 def main():
     o = sys.stdout
     o.write("// Generated by gen_consts.py -- do not edit.\n#pragma once\n#include <cstdint>\n\n")
-    o.write(f"#define SIGOPS_LIMB_BITS {LIMB_BITS}\n#define SIGOPS_NLIMBS {NLIMBS}\n#define SIGOPS_WG {WG}\n\n")
+    o.write(f"#define SIGOPS_LIMB_BITS {LIMB_BITS}\n#define SIGOPS_NLIMBS {NLIMBS}\n#define SIGOPS_WG {WG}\n"
+            f"#define SIGOPS_ED_LAT_BITS {ED_LAT_BITS}\n\n")
     o.write("namespace sigops {\n")
     for tag, m in [("ModPk1", P_K1), ("ModNk1", N_K1), ("ModPr1", P_R1), ("ModNr1", N_R1), ("ModPed", P_ED),
                    ("ModL", L_ED)]:
@@ -467,7 +469,8 @@ def main():
             vals += limbs(pow(2, -nl * lb, p) * 2**522 % p)
         o.write(arr(f"REFCONV_{tag}", vals))
     o.write("\n// host-side generator tables: entries of 20 u32 (x[9], y[9], pad[2]) for secp (odd multiples 1..511),\n")
-    o.write("// 28 u32 (ypx[9], ymx[9], t2d[9], pad) for ed25519; Montgomery form\n")
+    o.write("// 28 u32 (ypx[9], ymx[9], t2d[9], pad) for ed25519 (odd multiples of B, then of [2^136] B);\n")
+    o.write("// Montgomery form\n")
     ntab = 1 << (WG - 1)
     # secp: odd multiples 1..511 of G, for fixed-base windows of 8 bits (the first 128) or 9
     for tag, G, p, a in [("K1", G_K1, P_K1, 0), ("R1", G_R1, P_R1, -3)]:
@@ -476,11 +479,20 @@ def main():
         for (x, y) in pts:
             vals += limbs(mont(x, p)) + limbs(mont(y, p)) + [0, 0]
         o.write(arr(f"{tag}_GTABLE", vals))
+    # Ed25519: odd multiples of B, then odd multiples of [2^ED_LAT_BITS] B (the high half of the
+    # lattice-reduced check's fixed-base scalar)
     B = ed_base()
-    B2 = e_add(B, B)
-    pts = [B]
-    for _ in range(ntab - 1):
-        pts.append(e_add(pts[-1], B2))
+    B_hi = (0, 1)
+    for bit in bin(1 << ED_LAT_BITS)[2:]:
+        B_hi = e_add(B_hi, B_hi)
+        if bit == "1":
+            B_hi = e_add(B_hi, B)
+    pts = []
+    for base in (B, B_hi):
+        base2 = e_add(base, base)
+        pts.append(base)
+        for _ in range(ntab - 1):
+            pts.append(e_add(pts[-1], base2))
     vals = []
     for (x, y) in pts:
         vals += limbs(mont((y + x) % P_ED, P_ED)) + limbs(mont((y - x) % P_ED, P_ED))

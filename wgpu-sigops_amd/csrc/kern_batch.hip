@@ -183,14 +183,17 @@ __global__ __launch_bounds__(256, WAVES) void k_batch_run(BatchArgs a, const uin
   static_assert(!((MASK & RUN_ED) && (MASK & RUN_ED_STRICT)), "one Ed25519 rule set per launch");
   constexpr bool K1 = MASK & RUN_K1, R1 = MASK & RUN_R1, ED = MASK & (RUN_ED | RUN_ED_STRICT);
   constexpr u32 GN = WGB == 9 ? GTAB9_ENTRIES : GTAB_ENTRIES;
+  // the kernels that serve only Ed25519 run the lattice-reduced check (both fixed-base tables)
+  constexpr bool LAT = ED && !K1 && !R1;
+  constexpr u32 BN = LAT ? BTAB_UINT4 : BTAB_ENTRIES * 7;
   bind_io(a);
   __shared__ uint4 s_k1[K1 ? GN * gtab_chunks<CurveK1>() : 1];
   __shared__ uint4 s_r1[R1 ? GN * gtab_chunks<CurveR1>() : 1];
-  __shared__ uint4 s_b[ED ? BTAB_ENTRIES * 7 : 1];
+  __shared__ uint4 s_b[ED ? BN : 1];
   if constexpr (K1) stage_gtab<CurveK1, GN>(s_k1, gk1);
   if constexpr (R1) stage_gtab<CurveR1, GN>(s_r1, gr1);
   if constexpr (ED)
-    for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
+    for (int t = threadIdx.x; t < BN; t += blockDim.x) s_b[t] = gb[t];
   __syncthreads();
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 lane = threadIdx.x & 63u;
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(256, WAVES) void k_batch_run(BatchArgs a, const uin
         load_words(a.sigs + 4 * (u64)i, sr, 4);
         load_words(a.aux + 2 * (u64)i, pr, 2);
         const u64 off = a.msg_off[i];
-        a.status[i] = (u8)ed25519_verify_item<(MASK & RUN_ED_STRICT) != 0>(
+        a.status[i] = (u8)ed25519_verify_item<(MASK & RUN_ED_STRICT) != 0, LAT>(
             sr, pr, MemDev{a.msgs}, off, (u32)(a.msg_off[i + 1] - off), ct, BTabLds{s_b});
         if (a.out) {
           const u32 zero[16] = {0};

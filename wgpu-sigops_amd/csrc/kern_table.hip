@@ -1,7 +1,8 @@
 // kern_table.hip -- converts a caller-supplied generator table in the reference's layout
 // (src/precompute.rs:12-69) into this engine's odd-multiple table, with the checks of table.hpp.
 // One workgroup of 128 lanes: lanes 0..15 check entry j, lane 0 shares entry 0 through LDS, then
-// lane j writes odd multiple 2j + 1. flags[0] = 0 for a table that passes (else TAB_BAD_* bits).
+// lane j writes odd multiple 2j + 1 (for Ed25519 also (2j + 1) [2^136] P0 in the second half).
+// flags[0] = 0 for a table that passes (else TAB_BAD_* bits).
 #include "kernels_impl.hpp"
 #include "table.hpp"
 
@@ -71,6 +72,11 @@ __global__ __launch_bounds__(128) void k_table_ed(const u32* __restrict__ limbs,
   tab_ed_build_entry(j, p0, x, y, bad, w);
 #pragma unroll
   for (int c = 0; c < 7; c++) btab_out[j * 7 + c] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
+  // the high-half table of the lattice-reduced check: (2j+1) [2^136] P0
+  tab_ed_build_hi_entry(j, p0, w);
+#pragma unroll
+  for (int c = 0; c < 7; c++)
+    btab_out[(BTAB_ENTRIES + j) * 7 + c] = make_uint4(w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]);
   if (bad) atomicOr(&s_bad, bad);
   __syncthreads();
   if (j == 0) flags[0] = s_bad;

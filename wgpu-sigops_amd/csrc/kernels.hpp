@@ -9,9 +9,12 @@ typedef uint8_t u8;
 
 constexpr uint32_t GTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of G
 constexpr uint32_t GTAB9_ENTRIES = 1u << (9 - 1);  // odd multiples 1..511 of G (9-bit windows; the HBM table's size)
-constexpr uint32_t BTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of B
+constexpr uint32_t BTAB_ENTRIES = 1u << (8 - 1);  // odd multiples 1..255 of B (and as many of [2^136] B)
+// the Ed25519 fixed-base tables as held in HBM and LDS: odd multiples of B, then of [2^136] B,
+// 7 uint4 per entry
+constexpr uint32_t BTAB_UINT4 = 2 * BTAB_ENTRIES * 7;
 constexpr uint32_t SECP_LANE_TAB_BYTES = 8 * 9 * 16;  // 8 entries x 9 uint4 (Jacobian + Z product, then affine)
-constexpr uint32_t ED_LANE_TAB_BYTES = 8 * 9 * 16;    // 8 cached entries x 9 uint4
+constexpr uint32_t ED_LANE_TAB_BYTES = 16 * 9 * 16;   // 16 cached entries (-A, then -R) x 9 uint4
 
 hipError_t launch_secp_recover_CurveK1(const void* sigs, const void* digs, void* out, void* status, uint32_t n,
                                       const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,

@@ -135,6 +135,23 @@ DEV void tab_ed_check_entry(u32 j, const u32* limbs, u32 nl, u32 lbits, fe& x, f
   if (!okx || !oky || !okt || !on || !feq<F>(t, mont_mul<F>(x, y))) bad |= TAB_BAD_FORMAT;
 }
 
+// a point in extended coordinates as a table entry: affine Niels form (y+x, y-x, 2d*x*y), canonical
+DEV void tab_ed_niels(const ext& Q, u32 out[28]) {
+  typedef ModPed F;
+  const fe zi = finv<F>(Q.Z);
+  const fe ax = mont_mul<F>(Q.X, zi), ay = mont_mul<F>(Q.Y, zi);
+  const fe ypx = fcanon<F>(fadd<F>(ay, ax));
+  const fe ymx = fcanon<F>(fsub<F>(ay, ax));
+  const fe t2d = fcanon<F>(mont_mul<F>(mont_mul<F>(ax, ay), fe_from_const(ED_D2)));
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    out[k] = ypx.v[k];
+    out[NL + k] = ymx.v[k];
+    out[2 * NL + k] = t2d.v[k];
+  }
+  out[27] = 0;
+}
+
 DEV ext tab_ed_mul_small(const cached& P0c, u32 k, int top) {
   ext acc = ext_identity();
 #pragma unroll 1
@@ -172,19 +189,25 @@ DEV void tab_ed_build_entry(u32 j, const u32 p0[2 * NL], const fe& x, const fe& 
     }
     if (!(fis_zero<F>(acc.X) && feq<F>(acc.Y, acc.Z))) bad |= TAB_BAD_ORDER;
   }
-  const ext Q = tab_ed_mul_small(P0c, 2 * j + 1, 7);
-  const fe zi = finv<F>(Q.Z);
-  const fe ax = mont_mul<F>(Q.X, zi), ay = mont_mul<F>(Q.Y, zi);
-  const fe ypx = fcanon<F>(fadd<F>(ay, ax));
-  const fe ymx = fcanon<F>(fsub<F>(ay, ax));
-  const fe t2d = fcanon<F>(mont_mul<F>(mont_mul<F>(ax, ay), fe_from_const(ED_D2)));
+  tab_ed_niels(tab_ed_mul_small(P0c, 2 * j + 1, 7), out);
+}
+
+// entry j of the high-half table: (2j+1) [2^136] P0 (P0 = (p0 x, p0 y), affine Montgomery form)
+DEV void tab_ed_build_hi_entry(u32 j, const u32 p0[2 * NL], u32 out[28]) {
+  typedef ModPed F;
+  ext P;
 #pragma unroll
   for (int k = 0; k < NL; k++) {
-    out[k] = ypx.v[k];
-    out[NL + k] = ymx.v[k];
-    out[2 * NL + k] = t2d.v[k];
+    P.X.v[k] = p0[k];
+    P.Y.v[k] = p0[NL + k];
   }
-  out[27] = 0;
+  BOUNDS_SET(P.X, LMASK, (1u << 24) - 1u, mod_ld<F>());
+  BOUNDS_SET(P.Y, LMASK, (1u << 24) - 1u, mod_ld<F>());
+  P.Z = fe_from_const(F::ONE);
+  P.T = fe_zero();  // doubling does not read T
+#pragma unroll 1
+  for (int b = 0; b < ED_LAT_BITS; b++) P = edbl(P, b == ED_LAT_BITS - 1);
+  tab_ed_niels(tab_ed_mul_small(to_cached(P), 2 * j + 1, 7), out);
 }
 
 }  // namespace sigops
