@@ -159,11 +159,13 @@ def test_inputs_above_max_batch_are_split():
 
 def test_context_refuses_unknown_variants():
     """sigops_create refuses a fixed-base window other than 8 or 9 bits, and 9-bit windows with the
-    one-wave kernel (that variant is not built); the valid combinations create contexts"""
+    one-wave kernel (that variant is not built); the valid combinations create contexts, and a
+    window left at 0 means 9 bits with two waves per SIMD, 8 with one"""
     for kw in ({"g_window": 7}, {"g_window": 10}, {"g_window": 9, "waves": 1}, {"waves": 3}, {"bucket_order": 3}):
         with pytest.raises(sigops.SigopsError):
             sigops.Context(tuned=False, max_batch=1024, **kw)
-    for kw in ({"g_window": 8}, {"g_window": 9}, {"g_window": 8, "waves": 1}):
+    for kw, want in (({"g_window": 8}, 8), ({"g_window": 9}, 9), ({"g_window": 8, "waves": 1}, 8), ({}, 9),
+                     ({"waves": 1}, 8), ({"waves": 2}, 9)):
         c = sigops.Context(tuned=False, max_batch=1024, **kw)
-        assert c.g_window == kw["g_window"]
+        assert c.g_window == want
         c.close()

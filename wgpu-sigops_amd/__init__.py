@@ -211,7 +211,7 @@ class Context:
         self.blocks_per_cu = blocks_per_cu or self.waves
         self.tile_rounds = tile_rounds or 1
         self.bucket_order = bucket_order
-        self.g_window = g_window or 8
+        self.g_window = g_window or (8 if self.waves == 1 else 9)
         cfg = _Config(device, blocks_per_cu, max_batch, tile_rounds, 0 if use_graph else 1, waves, bucket_order,
                       g_window)
         self._h = lib().sigops_create(ctypes.byref(cfg))

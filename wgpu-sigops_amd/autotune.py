@@ -37,12 +37,14 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED_PATH = os.path.join(HERE, "tuned.json")
-DEFAULT = {"waves": 2, "tile_rounds": 1, "bucket_order": 0, "g_window": 8}
-CANDIDATES = (DEFAULT, dict(DEFAULT, waves=1), dict(DEFAULT, tile_rounds=2), dict(DEFAULT, bucket_order=1),
-              dict(DEFAULT, bucket_order=2), dict(DEFAULT, g_window=9),
-              # the 9-bit windows with each other knob (the one-wave kernel is not built with them)
-              dict(DEFAULT, g_window=9, tile_rounds=2), dict(DEFAULT, g_window=9, bucket_order=1),
-              dict(DEFAULT, g_window=9, bucket_order=2))
+# the library's defaults (a Context with no settings)
+DEFAULT = {"waves": 2, "tile_rounds": 1, "bucket_order": 0, "g_window": 9}
+# each knob changed alone, and the 8-bit windows with each of them (the one-wave kernel is built
+# with 8-bit windows only)
+_W8 = dict(DEFAULT, g_window=8)
+CANDIDATES = (DEFAULT, dict(DEFAULT, waves=1, g_window=8), dict(DEFAULT, tile_rounds=2), dict(DEFAULT, bucket_order=1),
+              dict(DEFAULT, bucket_order=2), _W8, dict(_W8, tile_rounds=2), dict(_W8, bucket_order=1),
+              dict(_W8, bucket_order=2))
 
 
 def device_key(device=0):

@@ -350,7 +350,9 @@ sigops_ctx* sigops_create(const sigops_config* cfg) {
     return nullptr;
   }
   c->waves = cfg->waves_per_simd ? cfg->waves_per_simd : 2;
-  c->g_window = cfg->g_window ? cfg->g_window : 8;
+  // 9-bit windows measured faster on the MI355X (and chosen by the autotuner); the one-wave kernel
+  // is built with 8-bit ones only
+  c->g_window = cfg->g_window ? cfg->g_window : (c->waves == 1 ? 8 : 9);
   const uint32_t bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : c->waves;
   c->grid = (uint32_t)cus * bpc;
   c->slots = c->grid * 256;
