@@ -25,6 +25,15 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(scope="module", params=[8, 9], ids=["g8", "g9"])
+def ctxw(request):
+    """a context with either fixed-base window of the secp streams (the streamed kernel is built
+    for both)"""
+    c = sigops.Context(tuned=False, g_window=request.param)
+    yield c
+    c.close()
+
+
 def _mixed(n, seed, corrupt_ppm=100000, max_len=600, long_every=0):
     """host arrays of a mixed batch made by the oracle's generators (one third per scheme)"""
     rng = np.random.default_rng(seed)
@@ -73,11 +82,11 @@ def _both(ctx, *arrs):
     return a, b
 
 
-def test_streamed_mixed_matches_copy_path_and_oracle(ctx):
+def test_streamed_mixed_matches_copy_path_and_oracle(ctxw):
     n = 60000
-    _say(f"streamed mixed n={n}")
+    _say(f"streamed mixed n={n}, {ctxw.g_window}-bit G windows")
     arrs = _mixed(n, 3, long_every=97)
-    (so, ss), (co, cs) = _both(ctx, *arrs)
+    (so, ss), (co, cs) = _both(ctxw, *arrs)
     assert np.array_equal(so, co) and np.array_equal(ss, cs)
     out, st, ii, v = _expect(*arrs)
     secp = arrs[0] < 2
@@ -85,8 +94,9 @@ def test_streamed_mixed_matches_copy_path_and_oracle(ctx):
     assert np.array_equal(ss[ii] == 0, v) and not so[ii].any()
 
 
-def test_streamed_edge_cases(ctx):
+def test_streamed_edge_cases(ctxw):
     """every recovery and Ed25519 edge case, repeated to a streamed size among valid items"""
+    ctx = ctxw
     items = []
     for c in (0, 1):
         cases = edge_cases.secp_cases(c)
@@ -100,7 +110,7 @@ def test_streamed_edge_cases(ctx):
     order = rng.permutation(len(items) * reps)
     items = [items[k % len(items)] for k in order]
     n = len(items)
-    _say(f"streamed edge cases n={n}")
+    _say(f"streamed edge cases n={n}, {ctx.g_window}-bit G windows")
     sch = np.array([it[0] for it in items], np.uint8)
     sigs = np.frombuffer(b"".join(it[1] for it in items), np.uint8).reshape(n, 64)
     aux = np.frombuffer(b"".join(it[2] for it in items), np.uint8).reshape(n, 32)

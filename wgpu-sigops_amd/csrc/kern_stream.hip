@@ -184,16 +184,19 @@ DEV void stream_store_out(const StreamArgs& a, u32 t, u32 l, const u32 o[16]) {
 // messages into HBM with 1 KB-wide loads (the hash reads them word by word), and the results go
 // back to pinned memory, after which the tile's done word is set. Secp items whose fast ladder
 // meets an exceptional addition go to the pending list for k_stream_careful. The launch ends when
-// its tiles run out: the batch's last slot, or the count the host closed.
+// its tiles run out: the batch's last slot, or the count the host closed. WGB: the secp streams'
+// fixed-base window, 8 bits (128 odd multiples of G staged in LDS per curve) or 9 (256).
+template <int WGB>
 __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamArgs a, const uint4* __restrict__ gk1,
                                                                     const uint4* __restrict__ gr1,
                                                                     const uint4* __restrict__ gb,
                                                                     uint4* __restrict__ lane_tab, u32 lane_slots) {
-  __shared__ uint4 s_k1[GTAB_ENTRIES * gtab_chunks<CurveK1>()];
-  __shared__ uint4 s_r1[GTAB_ENTRIES * gtab_chunks<CurveR1>()];
+  constexpr u32 GN = WGB == 9 ? GTAB9_ENTRIES : GTAB_ENTRIES;
+  __shared__ uint4 s_k1[GN * gtab_chunks<CurveK1>()];
+  __shared__ uint4 s_r1[GN * gtab_chunks<CurveR1>()];
   __shared__ uint4 s_b[BTAB_ENTRIES * 7];
-  stage_gtab<CurveK1>(s_k1, gk1);
-  stage_gtab<CurveR1>(s_r1, gr1);
+  stage_gtab<CurveK1, GN>(s_k1, gk1);
+  stage_gtab<CurveR1, GN>(s_r1, gr1);
   for (int t = threadIdx.x; t < BTAB_ENTRIES * 7; t += blockDim.x) s_b[t] = gb[t];
   __syncthreads();
   const u32 lane = threadIdx.x & 63u;
@@ -260,8 +263,8 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
     } else if (lane < cnt) {
       u32 sr[16], ar[8], o[16];
       stream_load(a, p, lane, sr, ar);
-      const u32 st = b == BUCKET_K1 ? secp_item<CurveK1>(sr, ar, jt, GTabOf<CurveK1>{s_k1}, o)
-                                    : secp_item<CurveR1>(sr, ar, jt, GTabOf<CurveR1>{s_r1}, o);
+      const u32 st = b == BUCKET_K1 ? secp_item<CurveK1, WGB>(sr, ar, jt, GTabOf<CurveK1>{s_k1}, o)
+                                    : secp_item<CurveR1, WGB>(sr, ar, jt, GTabOf<CurveR1>{s_r1}, o);
       if (st == ST_PENDING) {
         const u32 k = atomicAdd(&a.ctr[STREAM_CTR_PENDING], 1u);
         if (k < a.pend_cap) a.pend_list[k] = p * 64 + lane;
@@ -336,8 +339,12 @@ hipError_t launch_stream(const StreamLaunch& L, hipStream_t s) {
   a.limit_ticks = L.limit_ticks;
   a.msg_cap = L.msg_cap;
   k_zero_words<<<1, 256, 0, s>>>(L.ctr, STREAM_CTR_BYTES / 4);
-  k_stream_run<<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
-                                      (uint4*)L.lane_tab, L.lane_slots);
+  if (L.g_window == 9)
+    k_stream_run<9><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+                                           (uint4*)L.lane_tab, L.lane_slots);
+  else
+    k_stream_run<8><<<L.grid, 256, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1, (const uint4*)L.gb,
+                                           (uint4*)L.lane_tab, L.lane_slots);
   k_stream_careful<<<64, 64, 0, s>>>(a, (const uint4*)L.gk1, (const uint4*)L.gr1);
   return hipGetLastError();
 }

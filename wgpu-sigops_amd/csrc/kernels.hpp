@@ -117,6 +117,7 @@ struct StreamLaunch {
   uint32_t slots;        // tiles a wavefront may take at most
   uint32_t base, mask;   // see qpos (a batch: 0 and ~0)
   uint32_t seq, lane_slots, grid;
+  uint32_t g_window;     // fixed-base window of the secp items, 8 or 9 bits
   uint64_t poll_ticks;   // wall-clock ticks between reads of the pinned count (device-wide)
   uint64_t limit_ticks;  // a wave that waits this long for a tile gives up
 };
@@ -138,6 +139,7 @@ struct RingResources {
   const void* gb;
   void* lane_tab;
   uint32_t lane_slots, grid;
+  uint32_t g_window;             // fixed-base window of the secp items, 8 or 9 bits
   const uint32_t* bucket_order;  // processing rank -> bucket, NBUCKETS entries
   hipStream_t stream;
 };

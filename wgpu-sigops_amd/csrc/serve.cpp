@@ -740,6 +740,7 @@ void RingEngine::publish_counts() {
   L.seq = seq;
   L.lane_slots = res.lane_slots;
   L.grid = res.grid;
+  L.g_window = res.g_window;
   sigops::stream_poll_ticks(S->cfg.device, L);
   const bool ok = e.ev && sigops::launch_stream(L, res.stream) == hipSuccess &&
                   hipEventRecord(e.ev, res.stream) == hipSuccess;

@@ -961,6 +961,7 @@ int batch_streamed(sigops_ctx* c, const uint8_t* schemes, const uint8_t* sigs, c
   L.seq = seq;
   L.lane_slots = c->slots;
   L.grid = std::min<uint32_t>(c->grid, (uint32_t)((g.slots + 3) / 4));
+  L.g_window = c->g_window;
   sigops::stream_poll_ticks(c->device, L);
   HIPCHK(c, hipEventRecord(c->s_tev[0], s));
   if (!(c->fault & SIGOPS_FAULT_SKIP_COMPUTE)) HIPCHK(c, sigops::launch_stream(L, s));
@@ -1029,6 +1030,7 @@ int sigops_ring_resources(sigops_ctx* c, sigops::RingResources* r) {
   r->lane_tab = c->d_lane_tab;
   r->lane_slots = c->slots;
   r->grid = c->grid;
+  r->g_window = c->g_window;
   r->bucket_order = c->bucket_order;
   r->stream = c->stream;
   return SIGOPS_OK;
