@@ -72,11 +72,11 @@ typedef struct {
   uint32_t bucket_order;    /* mixed batch: processing order of the buckets, 0 (default) secp256r1,
                                secp256k1, Ed25519 by decreasing SHA-512 block count; 1 secp256r1, Ed25519,
                                secp256k1; 2 secp256k1, secp256r1, Ed25519 */
-  uint32_t g_window;        /* mixed and streamed batches, serving ring: fixed-base window of the secp256k1 /
-                               secp256r1 G streams, 8 or 9 bits (256 odd multiples of G in LDS, 4 fewer
-                               additions per secp256k1 item and 3 per secp256r1 item); 9 needs
-                               waves_per_simd 2. 0: 9 with two waves per SIMD (measured faster on the
-                               MI355X), 8 with one */
+  uint32_t g_window;        /* every secp kernel: fixed-base window of the secp256k1 / secp256r1 G streams,
+                               8 or 9 bits (256 odd multiples of G in LDS, 4 fewer additions per
+                               secp256k1 item and 3 per secp256r1 item); 9 needs waves_per_simd 2; calls
+                               with a caller's generator table use 8. 0: 9 with two waves per SIMD
+                               (measured faster on the MI355X), 8 with one */
 } sigops_config;
 
 /* Creates a context on one GPU: uploads the fixed-base generator tables to HBM once (the

@@ -27,15 +27,23 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(scope="module", params=[8, 9], ids=["g8", "g9"])
+def ctxw(request):
+    """a context with either fixed-base window (the per-scheme secp kernels are built for both)"""
+    c = sigops.Context(tuned=False, g_window=request.param)
+    yield c
+    c.close()
+
+
 def _say(*a):
     print(*a, flush=True)
 
 
 @pytest.mark.parametrize("curve,n", [(0, 1024), (1, 8192)])
-def test_secp_recover_matches_oracle(ctx, curve, n):
-    _say(f"secp recover curve={curve} n={n}")
+def test_secp_recover_matches_oracle(ctxw, curve, n):
+    _say(f"secp recover curve={curve} n={n}, {ctxw.g_window}-bit G windows")
     sigs, dig, pubs, cor = O.gen_secp(curve, 1000 + curve, n, corrupt_ppm=100000)
-    fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
+    fn = ctxw.secp256k1_recover if curve == 0 else ctxw.secp256r1_recover
     out, st = fn(sigs, dig)
     ref_out, ref_st = O.recover_batch(curve, sigs, dig)
     assert np.array_equal(st, ref_st)
@@ -45,13 +53,13 @@ def test_secp_recover_matches_oracle(ctx, curve, n):
 
 
 @pytest.mark.parametrize("curve", [0, 1])
-def test_secp_edge_cases(ctx, curve):
+def test_secp_edge_cases(ctxw, curve):
     cases = edge_cases.secp_cases(curve)
     exp = edge_cases.secp_expected(curve, cases)
-    _say(f"secp edge cases curve={curve} n={len(cases)}")
+    _say(f"secp edge cases curve={curve} n={len(cases)}, {ctxw.g_window}-bit G windows")
     sigs = np.frombuffer(b"".join(c[1] for c in cases), np.uint8).reshape(-1, 64)
     digs = np.frombuffer(b"".join(c[2] for c in cases), np.uint8).reshape(-1, 32)
-    fn = ctx.secp256k1_recover if curve == 0 else ctx.secp256r1_recover
+    fn = ctxw.secp256k1_recover if curve == 0 else ctxw.secp256r1_recover
     out, st = fn(sigs, digs)
     for i, (name, _, _) in enumerate(cases):
         assert (int(st[i]), bytes(out[i])) == exp[i], name

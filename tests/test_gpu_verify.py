@@ -21,6 +21,14 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(scope="module", params=[8, 9], ids=["g8", "g9"])
+def ctxw(request):
+    """a context with either fixed-base window (the per-scheme secp kernels are built for both)"""
+    c = sigops.Context(tuned=False, g_window=request.param)
+    yield c
+    c.close()
+
+
 def _say(*a):
     print(*a, flush=True)
 
@@ -42,11 +50,11 @@ def _mutate(curve, sigs, pubs, seed):
 
 
 @pytest.mark.parametrize("curve,n", [(0, 4096), (1, 8192)])
-def test_secp_verify_matches_oracle(ctx, curve, n):
-    _say(f"secp verify curve={curve} n={n}")
+def test_secp_verify_matches_oracle(ctxw, curve, n):
+    _say(f"secp verify curve={curve} n={n}, {ctxw.g_window}-bit G windows")
     sigs, dig, pubs, cor = O.gen_secp(curve, 2000 + curve, n, corrupt_ppm=100000)
     sigs, pubs = _mutate(curve, sigs, pubs, 5 + curve)
-    fn = ctx.secp256k1_verify if curve == 0 else ctx.secp256r1_verify
+    fn = ctxw.secp256k1_verify if curve == 0 else ctxw.secp256r1_verify
     valid, st = fn(sigs, dig, pubs)
     ref = O.secp_verify_batch(curve, sigs, dig, pubs)
     assert np.array_equal(st, ref)
@@ -55,10 +63,11 @@ def test_secp_verify_matches_oracle(ctx, curve, n):
 
 
 @pytest.mark.parametrize("curve", [0, 1])
-def test_secp_verify_edge_cases(ctx, curve):
+def test_secp_verify_edge_cases(ctxw, curve):
+    ctx = ctxw
     cases = edge_cases.secp_verify_cases(curve)
     exp = edge_cases.secp_verify_expected(curve, cases)
-    _say(f"secp verify edge cases curve={curve} n={len(cases)}")
+    _say(f"secp verify edge cases curve={curve} n={len(cases)}, {ctx.g_window}-bit G windows")
     sigs = np.frombuffer(b"".join(c[1] for c in cases), np.uint8).reshape(-1, 64)
     digs = np.frombuffer(b"".join(c[2] for c in cases), np.uint8).reshape(-1, 32)
     pubs = np.frombuffer(b"".join(c[3] for c in cases), np.uint8).reshape(-1, 64)

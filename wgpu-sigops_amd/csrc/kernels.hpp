@@ -16,20 +16,22 @@ constexpr uint32_t BTAB_UINT4 = 2 * BTAB_ENTRIES * 7;
 constexpr uint32_t SECP_LANE_TAB_BYTES = 8 * 9 * 16;  // 8 entries x 9 uint4 (Jacobian + Z product, then affine)
 constexpr uint32_t ED_LANE_TAB_BYTES = 16 * 9 * 16;   // 16 cached entries (-A, then -R) x 9 uint4
 
+// g_window: the fixed-base window, 8 or 9 bits (9 needs a table of 256 odd multiples of G: the
+// resident one)
 hipError_t launch_secp_recover_CurveK1(const void* sigs, const void* digs, void* out, void* status, uint32_t n,
                                       const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
-                                      uint32_t* pend_count, uint32_t grid, hipStream_t s);
+                                      uint32_t* pend_count, uint32_t grid, uint32_t g_window, hipStream_t s);
 hipError_t launch_secp_recover_CurveR1(const void* sigs, const void* digs, void* out, void* status, uint32_t n,
                                       const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
-                                      uint32_t* pend_count, uint32_t grid, hipStream_t s);
+                                      uint32_t* pend_count, uint32_t grid, uint32_t g_window, hipStream_t s);
 
 // verification with a known public key (64 bytes X || Y per item); status only
 hipError_t launch_secp_verify_CurveK1(const void* sigs, const void* digs, const void* pubs, void* status, uint32_t n,
                                      const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
-                                     uint32_t* pend_count, uint32_t grid, hipStream_t s);
+                                     uint32_t* pend_count, uint32_t grid, uint32_t g_window, hipStream_t s);
 hipError_t launch_secp_verify_CurveR1(const void* sigs, const void* digs, const void* pubs, void* status, uint32_t n,
                                      const void* gtab, void* rtab, uint32_t slots, uint32_t* pend_list,
-                                     uint32_t* pend_count, uint32_t grid, hipStream_t s);
+                                     uint32_t* pend_count, uint32_t grid, uint32_t g_window, hipStream_t s);
 
 // ---- mixed batch ----
 constexpr uint32_t BUCKET_K1 = 0, BUCKET_R1 = 1, BUCKET_ED0 = 2;

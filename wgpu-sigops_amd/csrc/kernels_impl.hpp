@@ -232,15 +232,18 @@ DEV void load_words(const uint4* p, u32* w, int nq) {
 }
 
 // ---- kernels ----
-template <class C>
+// WGB: the fixed-base window, 8 bits (128 odd multiples of G staged in LDS) or 9 (256; the resident
+// table only, a caller's converted table holds 128)
+template <class C, int WGB = 8>
 __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_recover(const uint4* __restrict__ sigs,
                                                       const uint4* __restrict__ digs, uint4* __restrict__ out,
                                                       u8* __restrict__ status, u32 n,
                                                       const uint4* __restrict__ gtab, uint4* __restrict__ rtab,
                                                       u32 slots, u32* __restrict__ pend_list,
                                                       u32* __restrict__ pend_count) {
-  __shared__ uint4 sg[GTAB_ENTRIES * gtab_chunks<C>()];
-  stage_gtab<C>(sg, gtab);
+  constexpr u32 GN = WGB == 9 ? GTAB9_ENTRIES : GTAB_ENTRIES;
+  __shared__ uint4 sg[GN * gtab_chunks<C>()];
+  stage_gtab<C, GN>(sg, gtab);
   __syncthreads();
   const GTabOf<C> gt{sg};
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_recover(const
     u32 st = pro.status;
     if (st == ST_OK) {
       bool exc = false;
-      const jac Q = secp_ladder<C>(pro, tab, gt, exc);
+      const jac Q = secp_ladder<C, WGB>(pro, tab, gt, exc);
       if (exc) {
         st = ST_PENDING;
         pend_list[atomicAdd(pend_count, 1u)] = i;
@@ -308,14 +311,15 @@ __global__ __launch_bounds__(64) void k_secp_careful(const uint4* __restrict__ s
 
 // ECDSA verification with a known key (fuel Signature::verify): the recovery kernel with the key
 // in place of R and the projective x comparison in place of the affine output. Lanes that meet an
-// exceptional addition are finished by k_secp_verify_careful.
-template <class C>
+// exceptional addition are finished by k_secp_verify_careful. WGB as in k_secp_recover.
+template <class C, int WGB = 8>
 __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_verify(
     const uint4* __restrict__ sigs, const uint4* __restrict__ digs, const uint4* __restrict__ pubs,
     u8* __restrict__ status, u32 n, const uint4* __restrict__ gtab, uint4* __restrict__ rtab, u32 slots,
     u32* __restrict__ pend_list, u32* __restrict__ pend_count) {
-  __shared__ uint4 sg[GTAB_ENTRIES * gtab_chunks<C>()];
-  stage_gtab<C>(sg, gtab);
+  constexpr u32 GN = WGB == 9 ? GTAB9_ENTRIES : GTAB_ENTRIES;
+  __shared__ uint4 sg[GN * gtab_chunks<C>()];
+  stage_gtab<C, GN>(sg, gtab);
   __syncthreads();
   const GTabOf<C> gt{sg};
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_secp_verify(
     u32 st = pro.status;
     if (st == ST_OK) {
       bool exc = false;
-      const jac P = secp_ladder<C>(pro, tab, gt, exc);
+      const jac P = secp_ladder<C, WGB>(pro, tab, gt, exc);
       if (exc) {
         st = ST_PENDING;
         pend_list[atomicAdd(pend_count, 1u)] = i;

@@ -186,20 +186,22 @@ __global__ void k_fill_bytes(uint8_t* p, uint8_t v, size_t n) {
     p[i] = v;
 }
 
-// gtab: the fixed-base table to use (NULL: the resident table of G)
+// gtab: the fixed-base table to use (NULL: the resident table of G, with the context's window; a
+// caller's converted table holds 128 entries, so 8-bit windows)
 int secp_recover_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_digs, size_t n, void* d_out,
                      void* d_status, hipStream_t s, const void* gtab = nullptr) {
   if (n > c->max_batch) return fail(c, SIGOPS_E_ARG, "n exceeds max_batch");
   if (n == 0) return SIGOPS_OK;
   if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return SIGOPS_OK;
+  const uint32_t w = gtab ? 8u : c->g_window;
   if (curve == 0)
     HIPCHK(c, sigops::launch_secp_recover_CurveK1(d_sigs, d_digs, d_out, d_status, (uint32_t)n,
                                                  gtab ? gtab : c->d_gtab_k1, c->d_lane_tab, c->slots, c->d_pend + 1,
-                                                 c->d_pend, c->grid, s));
+                                                 c->d_pend, c->grid, w, s));
   else
     HIPCHK(c, sigops::launch_secp_recover_CurveR1(d_sigs, d_digs, d_out, d_status, (uint32_t)n,
                                                  gtab ? gtab : c->d_gtab_r1, c->d_lane_tab, c->slots, c->d_pend + 1,
-                                                 c->d_pend, c->grid, s));
+                                                 c->d_pend, c->grid, w, s));
   return SIGOPS_OK;
 }
 
@@ -275,10 +277,12 @@ int secp_verify_dev(sigops_ctx* c, int curve, const void* d_sigs, const void* d_
   if (c->fault & SIGOPS_FAULT_SKIP_COMPUTE) return SIGOPS_OK;
   if (curve == 0)
     HIPCHK(c, sigops::launch_secp_verify_CurveK1(d_sigs, d_digs, d_pubs, d_status, (uint32_t)n, c->d_gtab_k1,
-                                                c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+                                                c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid,
+                                                c->g_window, s));
   else
     HIPCHK(c, sigops::launch_secp_verify_CurveR1(d_sigs, d_digs, d_pubs, d_status, (uint32_t)n, c->d_gtab_r1,
-                                                c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid, s));
+                                                c->d_lane_tab, c->slots, c->d_pend + 1, c->d_pend, c->grid,
+                                                c->g_window, s));
   return SIGOPS_OK;
 }
 
