@@ -218,11 +218,14 @@ __global__ __launch_bounds__(256, SIGOPS_WAVES_PER_EU) void k_stream_run(StreamA
     const int q = stream_claim(a, home, t);
     if (q < 0) break;
     // the tile's position in the layout: the queue's position table, or base + t
-    const u32 p = a.qpos ? __hip_atomic_load(a.qpos + (u64)q * (a.mask + 1) + (t & a.mask), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_SYSTEM)
-                         : (a.base + t) & a.mask;
-    const uint4 d = a.desc[p];
-    const u32 info = __builtin_amdgcn_readfirstlane(d.x);
+    const u32 p = __builtin_amdgcn_readfirstlane(
+        a.qpos ? __hip_atomic_load(a.qpos + (u64)q * (a.mask + 1) + (t & a.mask), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM)
+               : (a.base + t) & a.mask);
+    const uint4 dv = a.desc[p];
+    const uint4 d = make_uint4(__builtin_amdgcn_readfirstlane(dv.x), __builtin_amdgcn_readfirstlane(dv.y),
+                               __builtin_amdgcn_readfirstlane(dv.z), __builtin_amdgcn_readfirstlane(dv.w));
+    const u32 info = d.x;
     const u32 cnt = info & 0xFFu, b = info >> 8;
     if (cnt == 0) continue;  // an unused position
     if (b >= BUCKET_ED0) {
